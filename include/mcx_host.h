@@ -1,0 +1,56 @@
+/*
+ * mcx_host.h - the simulation entry points of the MI355X build (C ABI).
+ *
+ * Drop-in for the reference's host runtime header (src/mcx_host.h:46-48,
+ * :168-174).  mcx_run_simulation keeps its signature and its contract: all
+ * inputs and outputs travel through the Config (exportfield, exportdetected,
+ * seeddata, detectedcount, energytot/energyesc/energyabs, normalizer,
+ * runtime, his); the two trailing pointers are unused, as in the reference.
+ *
+ * mcx_list_gpu keeps its name, its GPUInfo output and its activedev count but
+ * drops the OpenCL handle types: it returns the number of HIP devices and
+ * fills activedevlist with HIP device ordinals (documented ABI change; the
+ * reference returns a cl_platform_id and fills cl_device_id handles).
+ */
+#ifndef MCX_AMD_HOST_H
+#define MCX_AMD_HOST_H
+
+#include "mcx_utils.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* replaces void mcx_run_simulation(Config*, float*, float*) (src/mcx_host.h:168) */
+void mcx_run_simulation(Config* cfg, float* fluence, float* totalenergy);
+
+/* replaces cl_platform_id mcx_list_gpu(Config*, unsigned int*, cl_device_id*, GPUInfo**)
+   (src/mcx_host.h:169); returns the number of usable devices */
+int mcx_list_gpu(Config* cfg, unsigned int* activedev, int* activedevlist, GPUInfo** info);
+
+/* non-exiting variant of mcx_run_simulation for language bindings: returns 0 on
+   success, the error id otherwise (message in mcx_last_error()) */
+int mcx_try_run_simulation(Config* cfg);
+
+/* number of HIP devices visible to this process (0 when none) */
+int mcx_device_count(void);
+
+/* first n words of the seeding generator after seeding with seed: the
+   glibc srand()/rand() sequence the reference uses to seed its per-thread
+   RNG streams (src/mcx_host.cpp:696-700, :759-768) */
+void mcx_host_seed_words(unsigned int seed, size_t n, unsigned int* out);
+
+/* self-checks of the device arithmetic, for the test suite: evaluate the
+   kernel's single-precision math (which: 0 exp, 1 log, 2 sin, 3 cos, 4 acos,
+   5 rsqrt, 6 divide x[i]/x[i+n], 7 sqrt) in parity mode (mode 1) or fast mode
+   (mode 0) on device `gpuid`; returns 0 on success */
+int mcx_device_fpmath(int gpuid, int which, int mode, const float* x, float* y, size_t n);
+
+/* first n floats of the device xorshift128+ stream seeded with seed4[4] */
+int mcx_device_rng_stream(int gpuid, const unsigned int seed4[4], unsigned int n, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,63 @@
+/*
+ * mcx_core.h - shared declarations between the gfx950 photon-transport kernels
+ * (mcx_core_*.hip) and the host runtime (mcx_host.hip).
+ */
+#ifndef MCX_CORE_H
+#define MCX_CORE_H
+
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+/* Kernel parameter block, passed by value (lives in the kernarg segment, read
+   with scalar loads).  Field meanings follow the reference's MCXParam
+   (src/mcx_host.h:106-166), restricted to what this kernel reads. */
+struct MCXKParam {
+    float4 srcpos, srcdir, srcparam1, srcparam2;
+    float maxidx_x, maxidx_y, maxidx_z;
+    uint32_t dimx, dimy, dimz, dimxy, nvox;
+    float twin0, twin1, oneoverc0, Rtstep, minenergy, minaccumtime;
+    uint32_t maxdetphoton, maxmedia, medianum, detnum, voidtime, srctype, maxvoidstep;
+    uint32_t issaveseed, issaveref, isspecular, maxgate, outputtype, threadphoton, savedetflag, reclen;
+    uint32_t partialdata, w0offset, gscatter, is2d, srcnum, doreflect, save2pt, savedet, nphase, extrasrclen;
+    uint32_t nthread, debugrng, mediaformat;
+    int32_t seed, oddphoton, replaydet, srcid;
+    uint32_t fieldlen;
+    unsigned char bc[12];
+
+    const uint32_t* media;        /* label volume, x fastest; bit 31 marks detector-adjacent voxels */
+    const float4* prop;           /* {mua, mus, g, n} per label */
+    const float4* detpos;         /* {x, y, z, r} per detector */
+    const float* invcdf;          /* sampled phase-function inverse CDF (nphase) */
+    const float4* srcdata;        /* extra sources, 4 float4 each */
+    const float* replayweight;
+    const float* replaytof;
+    const int* replaydetid;
+    const uint64_t* replayseed;   /* 2 words per replayed photon */
+    const uint32_t* nseed;        /* 4 words per thread */
+    double* field;                /* fluence accumulator, fieldlen entries */
+    float* genergy;               /* {escaped, launched} per thread */
+    float* detrec;                /* detected-photon records, reclen floats each */
+    uint64_t* detseed;            /* 2 words per detected photon */
+    uint32_t* detcount;
+};
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* launch the transport kernel; variant flags select template instances */
+hipError_t mcx_launch_parity(const MCXKParam* prm, int nblock, int nthreadblock, size_t shmem, hipStream_t stream,
+                             int savedet, int reflect);
+hipError_t mcx_launch_fast(const MCXKParam* prm, int nblock, int nthreadblock, size_t shmem, hipStream_t stream,
+                           int savedet, int reflect);
+/* RNG stream dump: field[i] for i = idx; i < fieldlen; i += nthread */
+hipError_t mcx_launch_rngdump(const MCXKParam* prm, int nblock, int nthreadblock, hipStream_t stream);
+/* out[i] = (float)(in[i]) * scale for the normalised float output */
+hipError_t mcx_launch_finalize(const double* in, float* out, size_t n, float scale, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

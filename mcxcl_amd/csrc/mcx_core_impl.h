@@ -1,0 +1,1091 @@
+/*
+ * mcx_core_impl.h - gfx950 photon-transport kernel body.
+ *
+ * Included by mcx_core_parity.hip (MCX_PARITY=1, built with
+ * -ffp-contract=off: deterministic FP32 math from mcx_fpmath.h, IEEE divide
+ * and sqrt, bit-identical to the CPU oracle) and by mcx_core_fast.hip
+ * (MCX_PARITY=0: gfx950 hardware transcendentals v_exp/v_log/v_sin/v_cos/
+ * v_rsq/v_rcp, the analogue of the reference's native_* build).
+ *
+ * Semantics: one persistent lane per reference thread, running its
+ * threadphoton(+1) photons back to back on a per-lane xorshift128+ stream
+ * (reference mcx_main_loop, src/mcx_core.cl:2307-3307, and launchnewphoton,
+ * :1466-2275).  The RNG draws, their order and every single-precision
+ * operation on the photon state match the reference's OpenCL branches.
+ *
+ * MI355X-specific structure:
+ *  - one relaunch site: the reference inlines launchnewphoton at six sites;
+ *    here every termination sets a flag and the loop head relaunches, so the
+ *    large launch body is emitted once and lanes reconverge there;
+ *  - the medium table and the phase-function inverse CDF are staged in LDS;
+ *    the per-lane partial-path/energy scratch lives in LDS (wave64 stride);
+ *  - fluence is accumulated with native global_atomic_add_f64 into a double
+ *    grid (no MAX_ACCUM shadow half is needed), converted and normalised by a
+ *    separate streaming kernel;
+ *  - detected photons are compacted with one atomic per record.
+ */
+#include <float.h>
+
+#include "mcx_core.h"
+#include "mcx_fpmath.h"
+
+#ifndef MCX_PARITY
+#error "MCX_PARITY must be defined"
+#endif
+
+#define K_EPS          FLT_EPSILON
+#define K_TWO_PI       6.28318530717959f
+#define K_ONE_PI       3.1415926535897932f
+#define K_JUST_BELOW_ONE 0.9998f
+#define K_ROULETTE     10.f
+#define K_DET_MASK     0x80000000u
+#define K_MED_MASK     0x7FFFFFFFu
+#define K_OUT_MIN      0xFFFFFFFFu
+#define K_OUT_MAX      0x7FFFFFFFu
+#define K_SEED_FROM_FILE (-999)
+
+enum {kReflect = 1, kAbsorb = 2, kMirror = 3, kCyclic = 4};
+enum {kFlux = 0, kFluence = 1, kEnergy = 2, kJacobian = 3, kWP = 4, kDCS = 5, kL = 7};
+enum {kPencil = 0, kIsotropic = 1, kCone = 2, kGaussian = 3, kPlanar = 4, kFourier = 6, kArcsine = 7, kDisk = 8,
+      kZGaussian = 11, kLine = 12, kSlit = 13, kPencilArray = 14, kRing = 17
+     };
+
+namespace mcxk {
+
+#if MCX_PARITY
+__device__ __forceinline__ float kexp(float x) {
+    return pm_exp(x);
+}
+__device__ __forceinline__ float klog(float x) {
+    return pm_log(x);
+}
+__device__ __forceinline__ void ksincos(float x, float* s, float* c) {
+    pm_sincos(x, s, c);
+}
+__device__ __forceinline__ float ksin(float x) {
+    return pm_sin(x);
+}
+__device__ __forceinline__ float kcos(float x) {
+    return pm_cos(x);
+}
+__device__ __forceinline__ float kacos(float x) {
+    return pm_acos(x);
+}
+__device__ __forceinline__ float ksqrt(float x) {
+    return __builtin_sqrtf(x);
+}
+__device__ __forceinline__ float krsqrt(float x) {
+    return 1.f / __builtin_sqrtf(x);
+}
+__device__ __forceinline__ float kdiv(float a, float b) {
+    return a / b;
+}
+#else
+__device__ __forceinline__ float kexp(float x) {
+    return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+}
+__device__ __forceinline__ float klog(float x) {
+    return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+}
+__device__ __forceinline__ void ksincos(float x, float* s, float* c) {
+    float r = x * 0.159154943091895336f;   /* v_sin/v_cos take revolutions */
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
+}
+__device__ __forceinline__ float ksin(float x) {
+    return __builtin_amdgcn_sinf(x * 0.159154943091895336f);
+}
+__device__ __forceinline__ float kcos(float x) {
+    return __builtin_amdgcn_cosf(x * 0.159154943091895336f);
+}
+__device__ __forceinline__ float ksqrt(float x) {
+    return __builtin_amdgcn_sqrtf(x);
+}
+__device__ __forceinline__ float krsqrt(float x) {
+    return __builtin_amdgcn_rsqf(x);
+}
+__device__ __forceinline__ float kdiv(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+}
+__device__ __forceinline__ float kacos(float x) {
+    /* same Cephes kernel as parity mode, with the hardware square root */
+    const float PI = 3.14159265358979f, PIO2 = 1.57079632679490f;
+
+    if (x > 0.5f) {
+        return 2.f * pm_asin_kernel(__builtin_amdgcn_sqrtf(0.5f * (1.f - x)));
+    }
+
+    if (x < -0.5f) {
+        return PI - 2.f * pm_asin_kernel(__builtin_amdgcn_sqrtf(0.5f * (1.f + x)));
+    }
+
+    return PIO2 - pm_asin_kernel(x);
+}
+#endif
+
+/* ---------------- RNG: xorshift128+ (src/mcx_core.cl:684-698) ---------------- */
+struct Rng {
+    uint64_t s0, s1;
+    __device__ __forceinline__ float next() {
+        uint64_t x = s0;
+        const uint64_t y = s1;
+        s0 = y;
+        x ^= x << 23;
+        s1 = x ^ y ^ (x >> 18) ^ (y >> 5);
+        uint32_t lo = (uint32_t)(s1 + y);
+        return __uint_as_float(0x3F800000u | (lo >> 9)) - 1.0f;
+    }
+    __device__ __forceinline__ float scatlen() {
+        return -klog(next() + K_EPS);
+    }
+};
+
+__device__ __forceinline__ int to_short(int v) {
+    return (int)(short)v;
+}
+
+__device__ __forceinline__ bool inside(const MCXKParam& g, int x, int y, int z) {
+    return (uint32_t)(x & 0xFFFF) < g.dimx && (uint32_t)(y & 0xFFFF) < g.dimy && (uint32_t)(z & 0xFFFF) < g.dimz;
+}
+
+__device__ __forceinline__ float nextafter_ref(float a, int dir) {
+    uint32_t u = __float_as_uint(a + 1000.f);
+    u += (uint32_t)(dir ^ (int)(u & 0x80000000u));
+    return __uint_as_float(u) - 1000.f;
+}
+
+/* distance to the next voxel face and the face id (src/mcx_core.cl:975-995) */
+__device__ __forceinline__ float hitgrid(const float4& p, const float4& v, int ix, int iy, int iz, int* face) {
+    float hx = fabsf((float)ix + (float)(v.x > 0.f) - p.x);
+    float hy = fabsf((float)iy + (float)(v.y > 0.f) - p.y);
+    float hz = fabsf((float)iz + (float)(v.z > 0.f) - p.z);
+    hx = fabsf(kdiv(hx + K_EPS, v.x));
+    hy = fabsf(kdiv(hy + K_EPS, v.y));
+    hz = fabsf(kdiv(hz + K_EPS, v.z));
+    float dist = fminf(fminf(hx, hy), hz);
+    *face = (dist == hx ? 0 : (dist == hy ? 1 : 2));
+    return dist;
+}
+
+/* src/mcx_core.cl:1025-1042, including the three sequential renormalisations */
+__device__ __forceinline__ void rotatevector(float4& v, float stheta, float ctheta, float sphi, float cphi) {
+    if (v.z > -1.f + K_EPS && v.z < 1.f - K_EPS) {
+        float tmp0 = 1.f - v.z * v.z;
+        float tmp1 = stheta * krsqrt(tmp0);
+        float nx = tmp1 * (v.x * v.z * cphi - v.y * sphi) + v.x * ctheta;
+        float ny = tmp1 * (v.y * v.z * cphi + v.x * sphi) + v.y * ctheta;
+        float nz = -tmp1 * tmp0 * cphi + v.z * ctheta;
+        v.x = nx;
+        v.y = ny;
+        v.z = nz;
+    } else {
+        float nx = stheta * cphi, ny = stheta * sphi, nz = (v.z > 0.f) ? ctheta : -ctheta;
+        v.x = nx;
+        v.y = ny;
+        v.z = nz;
+    }
+
+    v.x *= krsqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    v.y *= krsqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    v.z *= krsqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+}
+
+__device__ __forceinline__ void rotatevector2d(float4& v, float stheta, float ctheta, int is2d) {
+    float nx = v.x, ny = v.y, nz = v.z;
+
+    if (is2d == 1) {
+        nx = 0.f;
+        ny = v.y * ctheta - v.z * stheta;
+        nz = v.y * stheta + v.z * ctheta;
+    } else if (is2d == 2) {
+        nx = v.x * ctheta - v.z * stheta;
+        ny = 0.f;
+        nz = v.x * stheta + v.z * ctheta;
+    } else if (is2d == 3) {
+        nx = v.x * ctheta - v.y * stheta;
+        ny = v.x * stheta + v.y * ctheta;
+        nz = 0.f;
+    }
+
+    v.x = nx;
+    v.y = ny;
+    v.z = nz;
+    float tmp0 = krsqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    v.x *= tmp0;
+    v.y *= tmp0;
+    v.z *= tmp0;
+}
+
+__device__ __forceinline__ void transmit(float4& v, float n1, float n2, int flipdir) {
+    float tmp0 = kdiv(n1, n2);
+    v.x *= tmp0;
+    v.y *= tmp0;
+    v.z *= tmp0;
+
+    if (flipdir == 0) {
+        v.x = ksqrt(1.f - v.y * v.y - v.z * v.z) * (float)((v.x > 0.f) - (v.x < 0.f));
+    } else if (flipdir == 1) {
+        v.y = ksqrt(1.f - v.x * v.x - v.z * v.z) * (float)((v.y > 0.f) - (v.y < 0.f));
+    } else {
+        v.z = ksqrt(1.f - v.x * v.x - v.y * v.y) * (float)((v.z > 0.f) - (v.z < 0.f));
+    }
+}
+
+__device__ __forceinline__ float reflectcoeff(const float4& v, float n1, float n2, int flipdir) {
+    float Icos = fabsf((flipdir == 0) ? v.x : (flipdir == 1 ? v.y : v.z));
+    float tmp0 = n1 * n1;
+    float tmp1 = n2 * n2;
+    float tmp2 = 1.f - kdiv(tmp0, tmp1) * (1.f - Icos * Icos);
+
+    if (tmp2 > 0.f) {
+        float Re = tmp0 * Icos * Icos + tmp1 * tmp2;
+        tmp2 = ksqrt(tmp2);
+        float Im = 2.f * n1 * n2 * Icos * tmp2;
+        float Rtotal = kdiv(Re - Im, Re + Im);
+        Re = tmp1 * Icos * Icos + tmp0 * tmp2 * tmp2;
+        return (Rtotal + kdiv(Re - Im, Re + Im)) * 0.5f;
+    }
+
+    return 1.f;
+}
+
+/* per-lane photon state */
+struct Photon {
+    float4 p, v, f, prop;
+    int fx, fy, fz, fw;          /* voxel index (short semantics) and hit face */
+    uint32_t idx1d, mediaid;
+    float w0, Lmove;
+    uint64_t seed0, seed1;       /* RNG state at launch (for seed saving) */
+};
+
+struct Shared {
+    float4* prop;                /* medium table */
+    float* invcdf;
+    float* pp;                   /* this lane's scratch: partial data, energies, w0, srcid */
+};
+
+__device__ __forceinline__ float4 getprop(const Shared& sh, uint32_t mediaid) {
+    return sh.prop[mediaid & K_MED_MASK];
+}
+
+/* src/mcx_core.cl:1350-1455 */
+__device__ int skipvoid(const MCXKParam& g, Photon& s) {
+    int count = 1, idx1d;
+    float4& p = s.p;
+    float4& v = s.v;
+    s.fx = to_short((int)floorf(p.x));
+    s.fy = to_short((int)floorf(p.y));
+    s.fz = to_short((int)floorf(p.z));
+    s.fw = -1;
+
+    while (1) {
+        if (inside(g, s.fx, s.fy, s.fz)) {
+            idx1d = s.fz * (int)g.dimxy + s.fy * (int)g.dimx + s.fx;
+
+            if (g.media[idx1d] & K_MED_MASK) {
+                p.x -= v.x;
+                p.y -= v.y;
+                p.z -= v.z;
+                s.fx = to_short((int)floorf(p.x));
+                s.fy = to_short((int)floorf(p.y));
+                s.fz = to_short((int)floorf(p.z));
+                s.f.y -= g.minaccumtime;
+                idx1d = s.fz * (int)g.dimxy + s.fy * (int)g.dimx + s.fx;
+                count = 0;
+
+                while (!inside(g, s.fx, s.fy, s.fz) || !(g.media[idx1d] & K_MED_MASK)) {
+                    int face;
+                    float dist = hitgrid(p, v, s.fx, s.fy, s.fz, &face);
+                    s.fw = face;
+                    s.f.y += g.minaccumtime * dist;
+                    p.x = p.x + dist * v.x;
+                    p.y = p.y + dist * v.y;
+                    p.z = p.z + dist * v.z;
+
+                    if (face == 0) {
+                        s.fx += (v.x > 0.f ? 1 : -1);
+                    }
+
+                    if (face == 1) {
+                        s.fy += (v.y > 0.f ? 1 : -1);
+                    }
+
+                    if (face == 2) {
+                        s.fz += (v.z > 0.f ? 1 : -1);
+                    }
+
+                    idx1d = s.fz * (int)g.dimxy + s.fy * (int)g.dimx + s.fx;
+
+                    if (count++ > 3) {
+                        break;
+                    }
+                }
+
+                s.f.y = g.voidtime ? s.f.y : 0.f;
+
+                if (!inside(g, s.fx, s.fy, s.fz)) {
+                    return -1;   /* entry search failed: the reference would read outside the volume */
+                }
+
+                float nin = g.prop[g.media[idx1d] & K_MED_MASK].w;
+
+                if (g.isspecular && nin != g.prop[0].w) {
+                    p.w *= 1.f - reflectcoeff(v, g.prop[0].w, nin, s.fw);
+
+                    if (p.w > K_EPS) {
+                        transmit(v, g.prop[0].w, nin, s.fw);
+                    }
+                }
+
+                return idx1d;
+            }
+        }
+
+        if ((p.x < 0.f && v.x <= 0.f) || (p.x >= g.maxidx_x && v.x >= 0.f) || (p.y < 0.f && v.y <= 0.f) ||
+                (p.y >= g.maxidx_y && v.y >= 0.f) || (p.z < 0.f && v.z <= 0.f) || (p.z >= g.maxidx_z && v.z >= 0.f)) {
+            return -1;
+        }
+
+        p.x = p.x + v.x;
+        p.y = p.y + v.y;
+        p.z = p.z + v.z;
+        s.fx = to_short((int)floorf(p.x));
+        s.fy = to_short((int)floorf(p.y));
+        s.fz = to_short((int)floorf(p.z));
+        s.f.y += g.minaccumtime;
+
+        if ((uint32_t)count++ > g.maxvoidstep) {
+            return -1;
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t photon_index(const MCXKParam& g, int idx, float fw) {
+    int odd = g.oddphoton;
+    return (uint32_t)(idx * (int)g.threadphoton + (idx < odd ? idx : odd) + (int)fw);
+}
+
+__device__ __forceinline__ int gate_of(const MCXKParam& g, float t) {
+    int gate = (int)floorf((t - g.twin0) * g.Rtstep);
+    return gate > (int)g.maxgate - 1 ? (int)g.maxgate - 1 : gate;
+}
+
+__device__ __forceinline__ void deposit(const MCXKParam& g, uint32_t addr, float val) {
+    atomicAdd(g.field + addr, (double)val);
+}
+
+/* src/mcx_core.cl:838-926 */
+template <bool SAVEDET>
+__device__ void save_detected(const MCXKParam& g, const Shared& sh, const Photon& s, uint32_t isdet) {
+    int detid = 0;
+
+    if (isdet == K_OUT_MIN) {
+        detid = -1;
+    } else {
+        for (uint32_t i = 0; i < g.detnum; i++) {
+            float4 d = g.detpos[i];
+
+            if ((d.x - s.p.x) * (d.x - s.p.x) + (d.y - s.p.y) * (d.y - s.p.y) + (d.z - s.p.z) * (d.z - s.p.z) <
+                    d.w * d.w) {
+                detid = (int)i + 1;
+                break;
+            }
+        }
+    }
+
+    if (!detid) {
+        return;
+    }
+
+    uint32_t slot = atomicAdd(g.detcount, 1u);
+
+    if (slot >= g.maxdetphoton) {
+        return;
+    }
+
+    if (g.issaveseed) {
+        g.detseed[(size_t)slot * 2] = s.seed0;
+        g.detseed[(size_t)slot * 2 + 1] = s.seed1;
+    }
+
+    float* rec = g.detrec + (size_t)slot * g.reclen;
+    uint32_t k = 0;
+
+    if (g.savedetflag & 0x1) {
+        if (g.extrasrclen && g.srcid <= 0) {
+            detid |= ((int)sh.pp[g.w0offset - 1]) << 16;
+        }
+
+        rec[k++] = (float)detid;
+    }
+
+    for (uint32_t i = 0; i < g.partialdata; i++) {
+        rec[k++] = sh.pp[i];
+    }
+
+    if ((g.savedetflag >> 4) & 1) {
+        rec[k++] = s.p.x;
+        rec[k++] = s.p.y;
+        rec[k++] = s.p.z;
+    }
+
+    if ((g.savedetflag >> 5) & 1) {
+        rec[k++] = s.v.x;
+        rec[k++] = s.v.y;
+        rec[k++] = s.v.z;
+    }
+
+    if ((g.savedetflag >> 6) & 1) {
+        rec[k++] = sh.pp[g.w0offset - 2];
+    }
+}
+
+/* terminate the current photon and launch the next one (src/mcx_core.cl:1466-2275).
+   Returns 0 on a launch, 1 when the lane's photon budget is used up, -1 when
+   no launch succeeded within maxvoidstep attempts. */
+template <bool SAVEDET, bool PENCIL>
+__device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rng& rng, int idx, uint32_t isdet) {
+    float* pp = sh.pp;
+    const uint32_t pd = g.partialdata;
+    float4& p = s.p;
+    float4& v = s.v;
+    float4& f = s.f;
+    s.w0 = 1.f;
+    s.Lmove = -1.f;
+
+    if (fabsf(p.w) >= 0.f) {
+        pp[pd] += p.w;
+
+        if (g.issaveref == 1 && s.mediaid == 0 && s.idx1d != K_OUT_MIN && s.idx1d != K_OUT_MAX && p.w > 0.f) {
+            int tshift = (int)g.maxgate - 1;
+            int t2 = (int)floorf((f.y - g.twin0) * g.Rtstep);
+            tshift = tshift < t2 ? tshift : t2;
+
+            if (g.extrasrclen && g.srcid < 0) {
+                tshift += ((int)pp[g.w0offset - 1] - 1) * (int)g.maxgate;
+            }
+
+            if (t2 >= 0) {
+                deposit(g, s.idx1d + (uint32_t)tshift * g.nvox, -p.w);
+            }
+        }
+
+        if (SAVEDET && (isdet & K_DET_MASK) == K_DET_MASK && s.mediaid == 0 && g.issaveref < 2) {
+            save_detected<SAVEDET>(g, sh, s, isdet);
+        }
+    }
+
+    if (SAVEDET) {
+        for (uint32_t i = 0; i < pd; i++) {
+            pp[i] = 0.f;
+        }
+    }
+
+    if (f.w >= (float)(g.threadphoton + (idx < g.oddphoton))) {
+        return 1;
+    }
+
+    if (g.seed == K_SEED_FROM_FILE) {
+        uint32_t pid = photon_index(g, idx, f.w < 0.f ? 0.f : f.w);
+        rng.s0 = g.replayseed[(size_t)pid * 2];
+        rng.s1 = g.replayseed[(size_t)pid * 2 + 1];
+    }
+
+    s.seed0 = rng.s0;
+    s.seed1 = rng.s1;
+    float4 lpos = g.srcpos, ldir = g.srcdir, lp1 = g.srcparam1, lp2 = g.srcparam2;
+
+    if (g.extrasrclen && g.srcid != 1) {
+        int sid;
+
+        if (g.srcid > 1) {
+            sid = g.srcid;
+        } else {
+            pp[g.w0offset - 1] = (float)((int)(rng.next() * K_JUST_BELOW_ONE * (float)(g.extrasrclen + 1)) + 1);
+            sid = (int)pp[g.w0offset - 1];
+        }
+
+        if (sid > 1) {
+            const float4* es = g.srcdata + (sid - 2) * 4;
+            lpos = es[0];
+            ldir = es[1];
+            lp1 = es[2];
+            lp2 = es[3];
+        }
+    }
+
+    if (g.seed == K_SEED_FROM_FILE && g.srcid >= 1) {
+        rng.next();
+    }
+
+    float* ppl = pp + pd;
+
+    do {
+        p = lpos;
+        v = ldir;
+        f.x = 0.f;
+        f.y = 0.f;
+        f.z = g.minaccumtime;
+        s.idx1d = __float_as_uint(lp2.z);
+        s.mediaid = __float_as_uint(lp2.w);
+        s.prop = make_float4(lpos.x, lpos.y, lpos.z, 0.f);
+
+        if (!PENCIL) {
+            const int st = (int)g.srctype;
+            bool positioned = false;
+
+            if (st == kPlanar || st == kFourier || st == kPencilArray) {
+                float rx = rng.next();
+                float ry = rng.next();
+                p.x = p.x + rx * lp1.x + ry * lp2.x;
+                p.y = p.y + rx * lp1.y + ry * lp2.y;
+                p.z = p.z + rx * lp1.z + ry * lp2.z;
+
+                if (st == kFourier) {
+                    p.w = lpos.w * (kcos((floorf(lp1.w) * rx + floorf(lp2.w) * ry + lp1.w - floorf(lp1.w)) * K_TWO_PI) *
+                                    (1.f - lp2.w + floorf(lp2.w)) + 1.f) * 0.5f;
+                } else if (st == kPencilArray) {
+                    p.x = lpos.x + kdiv(floorf(rx * lp1.w) * lp1.x, lp1.w - 1.f) + kdiv(floorf(ry * lp2.w) * lp2.x, lp2.w - 1.f);
+                    p.y = lpos.y + kdiv(floorf(rx * lp1.w) * lp1.y, lp1.w - 1.f) + kdiv(floorf(ry * lp2.w) * lp2.y, lp2.w - 1.f);
+                    p.z = lpos.z + kdiv(floorf(rx * lp1.w) * lp1.z, lp1.w - 1.f) + kdiv(floorf(ry * lp2.w) * lp2.z, lp2.w - 1.f);
+                }
+
+                positioned = true;
+                s.prop.x = s.prop.x + (lp1.x + lp2.x) * 0.5f;
+                s.prop.y = s.prop.y + (lp1.y + lp2.y) * 0.5f;
+                s.prop.z = s.prop.z + (lp1.z + lp2.z) * 0.5f;
+                s.prop.w = 0.f;
+            } else if (st == kDisk || st == kGaussian || st == kRing) {
+                float sphi, cphi, phi, r;
+
+                if ((st == kDisk || st == kRing) && (lp1.z > 0.f || lp1.w > 0.f)) {
+                    phi = fabsf(lp1.z - lp1.w) * rng.next() + fminf(lp1.z, lp1.w);
+                } else {
+                    phi = K_TWO_PI * rng.next();
+                }
+
+                ksincos(phi, &sphi, &cphi);
+
+                if (st == kDisk || st == kRing) {
+                    r = ksqrt(rng.next() * fabsf(lp1.x * lp1.x - lp1.y * lp1.y) + lp1.y * lp1.y);
+                } else if (fabsf(ldir.w) < 1e-5f || fabsf(lp1.y) < 1e-5f) {
+                    r = ksqrt(-0.5f * klog(rng.next())) * lp1.x;
+                } else {
+                    r = kdiv(lp1.x * lp1.x * K_ONE_PI, lp1.y);
+                    r = ksqrt(-0.5f * klog(rng.next()) * (1.f + kdiv(ldir.w * ldir.w, r * r))) * lp1.x;
+                }
+
+                if (v.z > -1.f + K_EPS && v.z < 1.f - K_EPS) {
+                    float tmp0 = 1.f - v.z * v.z;
+                    float tmp1 = r * krsqrt(tmp0);
+                    p.x = p.x + tmp1 * (v.x * v.z * cphi - v.y * sphi);
+                    p.y = p.y + tmp1 * (v.y * v.z * cphi + v.x * sphi);
+                    p.z = p.z - tmp1 * tmp0 * cphi;
+                } else {
+                    p.x += r * cphi;
+                    p.y += r * sphi;
+                }
+
+                positioned = true;
+            } else if (st == kCone || st == kIsotropic || st == kArcsine) {
+                float ang, stheta, ctheta, sphi, cphi;
+                ang = K_TWO_PI * rng.next();
+                ksincos(ang, &sphi, &cphi);
+
+                if (st == kCone) {
+                    do {
+                        ang = (lp1.y > 0) ? K_TWO_PI * rng.next() : kacos(2.f * rng.next() - 1.f);
+                    } while (ang > lp1.x);
+                } else if (st == kIsotropic) {
+                    ang = kacos(2.f * rng.next() - 1.f);
+                } else {
+                    ang = K_ONE_PI * rng.next();
+                }
+
+                ksincos(ang, &stheta, &ctheta);
+                rotatevector(v, stheta, ctheta, sphi, cphi);
+                s.Lmove = 0.f;
+            } else if (st == kZGaussian) {
+                float ang, stheta, ctheta, sphi, cphi;
+                ang = K_TWO_PI * rng.next();
+                ksincos(ang, &sphi, &cphi);
+                float u1 = rng.next();
+                float u2 = rng.next();
+                ang = ksqrt(-2.f * klog(u1)) * (1.f - 2.f * u2) * lp1.x;
+                ksincos(ang, &stheta, &ctheta);
+                rotatevector(v, stheta, ctheta, sphi, cphi);
+                s.Lmove = 0.f;
+            } else if (st == kLine || st == kSlit) {
+                float r_l = rng.next();
+                p.x = p.x + r_l * lp1.x;
+                p.y = p.y + r_l * lp1.y;
+                p.z = p.z + r_l * lp1.z;
+
+                if (st == kLine) {
+                    float sphi_l, cphi_l;
+                    r_l = krsqrt(lp1.x * lp1.x + lp1.y * lp1.y + lp1.z * lp1.z);
+
+                    if (lp2.x > 0.f) {
+                        float ax = lp1.x * r_l, ay = lp1.y * r_l, az = lp1.z * r_l;
+                        float vdotaxis = v.x * ax + v.y * ay + v.z * az;
+                        v.x = v.x - vdotaxis * ax;
+                        v.y = v.y - vdotaxis * ay;
+                        v.z = v.z - vdotaxis * az;
+                        float vnorm = krsqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+                        v.x = v.x * vnorm;
+                        v.y = v.y * vnorm;
+                        v.z = v.z * vnorm;
+                        r_l = lp2.x * (2.f * rng.next() - 1.f);
+                        ksincos(r_l, &sphi_l, &cphi_l);
+                        float cx = ay * v.z - az * v.y;
+                        float cy = az * v.x - ax * v.z;
+                        float cz = ax * v.y - ay * v.x;
+                        v.x = v.x * cphi_l + cx * sphi_l;
+                        v.y = v.y * cphi_l + cy * sphi_l;
+                        v.z = v.z * cphi_l + cz * sphi_l;
+                    } else {
+                        v.x = lp1.x * r_l;
+                        v.y = lp1.y * r_l;
+                        v.z = lp1.z * r_l;
+                        r_l = K_TWO_PI * rng.next();
+                        ksincos(r_l, &sphi_l, &cphi_l);
+                        rotatevector(v, 1.f, 0.f, sphi_l, cphi_l);
+                    }
+                } else if (lp2.x > 0.f || lp2.y > 0.f) {
+                    float sphi_s, cphi_s;
+                    r_l = K_TWO_PI * rng.next();
+                    ksincos(r_l, &sphi_s, &cphi_s);
+                    r_l = ksqrt(2.f * rng.scatlen());
+                    cphi_s *= lp2.x * r_l;
+                    sphi_s *= lp2.y * r_l;
+                    sphi_s *= krsqrt(lp1.x * lp1.x + lp1.y * lp1.y + lp1.z * lp1.z);
+                    float4 c = make_float4(lp1.y * v.z - lp1.z * v.y, lp1.z * v.x - lp1.x * v.z, lp1.x * v.y - lp1.y * v.x, 0.f);
+                    cphi_s *= krsqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+                    v.x += cphi_s * c.x + sphi_s * lp1.x;
+                    v.y += cphi_s * c.y + sphi_s * lp1.y;
+                    v.z += cphi_s * c.z + sphi_s * lp1.z;
+                    r_l = krsqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+                    v.x *= r_l;
+                    v.y *= r_l;
+                    v.z *= r_l;
+                }
+
+                positioned = true;
+                s.prop.x = lpos.x + lp1.x * 0.5f;
+                s.prop.y = lpos.y + lp1.y * 0.5f;
+                s.prop.z = lpos.z + lp1.z * 0.5f;
+                s.prop.w = 0.f;
+                s.Lmove = -1.f;
+            }
+
+            if (positioned) {
+                s.idx1d = (uint32_t)((int)floorf(p.z) * (int)g.dimxy + (int)floorf(p.y) * (int)g.dimx + (int)floorf(p.x));
+
+                if (p.x < 0.f || p.y < 0.f || p.z < 0.f || p.x >= g.maxidx_x || p.y >= g.maxidx_y || p.z >= g.maxidx_z) {
+                    s.mediaid = 0;
+                } else {
+                    s.mediaid = g.media[s.idx1d];
+                }
+            }
+        }
+
+        if (fabsf(p.w) <= g.minenergy) {
+            continue;   /* re-tests the loop condition, as the reference does */
+        }
+
+        if (s.Lmove < 0.f) {
+            if (ldir.w != ldir.w) {
+                float ang, stheta, ctheta, sphi, cphi;
+                ang = K_TWO_PI * rng.next();
+                ksincos(ang, &sphi, &cphi);
+                ang = kacos(2.f * rng.next() - 1.f);
+                ksincos(ang, &stheta, &ctheta);
+                rotatevector(v, stheta, ctheta, sphi, cphi);
+            } else if (ldir.w < 0.f && isinf(ldir.w)) {
+                float ang, stheta, ctheta, sphi, cphi;
+                ang = K_TWO_PI * rng.next();
+                ksincos(ang, &sphi, &cphi);
+                stheta = ksqrt(rng.next());
+                ctheta = ksqrt(1.f - stheta * stheta);
+                rotatevector(v, stheta, ctheta, sphi, cphi);
+            } else if (ldir.w != 0.f) {
+                float Rn2 = (float)((ldir.w > 0.f) - (ldir.w < 0.f));
+                s.prop.x += ldir.w * v.x;
+                s.prop.y += ldir.w * v.y;
+                s.prop.z += ldir.w * v.z;
+                v.x = Rn2 * (s.prop.x - p.x);
+                v.y = Rn2 * (s.prop.y - p.y);
+                v.z = Rn2 * (s.prop.z - p.z);
+                Rn2 = krsqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+                v.x *= Rn2;
+                v.y *= Rn2;
+                v.z *= Rn2;
+            }
+        }
+
+        if ((s.mediaid & K_MED_MASK) == 0) {
+            int id = skipvoid(g, s);
+
+            if (id >= 0) {
+                s.idx1d = (uint32_t)id;
+                s.mediaid = g.media[s.idx1d];
+            }
+        }
+
+        s.fx = to_short((int)floorf(p.x));
+        s.fy = to_short((int)floorf(p.y));
+        s.fz = to_short((int)floorf(p.z));
+        s.w0 += 1.f;
+
+        if (s.w0 > (float)g.maxvoidstep) {
+            return -1;
+        }
+    } while ((s.mediaid & K_MED_MASK) == 0 || fabsf(p.w) <= g.minenergy);
+
+    f.w += 1.f;
+    s.prop = getprop(sh, s.mediaid);
+    ppl[1] += p.w;
+    s.w0 = p.w;
+    ppl[2] = (g.srcnum > 1) ? ppl[2] : p.w;
+    v.w = K_EPS;
+    s.Lmove = 0.f;
+    return 0;
+}
+
+template <bool SAVEDET, bool REFLECT, bool PENCIL>
+__global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
+    extern __shared__ float4 lds[];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t pplen = g.w0offset + g.srcnum;
+    Shared sh;
+    sh.prop = lds;
+    sh.invcdf = (float*)(lds + g.medianum);
+    float* ppbase = sh.invcdf + ((g.nphase + 3) & ~3u);
+
+    for (uint32_t i = threadIdx.x; i < g.medianum; i += blockDim.x) {
+        sh.prop[i] = g.prop[i];
+    }
+
+    for (uint32_t i = threadIdx.x; i < g.nphase; i += blockDim.x) {
+        sh.invcdf[i] = g.invcdf[i];
+    }
+
+    /* lane-interleaved scratch: element k of lane t at [k*blockDim + t] would avoid
+       bank conflicts, but the record copy reads it sequentially; a per-lane block
+       keeps the indexing simple and the scratch is touched rarely */
+    sh.pp = ppbase + threadIdx.x * pplen;
+
+    for (uint32_t i = 0; i < pplen; i++) {
+        sh.pp[i] = 0.f;
+    }
+
+    __syncthreads();
+
+    if ((uint32_t)idx >= g.threadphoton * g.nthread + (uint32_t)g.oddphoton || (uint32_t)idx >= g.nthread) {
+        return;
+    }
+
+    const uint32_t pd = g.partialdata;
+    sh.pp[pd] = g.genergy[idx * 2];
+    sh.pp[pd + 1] = g.genergy[idx * 2 + 1];
+    Rng rng;
+    rng.s0 = (uint64_t)g.nseed[idx * 4] << 32 | g.nseed[idx * 4 + 1];
+    rng.s1 = (uint64_t)g.nseed[idx * 4 + 2] << 32 | g.nseed[idx * 4 + 3];
+
+    Photon s;
+    s.p = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x7FC00000u));
+    s.v = make_float4(0.f, 0.f, 0.f, 0.f);
+    s.f = make_float4(0.f, 0.f, 0.f, 0.f);
+    s.prop = make_float4(0.f, 0.f, 0.f, 0.f);
+    s.fx = s.fy = s.fz = 0;
+    s.fw = -1;
+    s.idx1d = 0;
+    s.mediaid = __float_as_uint(g.srcparam2.w);
+    s.w0 = 0.f;
+    s.Lmove = 0.f;
+    s.seed0 = s.seed1 = 0;
+
+    uint32_t isdet = 0, mediaidold = 0, idx1dold = 0, launchflag = 0;
+    float pathlen = 0.f, n1 = 0.f;
+    bool relaunch = true;
+
+    while (true) {
+        if (relaunch) {
+            if (launch_photon<SAVEDET, PENCIL>(g, sh, s, rng, idx, launchflag)) {
+                break;
+            }
+
+            isdet = s.mediaid & K_DET_MASK;
+            s.mediaid &= K_MED_MASK;
+            relaunch = false;
+        }
+
+        float4& p = s.p;
+        float4& v = s.v;
+        float4& f = s.f;
+
+        if (f.x <= 0.f) {
+            f.x = rng.scatlen();
+
+            if (v.w != K_EPS) {
+                float cphi = 1.f, sphi = 0.f, theta, stheta, ctheta, tmp0 = 0.f;
+
+                if (!g.is2d) {
+                    tmp0 = K_TWO_PI * rng.next();
+                    ksincos(tmp0, &sphi, &cphi);
+                }
+
+                if (g.nphase > 2) {
+                    tmp0 = rng.next() * (float)(g.nphase - 1);
+                    theta = tmp0 - (float)((int)tmp0);
+                    uint32_t i0 = (uint32_t)tmp0 >= g.nphase ? g.nphase - 1 : (uint32_t)(int)tmp0;
+                    uint32_t i1 = (uint32_t)tmp0 + 1 >= g.nphase ? g.nphase - 1 : (uint32_t)((int)tmp0 + 1);
+                    tmp0 = (1.f - theta) * sh.invcdf[i0] + theta * sh.invcdf[i1];
+                    theta = kacos(tmp0);
+                    stheta = ksin(theta);
+                    ctheta = tmp0;
+                } else {
+                    tmp0 = (v.w > (float)g.gscatter) ? 0.f : s.prop.z;
+
+                    if (fabsf(tmp0) > K_EPS) {
+                        tmp0 = kdiv(1.f - s.prop.z * s.prop.z, 1.f - s.prop.z + 2.f * s.prop.z * rng.next());
+                        tmp0 *= tmp0;
+                        tmp0 = kdiv(1.f + s.prop.z * s.prop.z - tmp0, 2.f * s.prop.z);
+                        tmp0 = fmaxf(-1.f, fminf(1.f, tmp0));
+                        theta = kacos(tmp0);
+                        stheta = ksin(theta);
+                        ctheta = tmp0;
+                    } else {
+                        theta = kacos(2.f * rng.next() - 1.f);
+                        ksincos(theta, &stheta, &ctheta);
+                    }
+                }
+
+                if (SAVEDET && (s.mediaid & K_MED_MASK) > 0) {
+                    if ((g.savedetflag >> 1) & 1) {
+                        uint32_t* cnt = (uint32_t*)sh.pp;
+                        cnt[(s.mediaid & K_MED_MASK) - 1]++;
+                    }
+
+                    if ((g.savedetflag >> 3) & 1) {
+                        sh.pp[g.maxmedia * (((g.savedetflag >> 1) & 1) + ((g.savedetflag >> 2) & 1)) +
+                                         (s.mediaid & K_MED_MASK) - 1] += 1.f - ctheta;
+                    }
+                }
+
+                if (g.is2d) {
+                    rotatevector2d(v, (rng.next() > 0.5f ? stheta : -stheta), ctheta, (int)g.is2d);
+                } else {
+                    rotatevector(v, stheta, ctheta, sphi, cphi);
+                }
+
+                v.w += 1.f;
+
+                if (g.outputtype == kWP || g.outputtype == kDCS) {
+                    uint32_t pid = photon_index(g, idx, f.w) - 1;
+                    tmp0 = (g.outputtype == kDCS) ? (1.f - ctheta) : 1.f;
+                    tmp0 *= g.replayweight[pid];
+                    int tshift = gate_of(g, g.replaytof[pid]) +
+                                 ((g.replaydet == -1) ? (((g.replaydetid[pid] & 0xFFFF) - 1) * (int)g.maxgate) : 0);
+
+                    if (g.extrasrclen && g.srcid < 0) {
+                        tshift += ((int)sh.pp[g.w0offset - 1] - 1) * ((g.replaydet == -1) ? (int)g.detnum : 1) *
+                                  (int)g.maxgate;
+                    }
+
+                    deposit(g, s.idx1d + (uint32_t)tshift * g.nvox, tmp0);
+                }
+            }
+
+            v.w = (float)(int)v.w;
+        }
+
+        n1 = s.prop.w;
+        s.prop = getprop(sh, s.mediaid);
+        int face;
+        f.z = hitgrid(p, v, s.fx, s.fy, s.fz, &face);
+        s.fw = face;
+        float gfac = (v.w + 1.f > (float)g.gscatter ? (1.f - s.prop.z) : 1.f);
+        float slen = f.z * s.prop.y * gfac;
+        slen = fminf(slen, f.x);
+        f.z = kdiv(slen, s.prop.y * gfac);
+        pathlen += f.z;
+        p.x = p.x + f.z * v.x;
+        p.y = p.y + f.z * v.y;
+        p.z = p.z + f.z * v.z;
+
+        if (face == 0) {
+            s.fx += (slen == f.x) ? 0 : (v.x > 0.f ? 1 : -1);
+        }
+
+        if (face == 1) {
+            s.fy += (slen == f.x) ? 0 : (v.y > 0.f ? 1 : -1);
+        }
+
+        if (face == 2) {
+            s.fz += (slen == f.x) ? 0 : (v.z > 0.f ? 1 : -1);
+        }
+
+        p.w *= kexp(-s.prop.x * f.z);
+        f.x -= slen;
+        f.y += f.z * s.prop.w * g.oneoverc0;
+
+        if (SAVEDET && ((g.savedetflag >> 2) & 1) && (s.mediaid & K_MED_MASK) > 0) {
+            sh.pp[g.maxmedia * ((g.savedetflag >> 1) & 1) + (s.mediaid & K_MED_MASK) - 1] += f.z;
+        }
+
+        mediaidold = s.mediaid | isdet;
+        idx1dold = s.idx1d;
+        s.idx1d = (uint32_t)(s.fz * (int)g.dimxy + s.fy * (int)g.dimx + s.fx);
+
+        if (!inside(g, s.fx, s.fy, s.fz)) {
+            s.mediaid = 0;
+            s.idx1d = (s.fx < 0 || s.fy < 0 || s.fz < 0) ? K_OUT_MIN : K_OUT_MAX;
+            isdet = g.bc[(s.idx1d == K_OUT_MAX) * 3 + face];
+            isdet = ((isdet & 0xF) == 0) ? (g.doreflect ? (uint32_t)kReflect : (uint32_t)kAbsorb) : isdet;
+        } else {
+            s.mediaid = g.media[s.idx1d];
+            isdet = s.mediaid & K_DET_MASK;
+            s.mediaid &= K_MED_MASK;
+        }
+
+        if (s.idx1d != idx1dold && idx1dold < g.nvox && mediaidold) {
+            if (g.save2pt && f.y >= g.twin0 && f.y < g.twin1) {
+                float weight = 0.f;
+                int tshift = gate_of(g, f.y);
+
+                if (g.outputtype == kEnergy) {
+                    weight = s.w0 - p.w;
+                } else if (g.outputtype == kFluence || g.outputtype == kFlux) {
+                    weight = (s.prop.x < K_EPS) ? (s.w0 * pathlen) : kdiv(s.w0 - p.w, s.prop.x);
+                } else if (g.seed == K_SEED_FROM_FILE) {
+                    if (g.outputtype == kJacobian) {
+                        uint32_t pid = photon_index(g, idx, f.w) - 1;
+                        weight = g.replayweight[pid] * pathlen;
+                        tshift = gate_of(g, g.replaytof[pid]) +
+                                 ((g.replaydet == -1) ? (((g.replaydetid[pid] & 0xFFFF) - 1) * (int)g.maxgate) : 0);
+                    }
+                } else if (g.outputtype == kL) {
+                    weight = s.w0 * pathlen;
+                }
+
+                if (g.extrasrclen && g.srcid < 0) {
+                    tshift += ((int)sh.pp[g.w0offset - 1] - 1) * ((g.replaydet == -1) ? (int)g.detnum : 1) *
+                              (int)g.maxgate;
+                }
+
+                if (fabsf(weight) > 0.f) {
+                    deposit(g, idx1dold + (uint32_t)tshift * g.nvox, weight);
+                }
+            }
+
+            s.w0 = p.w;
+            pathlen = 0.f;
+        } else {
+            s.mediaid = mediaidold;
+        }
+
+        if ((s.mediaid == 0 && ((isdet & 0xF) == kAbsorb || (isdet & 0xF) == kCyclic ||
+                                ((isdet & 0xF) == kReflect && n1 == sh.prop[0].w))) || f.y > g.twin1) {
+            if (isdet == kCyclic) {
+                if (face == 0) {
+                    p.x = nextafter_ref(rintf((s.idx1d == K_OUT_MIN) ? g.maxidx_x : 0.f), (v.x > 0.f) - (v.x < 0.f));
+                    s.fx = to_short((int)floorf(p.x));
+                }
+
+                if (face == 1) {
+                    p.y = nextafter_ref(rintf((s.idx1d == K_OUT_MIN) ? g.maxidx_y : 0.f), (v.y > 0.f) - (v.y < 0.f));
+                    s.fy = to_short((int)floorf(p.y));
+                }
+
+                if (face == 2) {
+                    p.z = nextafter_ref(rintf((s.idx1d == K_OUT_MIN) ? g.maxidx_z : 0.f), (v.z > 0.f) - (v.z < 0.f));
+                    s.fz = to_short((int)floorf(p.z));
+                }
+
+                if (inside(g, s.fx, s.fy, s.fz)) {
+                    s.idx1d = (uint32_t)(s.fz * (int)g.dimxy + s.fy * (int)g.dimx + s.fx);
+                    s.mediaid = g.media[s.idx1d];
+                    isdet = s.mediaid & K_DET_MASK;
+                    s.mediaid &= K_MED_MASK;
+                    continue;
+                }
+            }
+
+            launchflag = (((s.idx1d == K_OUT_MAX && g.bc[9 + face]) || (s.idx1d == K_OUT_MIN && g.bc[6 + face])) ?
+                          K_OUT_MIN : (mediaidold & K_DET_MASK));
+            relaunch = true;
+            continue;
+        }
+
+        if (fabsf(p.w) < g.minenergy) {
+            if (rng.next() * K_ROULETTE <= 1.f) {
+                p.w *= K_ROULETTE;
+            } else {
+                launchflag = mediaidold & K_DET_MASK;
+                relaunch = true;
+                continue;
+            }
+        }
+
+        if (REFLECT) {
+            s.prop = getprop(sh, s.mediaid);
+
+            if (((s.mediaid && g.doreflect) ||
+                    (s.mediaid == 0 && (((isdet & 0xF) == 0 && g.doreflect) || ((isdet & 0xF) == kReflect ||
+                                        (isdet & 0xF) == kMirror)))) &&
+                    (((isdet & 0xF) == kMirror) || n1 != s.prop.w)) {
+                float Rtotal = 1.f;
+                float cphi, sphi, stheta, ctheta;
+                float tmp0 = n1 * n1;
+                float tmp1 = s.prop.w * s.prop.w;
+                cphi = fabsf((face == 0) ? v.x : (face == 1 ? v.y : v.z));
+                sphi = 1.f - cphi * cphi;
+                f.z = 1.f - kdiv(tmp0, tmp1) * sphi;
+
+                if (f.z > 0.f && (isdet & 0xF) != kMirror) {
+                    ctheta = tmp0 * cphi * cphi + tmp1 * f.z;
+                    stheta = 2.f * n1 * s.prop.w * cphi * ksqrt(f.z);
+                    Rtotal = kdiv(ctheta - stheta, ctheta + stheta);
+                    ctheta = tmp1 * cphi * cphi + tmp0 * f.z;
+                    Rtotal = (Rtotal + kdiv(ctheta - stheta, ctheta + stheta)) * 0.5f;
+                }
+
+                if (Rtotal < 1.f && (!(s.mediaid == 0 && ((isdet & 0xF) == kMirror))) && rng.next() > Rtotal) {
+                    transmit(v, n1, s.prop.w, face);
+
+                    if (s.mediaid == 0) {
+                        launchflag = (((s.idx1d == K_OUT_MAX && g.bc[9 + face]) ||
+                                       (s.idx1d == K_OUT_MIN && g.bc[6 + face])) ? K_OUT_MIN : (mediaidold & K_DET_MASK));
+                        relaunch = true;
+                        continue;
+                    }
+                } else {
+                    if (face == 0) {
+                        v.x = -v.x;
+                        p.x = nextafter_ref(rintf(p.x), 0);
+                        s.fx = to_short((int)rintf(p.x));
+                    } else if (face == 1) {
+                        v.y = -v.y;
+                        p.y = nextafter_ref(rintf(p.y), 0);
+                        s.fy = to_short((int)rintf(p.y));
+                    } else {
+                        v.z = -v.z;
+                        p.z = nextafter_ref(rintf(p.z), 0);
+                        s.fz = to_short((int)rintf(p.z));
+                    }
+
+                    s.idx1d = idx1dold;
+                    s.mediaid = (g.media[s.idx1d] & K_MED_MASK);
+                    s.prop = getprop(sh, s.mediaid);
+                    n1 = s.prop.w;
+                }
+            }
+        }
+    }
+
+    g.genergy[idx * 2] = sh.pp[pd];
+    g.genergy[idx * 2 + 1] = sh.pp[pd + 1];
+}
+
+}  // namespace mcxk

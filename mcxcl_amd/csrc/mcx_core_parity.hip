@@ -1,0 +1,143 @@
+/*
+ * mcx_core_parity.hip - transport kernel instances with bit-reproducible
+ * arithmetic (deterministic FP32 math, IEEE divide/sqrt).  Must be compiled
+ * with -ffp-contract=off so no multiply-add is fused.
+ */
+#define MCX_PARITY 1
+#include "mcx_core_impl.h"
+
+template <bool SAVEDET, bool REFLECT>
+static hipError_t launch_variant(const MCXKParam* prm, int nblock, int nthreadblock, size_t shmem, hipStream_t stream) {
+    if (prm->srctype == kPencil) {
+        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, true>), dim3(nblock), dim3(nthreadblock), shmem,
+                           stream, *prm);
+    } else {
+        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, false>), dim3(nblock), dim3(nthreadblock), shmem,
+                           stream, *prm);
+    }
+
+    return hipGetLastError();
+}
+
+extern "C" hipError_t mcx_launch_parity(const MCXKParam* prm, int nblock, int nthreadblock, size_t shmem,
+                                        hipStream_t stream, int savedet, int reflect) {
+    if (savedet) {
+        return reflect ? launch_variant<true, true>(prm, nblock, nthreadblock, shmem, stream)
+               : launch_variant<true, false>(prm, nblock, nthreadblock, shmem, stream);
+    }
+
+    return reflect ? launch_variant<false, true>(prm, nblock, nthreadblock, shmem, stream)
+           : launch_variant<false, false>(prm, nblock, nthreadblock, shmem, stream);
+}
+
+/* RNG stream dump (reference debug mode -D R, src/mcx_core.cl:2408-2414) */
+__global__ void rngdump_kernel(const MCXKParam g) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+
+    if (idx >= g.nthread) {
+        return;
+    }
+
+    mcxk::Rng rng;
+    rng.s0 = (uint64_t)g.nseed[idx * 4] << 32 | g.nseed[idx * 4 + 1];
+    rng.s1 = (uint64_t)g.nseed[idx * 4 + 2] << 32 | g.nseed[idx * 4 + 3];
+
+    for (uint32_t i = idx; i < g.fieldlen; i += g.nthread) {
+        g.field[i] = (double)rng.next();
+    }
+}
+
+extern "C" hipError_t mcx_launch_rngdump(const MCXKParam* prm, int nblock, int nthreadblock, hipStream_t stream) {
+    hipLaunchKernelGGL(rngdump_kernel, dim3(nblock), dim3(nthreadblock), 0, stream, *prm);
+    return hipGetLastError();
+}
+
+/* double accumulator -> normalised float output, grid-stride */
+__global__ void finalize_kernel(const double* __restrict__ in, float* __restrict__ out, size_t n, float scale) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float v = (float)in[i];
+        out[i] = v * scale;
+    }
+}
+
+extern "C" hipError_t mcx_launch_finalize(const double* in, float* out, size_t n, float scale, hipStream_t stream) {
+    int nblock = (int)((n + 255) / 256);
+    nblock = nblock > 8192 ? 8192 : (nblock < 1 ? 1 : nblock);
+    hipLaunchKernelGGL(finalize_kernel, dim3(nblock), dim3(256), 0, stream, in, out, n, scale);
+    return hipGetLastError();
+}
+
+/* ---- device self-checks: math in parity mode, raw RNG stream ---- */
+__global__ void fpmath_kernel(int which, const float* __restrict__ x, float* __restrict__ y, size_t n) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+
+    if (i >= n) {
+        return;
+    }
+
+    float a = x[i], r = 0.f, s, c;
+
+    switch (which) {
+        case 0:
+            r = mcxk::kexp(a);
+            break;
+
+        case 1:
+            r = mcxk::klog(a);
+            break;
+
+        case 2:
+            mcxk::ksincos(a, &s, &c);
+            r = s;
+            break;
+
+        case 3:
+            mcxk::ksincos(a, &s, &c);
+            r = c;
+            break;
+
+        case 4:
+            r = mcxk::kacos(a);
+            break;
+
+        case 5:
+            r = mcxk::krsqrt(a);
+            break;
+
+        case 6:
+            r = mcxk::kdiv(a, x[i + n]);
+            break;
+
+        default:
+            r = mcxk::ksqrt(a);
+            break;
+    }
+
+    y[i] = r;
+}
+
+extern "C" hipError_t mcx_launch_fpmath_parity(int which, const float* x, float* y, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(fpmath_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, which, x, y, n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
+__global__ void rngstream_kernel(const uint32_t* seed4, unsigned int n, float* out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) {
+        return;
+    }
+
+    mcxk::Rng rng;
+    rng.s0 = (uint64_t)seed4[0] << 32 | seed4[1];
+    rng.s1 = (uint64_t)seed4[2] << 32 | seed4[3];
+
+    for (unsigned int i = 0; i < n; i++) {
+        out[i] = rng.next();
+    }
+}
+
+extern "C" hipError_t mcx_launch_rngstream(const uint32_t* seed4, unsigned int n, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(rngstream_kernel, dim3(1), dim3(64), 0, s, seed4, n, out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}

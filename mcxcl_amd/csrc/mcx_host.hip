@@ -1,0 +1,742 @@
+/*
+ * mcx_host.hip - host runtime of the MI355X build: device discovery, buffer
+ * staging, kernel launch, read-back and normalisation behind the reference's
+ * C entry points (mcx_run_simulation / mcx_list_gpu, src/mcx_host.cpp:252-1849).
+ *
+ * Device memory layout (one simulation, all resident in HBM):
+ *   media    uint32[nvox]          labels, x fastest, bit 31 = detector mask
+ *   prop     float4[medianum]      {mua, mus, g, n}; staged into LDS per block
+ *   field    double[fieldlen]      fieldlen = nvox * gates (* sources * detectors)
+ *   nseed    uint32[4*nthread]     per-thread xorshift128+ seed words
+ *   genergy  float[2*nthread]      escaped / launched weight per thread
+ *   detrec   float[maxdet*reclen]  compacted detected-photon records
+ *   detseed  uint64[2*maxdet]      launch RNG state per detected photon
+ * All time gates are simulated in a single launch (288 GB of HBM removes the
+ * reference's maxgate windowing, src/mcx_host.cpp:628-632, :1080).
+ */
+#include <math.h>
+#include <setjmp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <hip/hip_runtime.h>
+
+#include "mcx_core.h"
+#include "mcx_host.h"
+
+extern "C" void* mcx_push_errjmp(void* jb);
+extern "C" void mcx_pop_errjmp(void* old);
+extern "C" hipError_t mcx_launch_fpmath_parity(int which, const float* x, float* y, size_t n, hipStream_t s);
+extern "C" hipError_t mcx_launch_fpmath_fast(int which, const float* x, float* y, size_t n, hipStream_t s);
+extern "C" hipError_t mcx_launch_rngstream(const uint32_t* seed4, unsigned int n, float* out, hipStream_t s);
+
+/* glibc random() TYPE_3 additive-feedback generator, written from its published
+   description: r[0]=seed, r[i]=16807*r[i-1] mod (2^31-1) for i<31,
+   r[i]=r[i-31] for 31<=i<34, r[i]=r[i-31]+r[i-3] (mod 2^32) afterwards; the k-th
+   output of rand() is r[k+344]>>1.  srand(0) behaves as srand(1). */
+extern "C" void mcx_host_seed_words(unsigned int seed, size_t n, unsigned int* out) {
+    uint32_t r[34 + 344];
+    int32_t s = (int32_t)(seed == 0 ? 1 : seed);
+    r[0] = (uint32_t)s;
+
+    for (int i = 1; i < 31; i++) {
+        int32_t hi = s / 127773, lo = s % 127773;
+        int32_t word = 16807 * lo - 2836 * hi;
+
+        if (word < 0) {
+            word += 2147483647;
+        }
+
+        s = word;
+        r[i] = (uint32_t)s;
+    }
+
+    for (int i = 31; i < 34; i++) {
+        r[i] = r[i - 31];
+    }
+
+    /* ring of the last 34 values */
+    uint32_t ring[34];
+    memcpy(ring, r, sizeof(ring));
+    size_t pos = 34;   /* index of the next value */
+
+    for (; pos < 344 + n; pos++) {
+        uint32_t v = ring[(pos - 31) % 34] + ring[(pos - 3) % 34];
+        ring[pos % 34] = v;
+
+        if (pos >= 344) {
+            out[pos - 344] = v >> 1;
+        }
+    }
+}
+
+extern "C" int mcx_device_count(void) {
+    int n = 0;
+
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        return 0;
+    }
+
+    return n;
+}
+
+/* reference src/mcx_host.cpp:252-432 (device enumeration and autopilot sizing,
+   AMD rule: 64-lane blocks, 2560 threads per compute unit, :607-609) */
+extern "C" int mcx_list_gpu(Config* cfg, unsigned int* activedev, int* activedevlist, GPUInfo** info) {
+    int n = mcx_device_count();
+    *activedev = 0;
+
+    if (n <= 0) {
+        if (info) {
+            *info = NULL;
+        }
+
+        return 0;
+    }
+
+    GPUInfo* gi = (GPUInfo*)calloc(n, sizeof(GPUInfo));
+
+    for (int i = 0; i < n; i++) {
+        hipDeviceProp_t prop;
+
+        if (hipGetDeviceProperties(&prop, i) != hipSuccess) {
+            continue;
+        }
+
+        snprintf(gi[i].name, MAX_SESSION_LENGTH, "%s (%s)", prop.name, prop.gcnArchName);
+        gi[i].id = i + 1;
+        gi[i].devcount = n;
+        gi[i].major = prop.major;
+        gi[i].minor = prop.minor;
+        gi[i].globalmem = prop.totalGlobalMem;
+        gi[i].constmem = prop.totalConstMem;
+        gi[i].sharedmem = prop.sharedMemPerBlock;
+        gi[i].regcount = prop.regsPerBlock;
+        gi[i].clock = prop.clockRate;
+        gi[i].sm = prop.multiProcessorCount;
+        gi[i].core = prop.multiProcessorCount * 64;
+        gi[i].autoblock = 64;
+        gi[i].autothread = (size_t)2560 * prop.multiProcessorCount;
+        gi[i].maxmpthread = prop.maxThreadsPerMultiProcessor;
+        gi[i].vendor = dvAMD;
+        gi[i].iscpu = 0;
+
+        /* gpuid 0 selects the first device; deviceid[] is a '1'/'0' mask */
+        int selected = (cfg && cfg->gpuid > 0) ? (cfg->gpuid == i + 1) :
+                       (cfg ? (i < MAX_DEVICE && cfg->deviceid[i] == '1') : (i == 0));
+
+        if (selected && activedevlist) {
+            activedevlist[(*activedev)] = i;
+        }
+
+        if (selected) {
+            (*activedev)++;
+        }
+    }
+
+    if (info) {
+        *info = gi;
+    } else {
+        free(gi);
+    }
+
+    return n;
+}
+
+/* ---------------- simulation ---------------- */
+namespace {
+
+struct DevBuf {
+    void* ptr[16];
+    int n;
+    void add(void* p) {
+        ptr[n++] = p;
+    }
+    void release() {
+        for (int i = 0; i < n; i++) {
+            if (ptr[i]) {
+                (void)hipFree(ptr[i]);
+            }
+        }
+
+        n = 0;
+    }
+};
+
+double wall_ms() {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+}  // namespace
+
+#define HIP_TRY(call)                                   \
+    do {                                                \
+        hipError_t e__ = (call);                        \
+        if (e__ != hipSuccess) {                        \
+            err = e__;                                  \
+            errline = __LINE__;                         \
+            goto fail;                                  \
+        }                                               \
+    } while (0)
+
+template <typename T>
+static hipError_t dalloc(DevBuf& bufs, T** p, size_t n) {
+    *p = NULL;
+
+    if (n == 0) {
+        return hipSuccess;
+    }
+
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+
+    if (e == hipSuccess) {
+        bufs.add(*p);
+    }
+
+    return e;
+}
+
+extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalenergy) {
+    (void)fluence;
+    (void)totalenergy;
+    hipError_t err = hipSuccess;
+    int errline = 0;
+    DevBuf bufs;
+    bufs.n = 0;
+    hipStream_t stream = NULL;
+    hipEvent_t ev0 = NULL, ev1 = NULL;
+    uint32_t* hseed = NULL;
+    float* hgenergy = NULL;
+    float* hfield = NULL;
+    uint32_t hdetcount = 0;
+    char msg[512];
+
+    unsigned int activedev = 0;
+    int devlist[MAX_DEVICE];
+    GPUInfo* gpu = NULL;
+    int ndev = mcx_list_gpu(cfg, &activedev, devlist, &gpu);
+
+    if (ndev == 0 || activedev == 0) {
+        free(gpu);
+        MCX_ERROR(-99, "No HIP device found (or the specified GPU does not exist)");
+    }
+
+    const int dev = devlist[0];
+
+    if (cfg->mediabyte > 4) {
+        free(gpu);
+        MCX_ERROR(-4, "unsupported media format for the GPU kernel");
+    }
+
+    if (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->srctype == MCX_SRC_FOURIERX ||
+            cfg->srctype == MCX_SRC_FOURIERX2D || cfg->srctype == MCX_SRC_HYPERBOLOID_GAUSSIAN) {
+        free(gpu);
+        MCX_ERROR(-4, "this source type is not supported by the MI355X kernel");
+    }
+
+    if (cfg->minenergy >= cfg->srcpos.w) {
+        free(gpu);
+        MCX_ERROR(-4, "minenergy must be below the source weight (no photon could ever launch)");
+    }
+
+    if (cfg->srctype == MCX_SRC_CONE && !(cfg->srcparam1.x > 0.f)) {
+        free(gpu);
+        MCX_ERROR(-4, "cone source needs a positive half angle in srcparam1.x");
+    }
+
+    if (cfg->nangle || cfg->polmedianum || cfg->srcnum > 1 || cfg->issaveref > 1 || cfg->issavedet == FILL_MAXDETPHOTON ||
+            (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY))) {
+        free(gpu);
+        MCX_ERROR(-4, "angle-CDF launch, polarization, photon sharing, issaveref>1, fill-max mode and trajectory "
+                  "saving are not supported by the MI355X kernel");
+    }
+
+    /* thread map: reference autopilot for AMD devices, or the user's nthread/nblocksize */
+    size_t nblocksize = cfg->autopilot ? gpu[dev].autoblock : cfg->nblocksize;
+    size_t nthread = cfg->autopilot ? gpu[dev].autothread : cfg->nthread;
+
+    if (nblocksize == 0 || nblocksize > 256 || (nblocksize & 63)) {
+        free(gpu);
+        MCX_ERROR(-4, "block size must be a multiple of 64 and at most 256");
+    }
+
+    nthread = (nthread / nblocksize) * nblocksize;
+
+    if (nthread == 0) {
+        free(gpu);
+        MCX_ERROR(-4, "thread number must be at least one block");
+    }
+
+    free(gpu);
+    gpu = NULL;
+
+    const unsigned int maxgate = (unsigned int)((cfg->tend - cfg->tstart) / cfg->tstep + 0.5);
+    cfg->maxgate = maxgate;
+    const size_t nvox = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
+    const unsigned int nsrcslot = (cfg->srcid < 0) ? cfg->extrasrclen + 1 : 1;
+    const unsigned int ndetslot = (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) ? cfg->detnum : 1;
+    const size_t fieldlen = nvox * maxgate * nsrcslot * ndetslot;
+    const unsigned int partialdata = mcx_partialdata(cfg);
+    const unsigned int reclen = mcx_detreclen(cfg);
+
+    if (fieldlen >= 0xFFFFFFFFull || nvox >= 0x7FFFFFFFull) {
+        MCX_ERROR(-4, "domain too large for 32-bit voxel indexing");
+    }
+
+    MCXKParam g;
+    memset(&g, 0, sizeof(g));
+    memcpy(&g.srcpos, &cfg->srcpos, 16);
+    memcpy(&g.srcdir, &cfg->srcdir, 16);
+    memcpy(&g.srcparam1, &cfg->srcparam1, 16);
+    memcpy(&g.srcparam2, &cfg->srcparam2, 16);
+    g.maxidx_x = (float)cfg->dim.x;
+    g.maxidx_y = (float)cfg->dim.y;
+    g.maxidx_z = (float)cfg->dim.z;
+    g.dimx = cfg->dim.x;
+    g.dimy = cfg->dim.y;
+    g.dimz = cfg->dim.z;
+    g.dimxy = cfg->dim.x * cfg->dim.y;
+    g.nvox = (uint32_t)nvox;
+    g.twin0 = cfg->tstart;
+    g.twin1 = cfg->tend;
+    g.oneoverc0 = R_C0 * cfg->unitinmm;
+    g.Rtstep = 1.f / cfg->tstep;
+    g.minenergy = cfg->minenergy;
+    {
+        float minstep = fminf(fminf(cfg->steps.x, cfg->steps.y), cfg->steps.z);
+        g.minaccumtime = minstep * R_C0 * cfg->unitinmm;
+    }
+    g.maxdetphoton = cfg->maxdetphoton;
+    g.maxmedia = cfg->medianum - 1;
+    g.medianum = cfg->medianum;
+    g.detnum = cfg->detnum;
+    g.voidtime = (uint32_t)cfg->voidtime;
+    g.srctype = (uint32_t)cfg->srctype;
+    g.maxvoidstep = (uint32_t)cfg->maxvoidstep;
+    g.issaveseed = cfg->issaveseed > 0;
+    g.issaveref = (uint32_t)cfg->issaveref;
+    g.isspecular = cfg->isspecular > 0;
+    g.maxgate = maxgate;
+    g.outputtype = (uint32_t)cfg->outputtype;
+    g.savedetflag = cfg->savedetflag;
+    g.reclen = reclen;
+    g.partialdata = partialdata;
+    g.w0offset = partialdata + 4;
+    g.gscatter = cfg->gscatter;
+    g.srcnum = cfg->srcnum;
+    g.doreflect = (uint32_t)cfg->isreflect;
+    g.save2pt = (uint32_t)cfg->issave2pt;
+    g.savedet = (uint32_t)cfg->issavedet;
+    g.nphase = cfg->nphase;
+    g.extrasrclen = cfg->extrasrclen;
+    g.nthread = (uint32_t)nthread;
+    g.debugrng = (cfg->debuglevel & MCX_DEBUG_RNG) ? 1 : 0;
+    g.mediaformat = cfg->mediabyte;
+    g.seed = cfg->seed;
+    g.replaydet = cfg->replaydet;
+    g.srcid = cfg->srcid;
+    g.fieldlen = (uint32_t)fieldlen;
+    memcpy(g.bc, cfg->bc, 12);
+
+    {
+        unsigned int is2d = (cfg->dim.x == 1 ? 1 : (cfg->dim.y == 1 ? 2 : (cfg->dim.z == 1 ? 3 : 0)));
+
+        if (is2d) {
+            is2d = is2d * (((cfg->dim.x > 1) + (cfg->dim.y > 1) + (cfg->dim.z > 1)) == 2);
+        }
+
+        g.is2d = is2d;
+    }
+
+    /* photon split across the persistent threads (src/mcx_host.cpp:1101-1102) */
+    g.threadphoton = (uint32_t)(cfg->nphoton / nthread);
+    g.oddphoton = (int32_t)(cfg->nphoton - (size_t)g.threadphoton * nthread);
+
+    /* reflection block rule (src/mcx_host.cpp:945-956) */
+    int reflect;
+    {
+        char bcs[13] = {0}, allabsorb[7] = {bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, 0};
+        memcpy(bcs, cfg->bc, 12);
+        reflect = cfg->isreflect || (strcmp(bcs, allabsorb) && bcs[0] != 0);
+
+        for (int i = 0; i < 6; i++) {
+            if (cfg->bc[i] == bcReflect || cfg->bc[i] == bcMirror) {
+                reflect = 1;
+            }
+        }
+    }
+
+    size_t pplen = g.w0offset + g.srcnum;
+    size_t shmem = cfg->medianum * sizeof(float4) + ((cfg->nphase + 3) & ~3u) * sizeof(float) +
+                   nblocksize * pplen * sizeof(float);
+
+    if (shmem > 64 * 1024) {
+        MCX_ERROR(-4, "too many media/phase samples for the LDS tables");
+    }
+
+    if (cfg->seed == SEED_FROM_FILE &&
+            (cfg->nphoton == 0 || !cfg->replay.seed || !cfg->replay.weight || !cfg->replay.tof || !cfg->replay.detid)) {
+        MCX_ERROR(-6, "replay mode needs seeds, weights, times of flight and detector ids");
+    }
+
+    cfg->detectedcount = 0;
+    cfg->his.detected = 0;
+    cfg->his.respin = cfg->respin;
+    cfg->his.colcount = reclen;
+    cfg->energytot = 0;
+    cfg->energyesc = 0;
+    cfg->runtime = 0;
+
+    double t0 = 0, t1 = 0;
+    float kernelms = 0.f;
+    uint32_t* dmedia = NULL;
+    float4* dprop = NULL;
+    float4* ddetpos = NULL;
+    float* dinvcdf = NULL;
+    float4* dsrcdata = NULL;
+    float* dreplayw = NULL;
+    float* dreplaytof = NULL;
+    int* dreplaydetid = NULL;
+    uint64_t* dreplayseed = NULL;
+    uint32_t* dnseed = NULL;
+    double* dfield = NULL;
+    float* dgenergy = NULL;
+    float* ddetrec = NULL;
+    uint64_t* ddetseed = NULL;
+    uint32_t* ddetcount = NULL;
+    float* dout = NULL;
+    size_t nrep = 0;
+
+    HIP_TRY(hipSetDevice(dev));
+    HIP_TRY(hipStreamCreate(&stream));
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+
+    HIP_TRY(dalloc(bufs, &dmedia, nvox));
+    HIP_TRY(hipMemcpyAsync(dmedia, cfg->vol, nvox * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    HIP_TRY(dalloc(bufs, &dprop, cfg->medianum));
+    HIP_TRY(hipMemcpyAsync(dprop, cfg->prop, cfg->medianum * sizeof(float4), hipMemcpyHostToDevice, stream));
+
+    if (cfg->detnum) {
+        HIP_TRY(dalloc(bufs, &ddetpos, cfg->detnum));
+        HIP_TRY(hipMemcpyAsync(ddetpos, cfg->detpos, cfg->detnum * sizeof(float4), hipMemcpyHostToDevice, stream));
+    }
+
+    if (cfg->nphase) {
+        HIP_TRY(dalloc(bufs, &dinvcdf, cfg->nphase));
+        HIP_TRY(hipMemcpyAsync(dinvcdf, cfg->invcdf, cfg->nphase * sizeof(float), hipMemcpyHostToDevice, stream));
+    }
+
+    if (cfg->extrasrclen) {
+        HIP_TRY(dalloc(bufs, &dsrcdata, (size_t)cfg->extrasrclen * 4));
+        HIP_TRY(hipMemcpyAsync(dsrcdata, cfg->srcdata, cfg->extrasrclen * sizeof(ExtraSrc), hipMemcpyHostToDevice, stream));
+    }
+
+    if (cfg->seed == SEED_FROM_FILE) {
+        nrep = cfg->nphoton;
+
+        HIP_TRY(dalloc(bufs, &dreplayw, nrep));
+        HIP_TRY(dalloc(bufs, &dreplaytof, nrep));
+        HIP_TRY(dalloc(bufs, &dreplaydetid, nrep));
+        HIP_TRY(dalloc(bufs, &dreplayseed, nrep * 2));
+        HIP_TRY(hipMemcpyAsync(dreplayw, cfg->replay.weight, nrep * sizeof(float), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(dreplaytof, cfg->replay.tof, nrep * sizeof(float), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(dreplaydetid, cfg->replay.detid, nrep * sizeof(int), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(dreplayseed, cfg->replay.seed, nrep * 16, hipMemcpyHostToDevice, stream));
+    }
+
+    /* seeds: glibc srand(seed) then one rand() per 32-bit word (src/mcx_host.cpp:696-700, :759-768) */
+    hseed = (uint32_t*)malloc(nthread * 4 * sizeof(uint32_t));
+    mcx_host_seed_words(cfg->seed > 0 ? (unsigned int)cfg->seed : (unsigned int)time(0), nthread * 4, hseed);
+    HIP_TRY(dalloc(bufs, &dnseed, nthread * 4));
+    HIP_TRY(hipMemcpyAsync(dnseed, hseed, nthread * 4 * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+
+    HIP_TRY(dalloc(bufs, &dfield, fieldlen));
+    HIP_TRY(hipMemsetAsync(dfield, 0, fieldlen * sizeof(double), stream));
+    HIP_TRY(dalloc(bufs, &dgenergy, nthread * 2));
+    HIP_TRY(hipMemsetAsync(dgenergy, 0, nthread * 2 * sizeof(float), stream));
+    HIP_TRY(dalloc(bufs, &ddetcount, 1));
+    HIP_TRY(hipMemsetAsync(ddetcount, 0, sizeof(uint32_t), stream));
+
+    if (cfg->issavedet) {
+        HIP_TRY(dalloc(bufs, &ddetrec, (size_t)cfg->maxdetphoton * (reclen ? reclen : 1)));
+
+        if (cfg->issaveseed) {
+            HIP_TRY(dalloc(bufs, &ddetseed, (size_t)cfg->maxdetphoton * 2));
+        }
+    }
+
+    g.media = dmedia;
+    g.prop = dprop;
+    g.detpos = ddetpos;
+    g.invcdf = dinvcdf;
+    g.srcdata = dsrcdata;
+    g.replayweight = dreplayw;
+    g.replaytof = dreplaytof;
+    g.replaydetid = dreplaydetid;
+    g.replayseed = dreplayseed;
+    g.nseed = dnseed;
+    g.field = dfield;
+    g.genergy = dgenergy;
+    g.detrec = ddetrec;
+    g.detseed = ddetseed;
+    g.detcount = ddetcount;
+
+    HIP_TRY(hipStreamSynchronize(stream));
+    t0 = wall_ms();
+    HIP_TRY(hipEventRecord(ev0, stream));
+
+    if (g.debugrng) {
+        HIP_TRY(mcx_launch_rngdump(&g, (int)(nthread / nblocksize), (int)nblocksize, stream));
+    } else if (cfg->mathmode == 1) {
+        HIP_TRY(mcx_launch_parity(&g, (int)(nthread / nblocksize), (int)nblocksize, shmem, stream, cfg->issavedet != 0,
+                                  reflect));
+    } else {
+        HIP_TRY(mcx_launch_fast(&g, (int)(nthread / nblocksize), (int)nblocksize, shmem, stream, cfg->issavedet != 0,
+                                reflect));
+    }
+
+    HIP_TRY(hipEventRecord(ev1, stream));
+    HIP_TRY(hipEventSynchronize(ev1));
+    t1 = wall_ms();
+    HIP_TRY(hipEventElapsedTime(&kernelms, ev0, ev1));
+    cfg->kernelms = kernelms;
+    cfg->runtimems = t1 - t0;
+    cfg->runtime = (unsigned int)(t1 - t0 + 0.5);
+
+    /* energy tallies, summed in thread order in double (src/mcx_host.cpp:1299-1306) */
+    hgenergy = (float*)malloc(nthread * 2 * sizeof(float));
+    HIP_TRY(hipMemcpy(hgenergy, dgenergy, nthread * 2 * sizeof(float), hipMemcpyDeviceToHost));
+
+    for (size_t i = 0; i < nthread; i++) {
+        cfg->energyesc += hgenergy[i * 2];
+        cfg->energytot += hgenergy[i * 2 + 1];
+    }
+
+    cfg->energyabs = cfg->energytot - cfg->energyesc;
+
+    /* detected photons (src/mcx_host.cpp:1195-1237) */
+    if (cfg->issavedet) {
+        HIP_TRY(hipMemcpy(&hdetcount, ddetcount, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        cfg->his.detected = hdetcount;
+
+        if (hdetcount > cfg->maxdetphoton) {
+            fprintf(cfg->flog ? cfg->flog : stdout,
+                    "WARNING: the detected photon number is more than what your have specified (%u > %u), please use "
+                    "the -H option to specify a greater number\n", hdetcount, cfg->maxdetphoton);
+        }
+
+        uint32_t nsave = hdetcount < cfg->maxdetphoton ? hdetcount : cfg->maxdetphoton;
+        cfg->exportdetected = (float*)realloc(cfg->exportdetected, (size_t)(nsave ? nsave : 1) * reclen * sizeof(float));
+
+        if (nsave && reclen) {
+            HIP_TRY(hipMemcpy(cfg->exportdetected, ddetrec, (size_t)nsave * reclen * sizeof(float), hipMemcpyDeviceToHost));
+        }
+
+        if (cfg->issaveseed) {
+            cfg->seeddata = realloc(cfg->seeddata, (size_t)(nsave ? nsave : 1) * 16);
+
+            if (nsave) {
+                HIP_TRY(hipMemcpy(cfg->seeddata, ddetseed, (size_t)nsave * 16, hipMemcpyDeviceToHost));
+            }
+        }
+
+        cfg->detectedcount = nsave;
+    }
+
+    /* fluence: double accumulator -> float, normalised on the device
+       (src/mcx_host.cpp:1382-1465, src/mcx_utils.c:1208-1218) */
+    if (cfg->issave2pt) {
+        float scale = 1.f;
+        int normalize = cfg->isnormalized && !g.debugrng;
+        float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
+
+        if (normalize) {
+            if (cfg->outputtype == otFlux || cfg->outputtype == otFluence) {
+                scale = cfg->unitinmm / (float)(cfg->energytot * Vvox * cfg->tstep);
+
+                if (cfg->outputtype == otFluence) {
+                    scale *= cfg->tstep;
+                }
+            } else if (cfg->outputtype == otEnergy || cfg->outputtype == otL) {
+                scale = 1.f / (float)cfg->energytot;
+            } else if (cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS) {
+                if (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
+                    scale = 0.f;   /* per-detector scaling below */
+                } else {
+                    float sw = 0.f;
+
+                    for (size_t i = 0; i < cfg->nphoton; i++) {
+                        sw += cfg->replay.weight[i];
+                    }
+
+                    scale = sw > 0.f ? cfg->unitinmm / sw : 0.f;
+                }
+            }
+
+            if (cfg->extrasrclen && cfg->srcid < 0) {
+                scale *= (float)(cfg->extrasrclen + 1);
+            }
+
+            if (scale != 0.f) {
+                cfg->normalizer = scale;
+                cfg->his.normalizer = scale;
+            }
+        }
+
+        if (cfg->exportfield64) {
+            HIP_TRY(hipMemcpy(cfg->exportfield64, dfield, fieldlen * sizeof(double), hipMemcpyDeviceToHost));
+        }
+
+        HIP_TRY(dalloc(bufs, &dout, fieldlen));
+        HIP_TRY(mcx_launch_finalize(dfield, dout, fieldlen, (normalize && scale != 0.f) ? scale : 1.f, stream));
+        hfield = (float*)malloc(fieldlen * sizeof(float));
+        HIP_TRY(hipMemcpyAsync(hfield, dout, fieldlen * sizeof(float), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+
+        if (normalize && scale == 0.f && cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
+            for (int d = 1; d <= (int)cfg->detnum; d++) {
+                float sd = 0.f;
+
+                for (size_t i = 0; i < cfg->nphoton; i++)
+                    if ((cfg->replay.detid[i] & 0xFFFF) == d) {
+                        sd += cfg->replay.weight[i];
+                    }
+
+                if (sd > 0.f) {
+                    sd = cfg->unitinmm / sd;
+                }
+
+                mcx_normalize(hfield + (size_t)(d - 1) * nvox * maxgate, sd, (int)(nvox * maxgate), 1, 0, 1);
+            }
+        }
+
+        if (!cfg->exportfield) {
+            cfg->exportfield = (float*)calloc(fieldlen * 2, sizeof(float));
+        }
+
+        for (size_t i = 0; i < fieldlen; i++) {
+            cfg->exportfield[i] += hfield[i];
+        }
+    }
+
+    free(hseed);
+    free(hgenergy);
+    free(hfield);
+    bufs.release();
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    (void)hipStreamDestroy(stream);
+    return;
+
+fail:
+    free(hseed);
+    free(hgenergy);
+    free(hfield);
+    bufs.release();
+
+    if (ev0) {
+        (void)hipEventDestroy(ev0);
+    }
+
+    if (ev1) {
+        (void)hipEventDestroy(ev1);
+    }
+
+    if (stream) {
+        (void)hipStreamDestroy(stream);
+    }
+
+    snprintf(msg, sizeof(msg), "HIP error %d (%s) at %s:%d", (int)err, hipGetErrorString(err), __FILE__, errline);
+    mcx_error(-(int)err, msg, __FILE__, errline);
+}
+
+extern "C" int mcx_try_run_simulation(Config* cfg) {
+    jmp_buf jb;
+    void* old = mcx_push_errjmp(&jb);
+    int rc = setjmp(jb);
+
+    if (rc == 0) {
+        mcx_run_simulation(cfg, NULL, NULL);
+    }
+
+    mcx_pop_errjmp(old);
+    return rc;
+}
+
+/* ---------------- device self-checks ---------------- */
+extern "C" int mcx_device_fpmath(int gpuid, int which, int mode, const float* x, float* y, size_t n) {
+    float *dx = NULL, *dy = NULL;
+    size_t nin = (which == 6) ? 2 * n : n;
+    hipError_t e = hipSetDevice(gpuid);
+
+    if (e == hipSuccess) {
+        e = hipMalloc((void**)&dx, nin * sizeof(float));
+    }
+
+    if (e == hipSuccess) {
+        e = hipMalloc((void**)&dy, n * sizeof(float));
+    }
+
+    if (e == hipSuccess) {
+        e = hipMemcpy(dx, x, nin * sizeof(float), hipMemcpyHostToDevice);
+    }
+
+    if (e == hipSuccess) {
+        e = mode ? mcx_launch_fpmath_parity(which, dx, dy, n, 0) : mcx_launch_fpmath_fast(which, dx, dy, n, 0);
+    }
+
+    if (e == hipSuccess) {
+        e = hipMemcpy(y, dy, n * sizeof(float), hipMemcpyDeviceToHost);
+    }
+
+    if (dx) {
+        (void)hipFree(dx);
+    }
+
+    if (dy) {
+        (void)hipFree(dy);
+    }
+
+    return (int)e;
+}
+
+extern "C" int mcx_device_rng_stream(int gpuid, const unsigned int seed4[4], unsigned int n, float* out) {
+    uint32_t* ds = NULL;
+    float* dout = NULL;
+    hipError_t e = hipSetDevice(gpuid);
+
+    if (e == hipSuccess) {
+        e = hipMalloc((void**)&ds, 4 * sizeof(uint32_t));
+    }
+
+    if (e == hipSuccess) {
+        e = hipMalloc((void**)&dout, n * sizeof(float));
+    }
+
+    if (e == hipSuccess) {
+        e = hipMemcpy(ds, seed4, 4 * sizeof(uint32_t), hipMemcpyHostToDevice);
+    }
+
+    if (e == hipSuccess) {
+        e = mcx_launch_rngstream(ds, n, dout, 0);
+    }
+
+    if (e == hipSuccess) {
+        e = hipMemcpy(out, dout, n * sizeof(float), hipMemcpyDeviceToHost);
+    }
+
+    if (ds) {
+        (void)hipFree(ds);
+    }
+
+    if (dout) {
+        (void)hipFree(dout);
+    }
+
+    return (int)e;
+}

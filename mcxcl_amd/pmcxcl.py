@@ -1,0 +1,367 @@
+"""pmcxcl-compatible Python interface: ``run(cfg)``, ``gpuinfo()``, ``version()``.
+
+Mirrors the reference binding (src/pmcxcl.cpp): the same configuration keys
+(parse_config, src/pmcxcl.cpp:428-1117), the same volume dtype rules
+(parseVolume, :93-427), the same call sequence (initcfg -> validatecfg ->
+run_simulation -> clearcfg, :1124-1553) and the same outputs: ``flux``
+(Fortran-ordered ``[Nx, Ny, Nz, Nt(, Ndet)]`` float32), ``detp``
+(``[reclen, ndetected]``), ``seeds`` (``[16, ndetected]`` uint8), ``dref``,
+``vol`` (with ``dumpmask``) and ``stat``.  Errors raise ``ValueError`` /
+``RuntimeError`` as the reference binding does.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from ._native import lib, Config, GPUInfo
+
+SRCTYPES = ["pencil", "isotropic", "cone", "gaussian", "planar", "pattern", "fourier", "arcsine", "disk",
+            "fourierx", "fourierx2d", "zgaussian", "line", "slit", "pencilarray", "pattern3d", "hyperboloid", "ring"]
+# pmcxcl output-type names (src/pmcxcl.cpp:854-877); indices >= 5 shift down by 2
+OUTPUTTYPES = ["flux", "fluence", "energy", "jacobian", "nscat", "wl", "wp", "wm", "rf", "length", "rfmus",
+               "wltof", "wptof", "adjoint", "adjoint_dcoeff", "adjoint_mus", "adjoint_musp", "adjoint_mua_d",
+               "adjoint_mua_musp"]
+SAVEFLAGS = "DSPMXVWI"
+DEBUGFLAGS = "RMP"
+SEED_FROM_FILE = -999
+
+_INT_FIELDS = ["nphoton", "nblocksize", "nthread", "maxdetphoton", "maxgate", "respin", "isreflect", "isref3",
+               "isrefint", "isnormalized", "isgpuinfo", "issrcfrom0", "autopilot", "printnum", "voidtime",
+               "issaveref", "issaveexit", "ismomentum", "isspecular", "istrajstokes", "replaydet", "faststep",
+               "maxvoidstep", "maxjumpdebug", "internalsrc", "gscatter", "srcnum", "srcid", "optlevel", "isatomic",
+               "issave2pt", "issavedet", "issaveseed", "mathmode"]
+_FLOAT_FIELDS = ["tstart", "tstep", "tend", "sradius", "minenergy", "unitinmm", "omega", "lambda"]
+
+
+def _set_vec(dst, values, n):
+    vals = [float(v) for v in np.asarray(values, dtype=np.float64).ravel()[:n]]
+    for i, v in enumerate(vals):
+        setattr(dst, "xyzw"[i], v)
+
+
+def _carray(arr, dtype):
+    """Copy a numpy array into C-allocated memory (freed by mcx_clearcfg)."""
+    a = np.ascontiguousarray(arr, dtype=dtype)
+    ptr = _native.cmalloc(a.nbytes)
+    ctypes.memmove(ptr, a.ctypes.data, a.nbytes)
+    return ptr
+
+
+def _parse_volume(user, c):
+    """reference parseVolume (src/pmcxcl.cpp:93-427): dtype selects the media format"""
+    if "vol" not in user:
+        raise ValueError("Configuration must specify a 2/3/4D volume.")
+    vol = np.asarray(user["vol"])
+    if vol.ndim == 4:
+        raise ValueError("4-D (multi-word) volumes are not supported by this build")
+    if vol.ndim != 3:
+        raise ValueError("the 'vol' field must be a 3-D array")
+    if vol.dtype in (np.uint8, np.int8):
+        c.mediabyte = 1
+    elif vol.dtype in (np.uint16, np.int16):
+        c.mediabyte = 2
+    elif vol.dtype in (np.uint32, np.int32):
+        c.mediabyte = 4
+    else:
+        raise ValueError("unsupported volume data type %s (use an integer label volume)" % vol.dtype)
+    c.dim.x, c.dim.y, c.dim.z = (int(d) for d in vol.shape)
+    flat = np.asfortranarray(vol).ravel(order="F")
+    flat = flat.view(np.uint8).astype(np.uint32) if flat.dtype == np.int8 else flat
+    flat = flat.view(np.uint16).astype(np.uint32) if flat.dtype == np.int16 else flat
+    c.vol = _carray(flat.astype(np.uint32), np.uint32)
+
+
+def _parse_srcvec(user, c, key):
+    """srcpos/srcdir/srcparam1/srcparam2: one row, or one row per source (extra sources)"""
+    a = np.atleast_2d(np.asarray(user[key], dtype=np.float32))
+    if a.shape[1] < 3 and key in ("srcpos", "srcdir"):
+        raise ValueError("the '%s' field must have 3 or 4 columns" % key)
+    if a.shape[1] > 4:
+        raise ValueError("the '%s' field must have at most 4 columns" % key)
+    _set_vec(getattr(c, key), a[0], a.shape[1])
+    if a.shape[0] > 1 or c.extrasrclen > 0:
+        nextra = a.shape[0] - 1
+        if c.extrasrclen and c.extrasrclen != nextra:
+            raise ValueError("Length of sub-elements of srcpos/srcdir/srcparam1/srcparam2 must match")
+        c.extrasrclen = nextra
+        if not c.srcdata:
+            c.srcdata = _native.cmalloc(64 * max(1, nextra))
+        base = (ctypes.c_float * (16 * nextra)).from_address(c.srcdata)
+        slot = {"srcpos": 0, "srcdir": 4, "srcparam1": 8, "srcparam2": 12}[key]
+        for i in range(nextra):
+            for j in range(a.shape[1]):
+                base[i * 16 + slot + j] = float(a[i + 1, j])
+
+
+def _parse_config(user, c):
+    """reference parse_config (src/pmcxcl.cpp:428-1117)"""
+    for k in _INT_FIELDS:
+        if k in user:
+            v = int(user[k])
+            if isinstance(getattr(c, k), bytes):
+                setattr(c, k, bytes([v & 0xFF]))
+            else:
+                setattr(c, k, v)
+    for k in _FLOAT_FIELDS:
+        if k in user:
+            setattr(c, k, float(user[k]))
+    for k in ("steps", "crop0", "crop1"):
+        if k in user:
+            v = np.asarray(user[k]).ravel()
+            for i, name in enumerate("xyz"):
+                setattr(getattr(c, k), name, type(getattr(getattr(c, k), name))(v[i]))
+    if "srciquv" in user:
+        _set_vec(c.srciquv, user["srciquv"], 4)
+    _parse_volume(user, c)
+    for key in ("srcpos", "srcdir", "srcparam1", "srcparam2"):
+        if key in user:
+            _parse_srcvec(user, c, key)
+    if "detpos" in user:
+        d = np.atleast_2d(np.asarray(user["detpos"], dtype=np.float32))
+        if d.size and d.shape[1] != 4:
+            raise ValueError("the 'detpos' field must have 4 columns (x,y,z,radius)")
+        c.detnum = d.shape[0] if d.size else 0
+        if c.detnum:
+            c.detpos = _carray(d, np.float32)
+    if "detdir" in user:
+        d = np.atleast_2d(np.asarray(user["detdir"], dtype=np.float32))
+        if d.shape[1] != 4:
+            raise ValueError("the 'detdir' field must have 4 columns (vx,vy,vz,focallength)")
+        if c.detnum and d.shape[0] != c.detnum:
+            raise ValueError("the number of rows in 'detdir' must match 'detpos'")
+        c.detdir = _carray(d, np.float32)
+    if "prop" in user:
+        p = np.atleast_2d(np.asarray(user["prop"], dtype=np.float32))
+        if p.shape[1] != 4:
+            raise ValueError("the 'prop' field must have 4 columns (mua,mus,g,n)")
+        c.medianum = p.shape[0]
+        c.prop = _carray(p, np.float32)
+    if "session" in user:
+        s = str(user["session"]).encode()
+        if len(s) >= 64:
+            raise ValueError("the 'session' field is too long")
+        c.session = s
+    if "srctype" in user:
+        st = str(user["srctype"]).lower()
+        if st not in SRCTYPES:
+            raise ValueError("the specified source type is not supported")
+        c.srctype = bytes([SRCTYPES.index(st)])
+    if "outputtype" in user:
+        ot = str(user["outputtype"]).lower()
+        if ot not in OUTPUTTYPES:
+            raise ValueError("the specified output type is not supported")
+        idx = OUTPUTTYPES.index(ot)
+        if idx >= 5:
+            idx -= 2
+        c.outputtype = bytes([idx])
+    if "debuglevel" in user:
+        lvl = 0
+        for ch in str(user["debuglevel"]).upper():
+            if ch in DEBUGFLAGS:
+                lvl |= 1 << DEBUGFLAGS.index(ch)
+        if lvl == 0:
+            raise ValueError("the specified debuglevel is not supported")
+        c.debuglevel = lvl
+    if "savedetflag" in user:
+        v = user["savedetflag"]
+        if isinstance(v, str):
+            flag = 0
+            for ch in v.upper():
+                if ch in SAVEFLAGS:
+                    flag |= 1 << SAVEFLAGS.index(ch)
+            c.savedetflag = flag
+        else:
+            c.savedetflag = int(v)
+    if "invcdf" in user:
+        v = np.asarray(user["invcdf"], dtype=np.float32).ravel()
+        if np.any(np.diff(v) < 0) or np.any(v > 1) or np.any(v < -1):
+            raise ValueError("cfg.invcdf contains invalid data; it must be a monotonically increasing vector "
+                             "with all values between -1 and 1")
+        full = np.concatenate([[-1.0], v, [1.0]]).astype(np.float32)
+        c.nphase = full.size
+        c.invcdf = _carray(full, np.float32)
+    if "angleinvcdf" in user:
+        raise ValueError("angleinvcdf launch distributions are not supported by this build")
+    if "shapes" in user:
+        s = str(user["shapes"]).encode()
+        ptr = _native.cmalloc(len(s) + 2)
+        ctypes.memmove(ptr, s, len(s))
+        c.shapedata = ptr
+    if "bc" in user:
+        s = str(user["bc"]).encode()
+        if not s or len(s) > 12:
+            raise ValueError("the 'bc' field must be a non-empty string of at most 12 characters")
+        c.bc = s
+    if "seed" in user:
+        sv = user["seed"]
+        if np.isscalar(sv):
+            c.seed = int(sv)
+        else:
+            a = np.asarray(sv, dtype=np.uint8)
+            if a.ndim != 2 or a.shape[0] != 16:
+                raise ValueError("the row number of cfg.seed does not match RNG seed byte-length")
+            c.replay.seed = _carray(np.asfortranarray(a).ravel(order="F"), np.uint8)
+            c.seed = SEED_FROM_FILE
+            c.nphoton = a.shape[1]
+    if "gpuid" in user:
+        gid = int(user["gpuid"])
+        if gid <= 0 or gid >= 256:
+            raise ValueError("GPU id must be positive and can not be more than 256")
+        c.gpuid = gid
+    if "workload" in user:
+        w = np.asarray(user["workload"], dtype=np.float32).ravel()
+        for i, v in enumerate(w[:256]):
+            c.workload[i] = float(v)
+
+
+class PreparedConfig:
+    """A validated Config (reference parse_config + mcx_validatecfg) that owns its
+    C memory; release it with ``close()`` (mcx_clearcfg)."""
+
+    def __init__(self, user):
+        self.user = user
+        self.c = Config()
+        lib.mcx_initcfg(ctypes.byref(self.c))
+        self.c.parentid = 2  # mpPython
+        self._keep = []
+        try:
+            _parse_config(user, self.c)
+            detps_ptr, dims, seedbyte = None, None, 0
+            if "detphotons" in user:
+                dp = np.asfortranarray(np.asarray(user["detphotons"], dtype=np.float32))
+                self._keep.append(dp)
+                detps_ptr = dp.ctypes.data
+                dims = (ctypes.c_int * 2)(dp.shape[0], dp.shape[1])
+                seedbyte = 16 if self.c.replay.seed else 0
+            rc = lib.mcx_try_validatecfg(ctypes.byref(self.c), detps_ptr, dims, seedbyte)
+            if rc:
+                raise ValueError(_native.last_error())
+            if not self.c.vol or self.c.medianum == 0:
+                raise ValueError("You must define 'vol' and 'prop' field.")
+        except Exception:
+            self.close()
+            raise
+        c = self.c
+        self.ngate = int((c.tend - c.tstart) / c.tstep + 0.5)
+        self.nvox = c.dim.x * c.dim.y * c.dim.z
+        self.ndet = c.detnum if (c.seed == SEED_FROM_FILE and c.replaydet == -1) else 1
+        self.nsrc = (c.extrasrclen + 1) if (c.extrasrclen and c.srcid < 0) else 1
+        self.fieldlen = self.nvox * self.ngate * self.ndet * self.nsrc
+        self.raw = None
+        if user.get("rawfield"):
+            # unnormalised double-precision accumulator, returned as "flux64"
+            self.raw = np.zeros(self.fieldlen, dtype=np.float64)
+            c.exportfield64 = self.raw.ctypes.data
+
+    def collect(self):
+        out = _collect(self.c, self.user, self.ngate, self.ndet, self.nsrc)
+        if self.raw is not None:
+            dims = [self.c.dim.x, self.c.dim.y, self.c.dim.z, self.ngate * self.ndet * self.nsrc]
+            out["flux64"] = self.raw.reshape(dims, order="F").copy()
+        return out
+
+    def close(self):
+        if self.c is not None:
+            self.c.exportfield64 = None
+            lib.mcx_clearcfg(ctypes.byref(self.c))
+            self.c = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def prepare(cfg=None, **kwargs):
+    user = dict(cfg or {})
+    user.update(kwargs)
+    return PreparedConfig(user)
+
+
+def run(cfg=None, **kwargs):
+    """Run one simulation; ``cfg`` is a dict of pmcxcl keys (or pass them as kwargs).
+
+    Keys beyond the reference's: ``mathmode`` (0: hardware transcendentals,
+    1: bit-reproducible arithmetic), ``rawfield`` (also return the unnormalised
+    double-precision accumulator as ``flux64``)."""
+    with prepare(cfg, **kwargs) as pc:
+        c = pc.c
+        if c.issave2pt:
+            c.exportfield = _native.cmalloc(4 * pc.fieldlen)
+        rc = lib.mcx_try_run_simulation(ctypes.byref(c))
+        if rc:
+            raise RuntimeError("MCX terminated due to an exception: " + _native.last_error())
+        return pc.collect()
+
+
+def _collect(c, user, ngate, ndet, nsrc):
+    out = {}
+    reclen = lib.mcx_detreclen(ctypes.byref(c))
+    n = c.detectedcount
+    if c.issaveseed and c.seeddata and c.issavedet:
+        seeds = np.ctypeslib.as_array((ctypes.c_uint8 * (16 * n)).from_address(c.seeddata)).copy()
+        out["seeds"] = seeds.reshape(n, 16).T.copy(order="F")
+    if user.get("dumpmask"):
+        vol = np.ctypeslib.as_array((ctypes.c_uint32 * (c.dim.x * c.dim.y * c.dim.z)).from_address(c.vol)).copy()
+        out["vol"] = vol.reshape((c.dim.x, c.dim.y, c.dim.z), order="F")
+    if c.issavedet and n > 0 and reclen > 0:
+        det = np.ctypeslib.as_array((ctypes.c_float * (reclen * n)).from_address(c.exportdetected)).copy()
+        out["detp"] = np.asfortranarray(det.reshape(n, reclen).T)
+    if c.issave2pt and c.exportfield:
+        dims = [c.srcnum * c.dim.x, c.dim.y, c.dim.z, ngate]
+        if ndet > 1:
+            dims.append(ndet)
+        if nsrc > 1:
+            dims += [1] * (5 - len(dims)) + [nsrc]
+        total = int(np.prod(dims))
+        flux = np.ctypeslib.as_array((ctypes.c_float * total).from_address(c.exportfield)).copy()
+        flux = flux.reshape(dims, order="F")
+        if c.issaveref:
+            volmask = np.ctypeslib.as_array(
+                (ctypes.c_uint32 * (c.dim.x * c.dim.y * c.dim.z)).from_address(c.vol)).copy()
+            inside = (volmask & 0x7FFFFFFF).reshape((c.dim.x, c.dim.y, c.dim.z), order="F") > 0
+            fl = flux.reshape((c.dim.x, c.dim.y, c.dim.z, -1), order="F")
+            dref = np.where(inside[..., None], 0.0, -fl).astype(np.float32)
+            fl[~inside] = 0.0
+            out["dref"] = dref.reshape(dims, order="F")
+            flux = fl.reshape(dims, order="F")
+        out["flux"] = flux
+    out["stat"] = {
+        "runtime": int(c.runtime),
+        "nphoton": int(c.nphoton) * (c.respin if c.respin > 1 else 1),
+        "energytot": float(c.energytot),
+        "energyabs": float(c.energyabs),
+        "normalizer": float(c.normalizer),
+        "unitinmm": float(c.unitinmm),
+        "workload": [1.0],
+        "kernelms": float(c.kernelms),
+        "runtimems": float(c.runtimems),
+        "detected": int(c.his.detected),
+    }
+    return out
+
+
+def gpuinfo():
+    """List the HIP devices (reference pmcxcl.gpuinfo, src/pmcxcl.cpp:1591-1646)."""
+    c = Config()
+    lib.mcx_initcfg(ctypes.byref(c))
+    active = ctypes.c_uint(0)
+    devs = (ctypes.c_int * 256)()
+    info = ctypes.POINTER(GPUInfo)()
+    n = lib.mcx_list_gpu(ctypes.byref(c), ctypes.byref(active), devs, ctypes.byref(info))
+    res = []
+    for i in range(n):
+        g = info[i]
+        res.append({k: (getattr(g, k).decode() if k == "name" else getattr(g, k))
+                    for k, _ in GPUInfo._fields_})
+    if n:
+        lib.mcx_cleargpuinfo(ctypes.byref(info))
+    lib.mcx_clearcfg(ctypes.byref(c))
+    return res
+
+
+def version():
+    return lib.mcx_version_string().decode()

@@ -1,0 +1,93 @@
+"""Benchmark and validation inputs as pmcxcl-style configuration dicts.
+
+``bench`` mirrors the reference package's built-in dicts (pmcxcl/pmcxcl/bench.py:22-46:
+cube60, cube60b, cube60planar).  The ``*_workload`` builders produce the
+benchmark workloads measured by bench.py and exercised by the tests; every
+volume is generated from fixed seeds by this module.
+"""
+
+import copy
+
+import numpy as np
+
+CUBE60_PROP = [[0, 0, 1, 1], [0.005, 1, 0.01, 1.37], [0.002, 5, 0.9, 1]]
+CUBE60_DETS = [[29, 19, 0, 1], [29, 39, 0, 1], [19, 29, 0, 1], [39, 29, 0, 1]]
+
+bench = {}
+bench["cube60"] = {
+    "nphoton": 1000000,
+    "vol": np.ones([60, 60, 60], dtype="uint8"),
+    "tstart": 0,
+    "tend": 5e-9,
+    "tstep": 5e-9,
+    "srcpos": [29, 29, 0],
+    "srcdir": [0, 0, 1],
+    "prop": CUBE60_PROP,
+    "isreflect": 0,
+    "seed": 1648335518,
+    "session": "cube60",
+    "detpos": CUBE60_DETS,
+    "issrcfrom0": 1,
+}
+bench["cube60b"] = copy.deepcopy(bench["cube60"])
+bench["cube60b"]["isreflect"] = 1
+bench["cube60planar"] = copy.deepcopy(bench["cube60b"])
+bench["cube60planar"]["srctype"] = "planar"
+bench["cube60planar"]["srcpos"] = [10, 10, -10]
+bench["cube60planar"]["srcparam1"] = [40, 0, 0, 0]
+bench["cube60planar"]["srcparam2"] = [0, 40, 0, 0]
+
+
+def sphere_volume(dim=60, center=(30.0, 30.0, 30.0), radius=15.0, inside=2, outside=1):
+    """Label volume with a ball, voxel-centre test |(i+.5,j+.5,k+.5)-O|<=R as the
+    reference Sphere rasteriser (src/mcx_shapes.c:249-263), Fortran-indexed [x,y,z]."""
+    c = np.arange(dim, dtype=np.float32) + np.float32(0.5)
+    dx = c[:, None, None] - np.float32(center[0])
+    dy = c[None, :, None] - np.float32(center[1])
+    dz = c[None, None, :] - np.float32(center[2])
+    r2 = np.float32(radius) * np.float32(radius)
+    ball = (dx * dx + dy * dy + dz * dz) <= r2
+    vol = np.full((dim, dim, dim), outside, dtype=np.uint8)
+    vol[ball] = inside
+    return vol
+
+
+def homogeneous_cube_workload(nphoton=1000000, seed=1648335518, **over):
+    """60^3 uint8 label-1 cube, 1 mm voxels, mua=0.005 mus=1 g=0.01 n=1.37 in n=1
+    background with boundary reflection, pencil beam at (29,29,0) along +z, one
+    0-5 ns gate, float32 fluence-rate output, no detectors."""
+    cfg = {
+        "nphoton": int(nphoton),
+        "vol": np.ones([60, 60, 60], dtype=np.uint8),
+        "tstart": 0.0,
+        "tend": 5e-9,
+        "tstep": 5e-9,
+        "srcpos": [29, 29, 0],
+        "srcdir": [0, 0, 1],
+        "prop": CUBE60_PROP,
+        "isreflect": 1,
+        "seed": int(seed),
+        "issrcfrom0": 1,
+        "issavedet": 0,
+        "session": "cube60b",
+    }
+    cfg.update(over)
+    return cfg
+
+
+def sphere_cube_workload(nphoton=100000000, seed=1648335518, **over):
+    """60^3 cube with a radius-15 ball of label 2 (mua=0.002 mus=5 g=0.9 n=1.0) at
+    (30,30,30); 10 time gates over 0-5 ns; four r=1 mm disk detectors on the z=0 face
+    at +-10 voxels; partial path lengths and exit positions saved (D,P,X)."""
+    cfg = homogeneous_cube_workload(nphoton=nphoton, seed=seed)
+    cfg.update({
+        "vol": sphere_volume(),
+        "tstep": 5e-10,
+        "detpos": CUBE60_DETS,
+        "issavedet": 1,
+        "savedetflag": "DPX",
+        "maxdetphoton": 4000000,
+        "session": "cubesph60b",
+    })
+    cfg.update(over)
+    return cfg

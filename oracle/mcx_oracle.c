@@ -1,0 +1,1499 @@
+/*
+ * mcx_oracle.c - CPU oracle for the photon-transport hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the gfx950 kernel
+ * is compared against; only tests/, the smoke check and bench.py's CPU
+ * baseline load it.  The product (mcxcl_amd) never links or calls it.
+ *
+ * It is a plain, scalar, single-precision restatement of the reference
+ * kernel's photon life cycle, written against the reference's OpenCL
+ * branches (src/mcx_core.cl, not the __NVCC__ ones) and its host driver
+ * (src/mcx_host.cpp:438-1465):
+ *
+ *   oracle_run            <- mcx_run_simulation: thread map, glibc seeding,
+ *                            time-gate loop, read-back, normalisation
+ *   run_thread            <- mcx_main_loop            (src/mcx_core.cl:2307-3307)
+ *   launch_photon         <- launchnewphoton          (src/mcx_core.cl:1466-2275)
+ *   hitgrid               <- hitgrid                  (src/mcx_core.cl:975-995)
+ *   rotatevector          <- rotatevector             (src/mcx_core.cl:1025-1042)
+ *   rotatevector2d        <- rotatevector2d           (src/mcx_core.cl:1008-1023)
+ *   transmit, reflectcoeff<- transmit/reflectcoeff    (src/mcx_core.cl:1044-1075)
+ *   skipvoid              <- skipvoid                 (src/mcx_core.cl:1350-1455)
+ *   save_detected         <- finddetector/savedetphoton (src/mcx_core.cl:838-926)
+ *   rng_next              <- xorshift128p_nextf       (src/mcx_core.cl:684-698)
+ *
+ * Arithmetic: every transcendental goes through the deterministic FP32
+ * functions of mcx_fpmath.h (shared with the kernel's parity mode and pinned
+ * against libm in tests/test_fpmath.py); division and square root are IEEE.
+ * The file is compiled with -ffp-contract=off and without fast-math, so
+ * each float operation rounds exactly once, in source order.
+ *
+ * Tallies: the fluence grid is accumulated in double precision (the "FP64
+ * truth" tally); per-thread escaped/launched energies are float, as in the
+ * reference (src/mcx_core.cl:2403-2404, :3301-3302), summed in double on the
+ * host in thread order (src/mcx_host.cpp:1303-1306).
+ *
+ * Deliberate deviations from the reference (also applied by the kernel); each
+ * removes an out-of-bounds access of the reference and changes nothing else:
+ *  - the deposit gate index is clamped to maxgate-1 (the reference can write
+ *    one gate past the end when (t-twin0)*Rtstep rounds up to maxgate);
+ *  - skipvoid returns "not found" when its 3-step entry search ends outside
+ *    the volume (the reference then reads media[] outside the volume);
+ *  - per-medium detector tallies skip label-0 voxels (the reference indexes
+ *    ppath[-1] there); the diffuse-reflectance deposit skips negative gates;
+ *  - replay photons are indexed idx*threadphoton+min(idx,oddphoton)+k, the
+ *    same map as forward photons; the reference uses min(idx,oddphoton-1)
+ *    (src/mcx_core.cl:1591), which replays one photon twice and reads before
+ *    the seed buffer when oddphoton is 0.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <xmmintrin.h>
+#include <pmmintrin.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "mcx_utils.h"
+#include "mcx_fpmath.h"
+
+#define K_EPS          FLT_EPSILON
+#define JUST_BELOW_ONE 0.9998f
+#define ROULETTE_SIZE  10.f
+#define MAX_ACCUM      1000.f
+
+typedef struct {
+    float x, y, z, w;
+} F4;
+
+typedef struct {
+    short x, y, z, w;
+} S4;
+
+/* event counters used by the roofline model (algorithmic work per photon) */
+typedef struct {
+    double nstep;      /* main-loop iterations */
+    double nscat;      /* scattering events (direction changes) */
+    double ndep;       /* voxel-exit deposits (atomic adds) */
+    double nbnd;       /* Fresnel evaluations at refractive-index changes */
+    double nlaunch;    /* successful photon launches */
+    double nrand;      /* random numbers drawn */
+} OracleCounters;
+
+typedef struct {
+    F4 srcpos, srcdir, srcparam1, srcparam2;
+    F4 maxidx;
+    uint32_t dimx, dimxy, nvox, fieldlen;
+    float twin0, twin1, oneoverc0, Rtstep, minenergy, minaccumtime;
+    uint32_t maxdetphoton, maxmedia, detnum, voidtime, srctype, maxvoidstep, issaveseed, issaveref, isspecular;
+    uint32_t maxgate, outputtype, threadphoton, savedetflag, reclen, partialdata, w0offset, mediaformat, gscatter;
+    uint32_t is2d, srcnum, doreflect, save2pt, savedet, nphase, debuglevel, extrasrclen;
+    int32_t seed, oddphoton, replaydet, srcid;
+    int reflectblock;  /* the reference compiles its reflection block under MCX_DO_REFLECTION */
+    unsigned char bc[12];
+    const F4* prop;
+    const F4* detpos;
+    const uint32_t* media;
+    const float* invcdf;
+    const ExtraSrc* srcdata;
+    const float* replayweight;
+    const float* replaytof;
+    const int* replaydetid;
+    const uint64_t* replayseed;
+} OParam;
+
+typedef struct {
+    double* field;        /* private double field, fieldlen entries */
+    float* det;           /* private detected-photon records */
+    uint64_t* seeds;      /* private detected-photon seeds (2 words each) */
+    size_t ndet, capdet;
+    OracleCounters cnt;
+} OWork;
+
+/* ---------------- RNG (src/mcx_core.cl:684-716) ---------------- */
+static inline float rng_next(uint64_t t[2], OWork* w) {
+    uint64_t s1 = t[0];
+    const uint64_t s0 = t[1];
+    t[0] = s0;
+    s1 ^= s1 << 23;
+    t[1] = s1 ^ s0 ^ (s1 >> 18) ^ (s0 >> 5);
+    uint32_t lo = (uint32_t)(t[1] + s0);
+    w->cnt.nrand += 1;
+    return pm_asfloat(0x3F800000u | (lo >> 9)) - 1.0f;
+}
+
+static inline float rng_scatlen(uint64_t t[2], OWork* w) {
+    return -pm_log(rng_next(t, w) + K_EPS);
+}
+
+/* ---------------- geometry helpers ---------------- */
+static inline float mcx_nextafterf(float a, int dir) {
+    mcx_fbits num;
+    num.f = a + 1000.f;
+    num.u += (uint32_t)(dir ^ (int)(num.u & 0x80000000u));
+    return num.f - 1000.f;
+}
+
+static inline float hitgrid(const F4* p, const F4* v, S4* id) {
+    float hx = pm_fabs((float)id->x + (float)(v->x > 0.f) - p->x);
+    float hy = pm_fabs((float)id->y + (float)(v->y > 0.f) - p->y);
+    float hz = pm_fabs((float)id->z + (float)(v->z > 0.f) - p->z);
+    hx = pm_fabs((hx + K_EPS) / v->x);
+    hy = pm_fabs((hy + K_EPS) / v->y);
+    hz = pm_fabs((hz + K_EPS) / v->z);
+    float dist = fminf(fminf(hx, hy), hz);
+    id->w = (short)(dist == hx ? 0 : (dist == hy ? 1 : 2));
+    return dist;
+}
+
+static inline void rotatevector(F4* v, float stheta, float ctheta, float sphi, float cphi) {
+    if (v->z > -1.f + K_EPS && v->z < 1.f - K_EPS) {
+        float tmp0 = 1.f - v->z * v->z;
+        float tmp1 = stheta * pm_rsqrt(tmp0);
+        F4 n;
+        n.x = tmp1 * (v->x * v->z * cphi - v->y * sphi) + v->x * ctheta;
+        n.y = tmp1 * (v->y * v->z * cphi + v->x * sphi) + v->y * ctheta;
+        n.z = -tmp1 * tmp0 * cphi + v->z * ctheta;
+        n.w = v->w;
+        *v = n;
+    } else {
+        F4 n = {stheta * cphi, stheta * sphi, (v->z > 0.f) ? ctheta : -ctheta, v->w};
+        *v = n;
+    }
+
+    v->x *= pm_rsqrt(v->x * v->x + v->y * v->y + v->z * v->z);
+    v->y *= pm_rsqrt(v->x * v->x + v->y * v->y + v->z * v->z);
+    v->z *= pm_rsqrt(v->x * v->x + v->y * v->y + v->z * v->z);
+}
+
+static inline void rotatevector2d(F4* v, float stheta, float ctheta, int is2d) {
+    F4 n = *v;
+
+    if (is2d == 1) {
+        n.x = 0.f;
+        n.y = v->y * ctheta - v->z * stheta;
+        n.z = v->y * stheta + v->z * ctheta;
+    } else if (is2d == 2) {
+        n.x = v->x * ctheta - v->z * stheta;
+        n.y = 0.f;
+        n.z = v->x * stheta + v->z * ctheta;
+    } else if (is2d == 3) {
+        n.x = v->x * ctheta - v->y * stheta;
+        n.y = v->x * stheta + v->y * ctheta;
+        n.z = 0.f;
+    }
+
+    *v = n;
+    float tmp0 = pm_rsqrt(v->x * v->x + v->y * v->y + v->z * v->z);
+    v->x *= tmp0;
+    v->y *= tmp0;
+    v->z *= tmp0;
+}
+
+static inline void transmit(F4* v, float n1, float n2, int flipdir) {
+    float tmp0 = n1 / n2;
+    v->x *= tmp0;
+    v->y *= tmp0;
+    v->z *= tmp0;
+
+    if (flipdir == 0) {
+        v->x = pm_sqrt(1.f - v->y * v->y - v->z * v->z) * (float)((v->x > 0.f) - (v->x < 0.f));
+    } else if (flipdir == 1) {
+        v->y = pm_sqrt(1.f - v->x * v->x - v->z * v->z) * (float)((v->y > 0.f) - (v->y < 0.f));
+    } else {
+        v->z = pm_sqrt(1.f - v->x * v->x - v->y * v->y) * (float)((v->z > 0.f) - (v->z < 0.f));
+    }
+}
+
+static inline float reflectcoeff(const F4* v, float n1, float n2, int flipdir) {
+    float Icos = pm_fabs((flipdir == 0) ? v->x : (flipdir == 1 ? v->y : v->z));
+    float tmp0 = n1 * n1;
+    float tmp1 = n2 * n2;
+    float tmp2 = 1.f - tmp0 / tmp1 * (1.f - Icos * Icos);
+
+    if (tmp2 > 0.f) {
+        float Re = tmp0 * Icos * Icos + tmp1 * tmp2;
+        tmp2 = pm_sqrt(tmp2);
+        float Im = 2.f * n1 * n2 * Icos * tmp2;
+        float Rtotal = (Re - Im) / (Re + Im);
+        Re = tmp1 * Icos * Icos + tmp0 * tmp2 * tmp2;
+        return (Rtotal + (Re - Im) / (Re + Im)) * 0.5f;
+    }
+
+    return 1.f;
+}
+
+static inline int inside(const OParam* g, short x, short y, short z) {
+    return (unsigned short)x < g->maxidx.x && (unsigned short)y < g->maxidx.y && (unsigned short)z < g->maxidx.z;
+}
+
+static inline F4 getprop(const OParam* g, uint32_t mediaid) {
+    return g->prop[mediaid & MED_MASK];
+}
+
+/* src/mcx_core.cl:1350-1455 */
+static int skipvoid(const OParam* g, F4* p, F4* v, F4* f, S4* fd) {
+    int count = 1, idx1d;
+    fd->x = (short)floorf(p->x);
+    fd->y = (short)floorf(p->y);
+    fd->z = (short)floorf(p->z);
+    fd->w = -1;
+
+    while (1) {
+        if (inside(g, fd->x, fd->y, fd->z)) {
+            idx1d = fd->z * (int)g->dimxy + fd->y * (int)g->dimx + fd->x;
+
+            if (g->media[idx1d] & MED_MASK) {
+                p->x -= v->x;
+                p->y -= v->y;
+                p->z -= v->z;
+                fd->x = (short)floorf(p->x);
+                fd->y = (short)floorf(p->y);
+                fd->z = (short)floorf(p->z);
+                f->y -= g->minaccumtime;
+                idx1d = fd->z * (int)g->dimxy + fd->y * (int)g->dimx + fd->x;
+                count = 0;
+
+                while (!inside(g, fd->x, fd->y, fd->z) || !(g->media[idx1d] & MED_MASK)) {
+                    float dist = hitgrid(p, v, fd);
+                    f->y += g->minaccumtime * dist;
+                    p->x = p->x + dist * v->x;
+                    p->y = p->y + dist * v->y;
+                    p->z = p->z + dist * v->z;
+
+                    if (fd->w == 0) {
+                        fd->x += (v->x > 0.f ? 1 : -1);
+                    }
+
+                    if (fd->w == 1) {
+                        fd->y += (v->y > 0.f ? 1 : -1);
+                    }
+
+                    if (fd->w == 2) {
+                        fd->z += (v->z > 0.f ? 1 : -1);
+                    }
+
+                    idx1d = fd->z * (int)g->dimxy + fd->y * (int)g->dimx + fd->x;
+
+                    if (count++ > 3) {
+                        break;
+                    }
+                }
+
+                f->y = g->voidtime ? f->y : 0.f;
+
+                if (!inside(g, fd->x, fd->y, fd->z)) {
+                    return -1;   /* entry search failed: the reference would read outside the volume */
+                }
+
+                F4 htime = getprop(g, g->media[idx1d]);
+
+                if (g->isspecular && htime.w != g->prop[0].w) {
+                    p->w *= 1.f - reflectcoeff(v, g->prop[0].w, g->prop[g->media[idx1d] & MED_MASK].w, fd->w);
+
+                    if (p->w > K_EPS) {
+                        transmit(v, g->prop[0].w, g->prop[g->media[idx1d] & MED_MASK].w, fd->w);
+                    }
+                }
+
+                return idx1d;
+            }
+        }
+
+        if ((p->x < 0.f && v->x <= 0.f) || (p->x >= g->maxidx.x && v->x >= 0.f) || (p->y < 0.f && v->y <= 0.f) ||
+                (p->y >= g->maxidx.y && v->y >= 0.f) || (p->z < 0.f && v->z <= 0.f) ||
+                (p->z >= g->maxidx.z && v->z >= 0.f)) {
+            return -1;
+        }
+
+        p->x = p->x + v->x;
+        p->y = p->y + v->y;
+        p->z = p->z + v->z;
+        fd->x = (short)floorf(p->x);
+        fd->y = (short)floorf(p->y);
+        fd->z = (short)floorf(p->z);
+        f->y += g->minaccumtime;
+
+        if ((uint32_t)count++ > g->maxvoidstep) {
+            return -1;
+        }
+    }
+}
+
+typedef struct {
+    uint64_t t[2];
+    uint64_t photonseed[2];
+    F4 p, v, f, prop;
+    S4 fd;
+    uint32_t idx1d, mediaid;
+    float w0, Lmove;
+    /* per-thread detected-photon scratch (src/mcx_host.cpp:486-492 layout):
+       [0, partialdata)   nscat (uint bits) / partial path / momentum per medium
+       [partialdata]      escaped energy tally
+       [partialdata+1]    launched energy tally
+       [partialdata+2]    initial weight of the current photon
+       [partialdata+3]    source id of the current photon */
+    union {
+        float f;
+        uint32_t u;
+    } pp[4 * 256 + 8];
+} OPhoton;
+
+/* src/mcx_core.cl:838-926 */
+static void save_detected(const OParam* g, OWork* w, OPhoton* s, uint32_t isdet) {
+    int detid = 0;
+
+    if (isdet == OUTSIDE_VOLUME_MIN) {
+        detid = -1;
+    } else {
+        for (uint32_t i = 0; i < g->detnum; i++) {
+            if ((g->detpos[i].x - s->p.x) * (g->detpos[i].x - s->p.x) +
+                    (g->detpos[i].y - s->p.y) * (g->detpos[i].y - s->p.y) +
+                    (g->detpos[i].z - s->p.z) * (g->detpos[i].z - s->p.z) < g->detpos[i].w * g->detpos[i].w) {
+                detid = (int)i + 1;
+                break;
+            }
+        }
+    }
+
+    if (!detid) {
+        return;
+    }
+
+    if (w->ndet >= w->capdet) {
+        w->capdet = w->capdet ? w->capdet * 2 : 1024;
+        w->det = (float*)realloc(w->det, w->capdet * g->reclen * sizeof(float));
+        w->seeds = (uint64_t*)realloc(w->seeds, w->capdet * 2 * sizeof(uint64_t));
+    }
+
+    float* rec = w->det + w->ndet * g->reclen;
+    w->seeds[w->ndet * 2] = s->photonseed[0];
+    w->seeds[w->ndet * 2 + 1] = s->photonseed[1];
+    uint32_t k = 0;
+
+    if (SAVE_DETID(g->savedetflag)) {
+        if (g->extrasrclen && g->srcid <= 0) {
+            detid |= ((int)s->pp[g->w0offset - 1].f) << 16;
+        }
+
+        rec[k++] = (float)detid;
+    }
+
+    for (uint32_t i = 0; i < g->partialdata; i++) {
+        memcpy(rec + k++, &s->pp[i].u, 4);
+    }
+
+    if (SAVE_PEXIT(g->savedetflag)) {
+        rec[k++] = s->p.x;
+        rec[k++] = s->p.y;
+        rec[k++] = s->p.z;
+    }
+
+    if (SAVE_VEXIT(g->savedetflag)) {
+        rec[k++] = s->v.x;
+        rec[k++] = s->v.y;
+        rec[k++] = s->v.z;
+    }
+
+    if (SAVE_W0(g->savedetflag)) {
+        rec[k++] = s->pp[g->w0offset - 2].f;
+    }
+
+    w->ndet++;
+}
+
+static inline void deposit(const OParam* g, OWork* w, uint32_t addr, float val) {
+    (void)g;
+    w->field[addr] += (double)val;
+    w->cnt.ndep += 1;
+}
+
+/* time gate of a time-of-flight, clamped to the last gate */
+static inline int gate_of(const OParam* g, float t) {
+    int gate = (int)floorf((t - g->twin0) * g->Rtstep);
+    return gate > (int)g->maxgate - 1 ? (int)g->maxgate - 1 : gate;
+}
+
+static inline uint32_t photon_index(const OParam* g, int idx, float fw) {
+    int odd = g->oddphoton;
+    return (uint32_t)(idx * (int)g->threadphoton + (idx < odd ? idx : odd) + (int)fw);
+}
+
+/* src/mcx_core.cl:1466-2275; returns 0 when a photon was launched, 1 when the
+   thread's photon budget is used up, -1 when no launch succeeded */
+static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_t isdet) {
+    float* ppf = (float*)s->pp;
+    const uint32_t pd = g->partialdata;
+    s->w0 = 1.f;
+    s->Lmove = -1.f;
+    F4 lsrcpos = g->srcpos, lsrcdir = g->srcdir, lp1 = g->srcparam1, lp2 = g->srcparam2;
+
+    if (pm_fabs(s->p.w) >= 0.f) {    /* false only for the NaN of the first call */
+        s->pp[pd].f += s->p.w;
+
+        if (s->mediaid == 0 && s->idx1d != OUTSIDE_VOLUME_MIN && s->idx1d != OUTSIDE_VOLUME_MAX && g->issaveref &&
+                s->p.w > 0.f && g->issaveref == 1) {
+            int tshift = (int)g->maxgate - 1;
+            int t2 = (int)floorf((s->f.y - g->twin0) * g->Rtstep);
+            tshift = tshift < t2 ? tshift : t2;
+
+            if (g->extrasrclen && g->srcid < 0) {
+                tshift += ((int)s->pp[g->w0offset - 1].f - 1) * (int)g->maxgate;
+            }
+
+            if (t2 >= 0) {
+                deposit(g, w, s->idx1d + (uint32_t)tshift * g->nvox, -s->p.w);
+            }
+        }
+
+        if (g->savedet && (isdet & DET_MASK) == DET_MASK && s->mediaid == 0 && g->issaveref < 2) {
+            save_detected(g, w, s, isdet);
+        }
+    }
+
+    if (g->savedet) {
+        for (uint32_t i = 0; i < pd; i++) {
+            s->pp[i].u = 0;
+        }
+    }
+
+    if (s->f.w >= (float)(g->threadphoton + (idx < g->oddphoton))) {
+        return 1;
+    }
+
+    if (g->seed == SEED_FROM_FILE) {
+        uint32_t pid = photon_index(g, idx, s->f.w < 0.f ? 0.f : s->f.w);
+        s->t[0] = g->replayseed[pid * 2];
+        s->t[1] = g->replayseed[pid * 2 + 1];
+    }
+
+    s->photonseed[0] = s->t[0];
+    s->photonseed[1] = s->t[1];
+
+    if (g->extrasrclen && g->srcid != 1) {
+        int sid;
+
+        if (g->srcid > 1) {
+            sid = g->srcid;
+        } else {
+            s->pp[g->w0offset - 1].f = (float)((int)(rng_next(s->t, w) * JUST_BELOW_ONE * (float)(g->extrasrclen + 1)) + 1);
+            sid = (int)s->pp[g->w0offset - 1].f;
+        }
+
+        if (sid > 1) {
+            const ExtraSrc* es = g->srcdata + (sid - 2);
+            memcpy(&lsrcpos, &es->srcpos, 16);
+            memcpy(&lsrcdir, &es->srcdir, 16);
+            memcpy(&lp1, &es->srcparam1, 16);
+            memcpy(&lp2, &es->srcparam2, 16);
+        }
+    }
+
+    if (g->seed == SEED_FROM_FILE && g->srcid >= 1) {
+        rng_next(s->t, w);
+    }
+
+    float* pp = ppf + pd;   /* the reference shifts its ppath pointer here (src/mcx_core.cl:1621) */
+    F4* p = &s->p;
+    F4* v = &s->v;
+    F4* f = &s->f;
+    S4* fd = &s->fd;
+
+    do {
+        *p = lsrcpos;
+        *v = lsrcdir;
+        f->x = 0.f;
+        f->y = 0.f;
+        f->z = g->minaccumtime;
+        s->idx1d = pm_asuint(lp2.z);
+        s->mediaid = pm_asuint(lp2.w);
+        F4 tmp = {lsrcpos.x, lsrcpos.y, lsrcpos.z, 0.f};
+        s->prop = tmp;
+        int st = (int)g->srctype;
+        int positioned = 0;   /* sources that move the launch point recompute idx1d/mediaid */
+
+        if (st == MCX_SRC_PLANAR || st == MCX_SRC_FOURIER || st == MCX_SRC_PENCILARRAY) {
+            float rx = rng_next(s->t, w);
+            float ry = rng_next(s->t, w);
+            p->x = p->x + rx * lp1.x + ry * lp2.x;
+            p->y = p->y + rx * lp1.y + ry * lp2.y;
+            p->z = p->z + rx * lp1.z + ry * lp2.z;
+
+            if (st == MCX_SRC_FOURIER) {
+                p->w = lsrcpos.w * (pm_cos((floorf(lp1.w) * rx + floorf(lp2.w) * ry + lp1.w - floorf(lp1.w)) * TWO_PI) *
+                                    (1.f - lp2.w + floorf(lp2.w)) + 1.f) * 0.5f;
+            } else if (st == MCX_SRC_PENCILARRAY) {
+                p->x = lsrcpos.x + floorf(rx * lp1.w) * lp1.x / (lp1.w - 1.f) + floorf(ry * lp2.w) * lp2.x / (lp2.w - 1.f);
+                p->y = lsrcpos.y + floorf(rx * lp1.w) * lp1.y / (lp1.w - 1.f) + floorf(ry * lp2.w) * lp2.y / (lp2.w - 1.f);
+                p->z = lsrcpos.z + floorf(rx * lp1.w) * lp1.z / (lp1.w - 1.f) + floorf(ry * lp2.w) * lp2.z / (lp2.w - 1.f);
+            }
+
+            positioned = 1;
+            s->prop.x = s->prop.x + (lp1.x + lp2.x) * 0.5f;
+            s->prop.y = s->prop.y + (lp1.y + lp2.y) * 0.5f;
+            s->prop.z = s->prop.z + (lp1.z + lp2.z) * 0.5f;
+            s->prop.w = 0.f;
+        } else if (st == MCX_SRC_DISK || st == MCX_SRC_GAUSSIAN || st == MCX_SRC_RING) {
+            float sphi, cphi, phi, r;
+
+            if (st == MCX_SRC_DISK || st == MCX_SRC_RING) {
+                if (lp1.z > 0.f || lp1.w > 0.f) {
+                    phi = pm_fabs(lp1.z - lp1.w) * rng_next(s->t, w) + fminf(lp1.z, lp1.w);
+                } else {
+                    phi = TWO_PI * rng_next(s->t, w);
+                }
+            } else {
+                phi = TWO_PI * rng_next(s->t, w);
+            }
+
+            pm_sincos(phi, &sphi, &cphi);
+
+            if (st == MCX_SRC_DISK || st == MCX_SRC_RING) {
+                r = pm_sqrt(rng_next(s->t, w) * pm_fabs(lp1.x * lp1.x - lp1.y * lp1.y) + lp1.y * lp1.y);
+            } else if (pm_fabs(lsrcdir.w) < 1e-5f || pm_fabs(lp1.y) < 1e-5f) {
+                r = pm_sqrt(-0.5f * pm_log(rng_next(s->t, w))) * lp1.x;
+            } else {
+                r = lp1.x * lp1.x * ONE_PI / lp1.y;
+                r = pm_sqrt(-0.5f * pm_log(rng_next(s->t, w)) * (1.f + (lsrcdir.w * lsrcdir.w / (r * r)))) * lp1.x;
+            }
+
+            if (v->z > -1.f + K_EPS && v->z < 1.f - K_EPS) {
+                float tmp0 = 1.f - v->z * v->z;
+                float tmp1 = r * pm_rsqrt(tmp0);
+                p->x = p->x + tmp1 * (v->x * v->z * cphi - v->y * sphi);
+                p->y = p->y + tmp1 * (v->y * v->z * cphi + v->x * sphi);
+                p->z = p->z - tmp1 * tmp0 * cphi;
+            } else {
+                p->x += r * cphi;
+                p->y += r * sphi;
+            }
+
+            positioned = 1;
+        } else if (st == MCX_SRC_CONE || st == MCX_SRC_ISOTROPIC || st == MCX_SRC_ARCSINE) {
+            float ang, stheta, ctheta, sphi, cphi;
+            ang = TWO_PI * rng_next(s->t, w);
+            pm_sincos(ang, &sphi, &cphi);
+
+            if (st == MCX_SRC_CONE) {
+                do {
+                    ang = (lp1.y > 0) ? TWO_PI * rng_next(s->t, w) : pm_acos(2.f * rng_next(s->t, w) - 1.f);
+                } while (ang > lp1.x);
+            } else if (st == MCX_SRC_ISOTROPIC) {
+                ang = pm_acos(2.f * rng_next(s->t, w) - 1.f);
+            } else {
+                ang = ONE_PI * rng_next(s->t, w);
+            }
+
+            pm_sincos(ang, &stheta, &ctheta);
+            rotatevector(v, stheta, ctheta, sphi, cphi);
+            s->Lmove = 0.f;
+        } else if (st == MCX_SRC_ZGAUSSIAN) {
+            float ang, stheta, ctheta, sphi, cphi;
+            ang = TWO_PI * rng_next(s->t, w);
+            pm_sincos(ang, &sphi, &cphi);
+            float u1 = rng_next(s->t, w);
+            float u2 = rng_next(s->t, w);
+            ang = pm_sqrt(-2.f * pm_log(u1)) * (1.f - 2.f * u2) * lp1.x;
+            pm_sincos(ang, &stheta, &ctheta);
+            rotatevector(v, stheta, ctheta, sphi, cphi);
+            s->Lmove = 0.f;
+        } else if (st == MCX_SRC_LINE || st == MCX_SRC_SLIT) {
+            float r_l = rng_next(s->t, w);
+            p->x = p->x + r_l * lp1.x;
+            p->y = p->y + r_l * lp1.y;
+            p->z = p->z + r_l * lp1.z;
+
+            if (st == MCX_SRC_LINE) {
+                float sphi_l, cphi_l;
+                r_l = pm_rsqrt(lp1.x * lp1.x + lp1.y * lp1.y + lp1.z * lp1.z);
+
+                if (lp2.x > 0.f) {
+                    float ax = lp1.x * r_l, ay = lp1.y * r_l, az = lp1.z * r_l;
+                    float vdotaxis = v->x * ax + v->y * ay + v->z * az;
+                    v->x = v->x - vdotaxis * ax;
+                    v->y = v->y - vdotaxis * ay;
+                    v->z = v->z - vdotaxis * az;
+                    float vnorm = pm_rsqrt(v->x * v->x + v->y * v->y + v->z * v->z);
+                    v->x = v->x * vnorm;
+                    v->y = v->y * vnorm;
+                    v->z = v->z * vnorm;
+                    r_l = lp2.x * (2.f * rng_next(s->t, w) - 1.f);
+                    pm_sincos(r_l, &sphi_l, &cphi_l);
+                    /* rotate_perpendicular_vector(v, axis, sin, cos) (src/mcx_core.cl:997-1006) */
+                    float cx = ay * v->z - az * v->y;
+                    float cy = az * v->x - ax * v->z;
+                    float cz = ax * v->y - ay * v->x;
+                    v->x = v->x * cphi_l + cx * sphi_l;
+                    v->y = v->y * cphi_l + cy * sphi_l;
+                    v->z = v->z * cphi_l + cz * sphi_l;
+                } else {
+                    v->x = lp1.x * r_l;
+                    v->y = lp1.y * r_l;
+                    v->z = lp1.z * r_l;
+                    r_l = TWO_PI * rng_next(s->t, w);
+                    pm_sincos(r_l, &sphi_l, &cphi_l);
+                    rotatevector(v, 1.f, 0.f, sphi_l, cphi_l);
+                }
+            } else if (lp2.x > 0.f || lp2.y > 0.f) {
+                float sphi_s, cphi_s;
+                r_l = TWO_PI * rng_next(s->t, w);
+                pm_sincos(r_l, &sphi_s, &cphi_s);
+                r_l = pm_sqrt(2.f * rng_scatlen(s->t, w));
+                cphi_s *= lp2.x * r_l;
+                sphi_s *= lp2.y * r_l;
+                sphi_s *= pm_rsqrt(lp1.x * lp1.x + lp1.y * lp1.y + lp1.z * lp1.z);
+                F4 c = {lp1.y * v->z - lp1.z * v->y, lp1.z * v->x - lp1.x * v->z, lp1.x * v->y - lp1.y * v->x, 0.f};
+                s->prop = c;
+                cphi_s *= pm_rsqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+                v->x += cphi_s * c.x + sphi_s * lp1.x;
+                v->y += cphi_s * c.y + sphi_s * lp1.y;
+                v->z += cphi_s * c.z + sphi_s * lp1.z;
+                r_l = pm_rsqrt(v->x * v->x + v->y * v->y + v->z * v->z);
+                v->x *= r_l;
+                v->y *= r_l;
+                v->z *= r_l;
+            }
+
+            positioned = 1;
+            s->prop.x = lsrcpos.x + lp1.x * 0.5f;
+            s->prop.y = lsrcpos.y + lp1.y * 0.5f;
+            s->prop.z = lsrcpos.z + lp1.z * 0.5f;
+            s->prop.w = 0.f;
+            s->Lmove = -1.f;
+        }
+
+        if (positioned) {
+            s->idx1d = (uint32_t)((int)floorf(p->z) * (int)g->dimxy + (int)floorf(p->y) * (int)g->dimx + (int)floorf(p->x));
+
+            if (p->x < 0.f || p->y < 0.f || p->z < 0.f || p->x >= g->maxidx.x || p->y >= g->maxidx.y ||
+                    p->z >= g->maxidx.z) {
+                s->mediaid = 0;
+            } else {
+                s->mediaid = g->media[s->idx1d];
+            }
+        }
+
+        if (pm_fabs(p->w) <= g->minenergy) {
+            continue;   /* a do-while continue re-tests the loop condition directly */
+        }
+
+        if (s->Lmove < 0.f) {
+            if (lsrcdir.w != lsrcdir.w) {   /* NaN focal length: isotropic launch */
+                float ang, stheta, ctheta, sphi, cphi;
+                ang = TWO_PI * rng_next(s->t, w);
+                pm_sincos(ang, &sphi, &cphi);
+                ang = pm_acos(2.f * rng_next(s->t, w) - 1.f);
+                pm_sincos(ang, &stheta, &ctheta);
+                rotatevector(v, stheta, ctheta, sphi, cphi);
+            } else if (lsrcdir.w < 0.f && isinf(lsrcdir.w)) {   /* -Inf: Lambertian */
+                float ang, stheta, ctheta, sphi, cphi;
+                ang = TWO_PI * rng_next(s->t, w);
+                pm_sincos(ang, &sphi, &cphi);
+                stheta = pm_sqrt(rng_next(s->t, w));
+                ctheta = pm_sqrt(1.f - stheta * stheta);
+                rotatevector(v, stheta, ctheta, sphi, cphi);
+            } else if (lsrcdir.w != 0.f) {   /* focused/diverging beam */
+                float Rn2 = (float)((lsrcdir.w > 0.f) - (lsrcdir.w < 0.f));
+                s->prop.x += lsrcdir.w * v->x;
+                s->prop.y += lsrcdir.w * v->y;
+                s->prop.z += lsrcdir.w * v->z;
+                v->x = Rn2 * (s->prop.x - p->x);
+                v->y = Rn2 * (s->prop.y - p->y);
+                v->z = Rn2 * (s->prop.z - p->z);
+                Rn2 = pm_rsqrt(v->x * v->x + v->y * v->y + v->z * v->z);
+                v->x *= Rn2;
+                v->y *= Rn2;
+                v->z *= Rn2;
+            }
+        }
+
+        if ((s->mediaid & MED_MASK) == 0) {
+            int id = skipvoid(g, p, v, f, fd);
+
+            if (id >= 0) {
+                s->idx1d = (uint32_t)id;
+                s->mediaid = g->media[s->idx1d];
+            }
+        }
+
+        fd->x = (short)floorf(p->x);
+        fd->y = (short)floorf(p->y);
+        fd->z = (short)floorf(p->z);
+        s->w0 += 1.f;
+
+        if (s->w0 > (float)g->maxvoidstep) {
+            return -1;
+        }
+    } while ((s->mediaid & MED_MASK) == 0 || pm_fabs(p->w) <= g->minenergy);
+
+    f->w += 1.f;
+    s->prop = getprop(g, s->mediaid);
+    pp[1] += p->w;
+    s->w0 = p->w;
+    pp[2] = (g->srcnum > 1) ? pp[2] : p->w;
+    v->w = K_EPS;
+    s->Lmove = 0.f;
+    w->cnt.nlaunch += 1;
+    return 0;
+}
+
+static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed, float* genergy) {
+    OPhoton S;
+    OPhoton* s = &S;
+    memset(s, 0, sizeof(OPhoton));
+    const uint32_t pd = g->partialdata;
+    F4* p = &s->p;
+    F4* v = &s->v;
+    F4* f = &s->f;
+    S4* fd = &s->fd;
+    p->w = NAN;
+    s->mediaid = pm_asuint(g->srcparam2.w);
+    fd->w = -1;
+    uint32_t isdet = 0, mediaidold = 0, idx1dold;
+    float pathlen = 0.f, n1;
+    const uint32_t nlimit = g->threadphoton + (idx < g->oddphoton);
+
+    s->pp[pd].f = genergy[idx * 2];
+    s->pp[pd + 1].f = genergy[idx * 2 + 1];
+    s->t[0] = (uint64_t)nseed[idx * 4] << 32 | nseed[idx * 4 + 1];
+    s->t[1] = (uint64_t)nseed[idx * 4 + 2] << 32 | nseed[idx * 4 + 3];
+
+    if (g->debuglevel & MCX_DEBUG_RNG) {
+        return;   /* handled by the caller */
+    }
+
+    if (launch_photon(g, w, s, idx, 0)) {
+        return;
+    }
+
+    isdet = s->mediaid & DET_MASK;
+    s->mediaid &= MED_MASK;
+
+    while (f->w <= (float)nlimit) {
+        w->cnt.nstep += 1;
+
+        if (f->x <= 0.f) {
+            f->x = rng_scatlen(s->t, w);
+
+            if (v->w != K_EPS) {
+                float cphi = 1.f, sphi = 0.f, theta, stheta, ctheta, tmp0 = 0.f;
+
+                if (!g->is2d) {
+                    tmp0 = TWO_PI * rng_next(s->t, w);
+                    pm_sincos(tmp0, &sphi, &cphi);
+                }
+
+                if (g->nphase > 2) {
+                    tmp0 = rng_next(s->t, w) * (float)(g->nphase - 1);
+                    theta = tmp0 - (float)((int)tmp0);
+                    uint32_t i0 = (uint32_t)tmp0 >= g->nphase ? g->nphase - 1 : (uint32_t)(int)tmp0;
+                    uint32_t i1 = (uint32_t)tmp0 + 1 >= g->nphase ? g->nphase - 1 : (uint32_t)((int)tmp0 + 1);
+                    tmp0 = (1.f - theta) * g->invcdf[i0] + theta * g->invcdf[i1];
+                    theta = pm_acos(tmp0);
+                    stheta = pm_sin(theta);
+                    ctheta = tmp0;
+                } else {
+                    tmp0 = (v->w > (float)g->gscatter) ? 0.f : s->prop.z;
+
+                    if (pm_fabs(tmp0) > K_EPS) {
+                        tmp0 = (1.f - s->prop.z * s->prop.z) / (1.f - s->prop.z + 2.f * s->prop.z * rng_next(s->t, w));
+                        tmp0 *= tmp0;
+                        tmp0 = (1.f + s->prop.z * s->prop.z - tmp0) / (2.f * s->prop.z);
+                        tmp0 = fmaxf(-1.f, fminf(1.f, tmp0));
+                        theta = pm_acos(tmp0);
+                        stheta = pm_sin(theta);
+                        ctheta = tmp0;
+                    } else {
+                        theta = pm_acos(2.f * rng_next(s->t, w) - 1.f);
+                        pm_sincos(theta, &stheta, &ctheta);
+                    }
+                }
+
+                if (g->savedet && (s->mediaid & MED_MASK) > 0) {
+                    if (SAVE_NSCAT(g->savedetflag)) {
+                        s->pp[(s->mediaid & MED_MASK) - 1].u++;
+                    }
+
+                    if (SAVE_MOM(g->savedetflag)) {
+                        s->pp[g->maxmedia * (SAVE_NSCAT(g->savedetflag) + SAVE_PPATH(g->savedetflag)) +
+                                          (s->mediaid & MED_MASK) - 1].f += 1.f - ctheta;
+                    }
+                }
+
+                if (g->is2d) {
+                    rotatevector2d(v, (rng_next(s->t, w) > 0.5f ? stheta : -stheta), ctheta, (int)g->is2d);
+                } else {
+                    rotatevector(v, stheta, ctheta, sphi, cphi);
+                }
+
+                v->w += 1.f;
+                w->cnt.nscat += 1;
+
+                if (g->outputtype == otWP || g->outputtype == otDCS) {
+                    uint32_t pid = photon_index(g, idx, f->w) - 1;
+                    tmp0 = (g->outputtype == otDCS) ? (1.f - ctheta) : 1.f;
+                    tmp0 *= g->replayweight[pid];
+                    int tshift = gate_of(g, g->replaytof[pid]) +
+                                 ((g->replaydet == -1) ? (((g->replaydetid[pid] & 0xFFFF) - 1) * (int)g->maxgate) : 0);
+
+                    if (g->extrasrclen && g->srcid < 0) {
+                        tshift += ((int)s->pp[g->w0offset - 1].f - 1) * ((g->replaydet == -1) ? (int)g->detnum : 1) *
+                                  (int)g->maxgate;
+                    }
+                    deposit(g, w, s->idx1d + (uint32_t)tshift * g->nvox, tmp0);
+                }
+            }
+
+            v->w = (float)(int)v->w;
+        }
+
+        n1 = s->prop.w;
+        s->prop = getprop(g, s->mediaid);
+        f->z = hitgrid(p, v, fd);
+        float gfac = (v->w + 1.f > (float)g->gscatter ? (1.f - s->prop.z) : 1.f);
+        float slen = f->z * s->prop.y * gfac;
+        slen = fminf(slen, f->x);
+        f->z = slen / (s->prop.y * gfac);
+        pathlen += f->z;
+        p->x = p->x + f->z * v->x;
+        p->y = p->y + f->z * v->y;
+        p->z = p->z + f->z * v->z;
+
+        if (fd->w == 0) {
+            fd->x += (slen == f->x) ? 0 : (v->x > 0.f ? 1 : -1);
+        }
+
+        if (fd->w == 1) {
+            fd->y += (slen == f->x) ? 0 : (v->y > 0.f ? 1 : -1);
+        }
+
+        if (fd->w == 2) {
+            fd->z += (slen == f->x) ? 0 : (v->z > 0.f ? 1 : -1);
+        }
+
+        p->w *= pm_exp(-s->prop.x * f->z);
+        f->x -= slen;
+        f->y += f->z * s->prop.w * g->oneoverc0;
+
+        if (g->savedet && SAVE_PPATH(g->savedetflag) && (s->mediaid & MED_MASK) > 0) {
+            s->pp[g->maxmedia * SAVE_NSCAT(g->savedetflag) + (s->mediaid & MED_MASK) - 1].f += f->z;
+        }
+
+        mediaidold = s->mediaid | isdet;
+        idx1dold = s->idx1d;
+        s->idx1d = (uint32_t)(fd->z * (int)g->dimxy + fd->y * (int)g->dimx + fd->x);
+
+        if (!inside(g, fd->x, fd->y, fd->z)) {
+            s->mediaid = 0;
+            s->idx1d = (fd->x < 0 || fd->y < 0 || fd->z < 0) ? OUTSIDE_VOLUME_MIN : OUTSIDE_VOLUME_MAX;
+            isdet = g->bc[(s->idx1d == OUTSIDE_VOLUME_MAX) * 3 + fd->w];
+            isdet = ((isdet & 0xF) == bcUnknown) ? (g->doreflect ? bcReflect : bcAbsorb) : isdet;
+        } else {
+            s->mediaid = g->media[s->idx1d];
+            isdet = s->mediaid & DET_MASK;
+            s->mediaid &= MED_MASK;
+        }
+
+        if (s->idx1d != idx1dold && idx1dold < g->nvox && mediaidold) {
+            if (g->save2pt && f->y >= g->twin0 && f->y < g->twin1) {
+                float weight = 0.f;
+                int tshift = gate_of(g, f->y);
+
+                if (g->outputtype == otEnergy) {
+                    weight = s->w0 - p->w;
+                } else if (g->outputtype == otFluence || g->outputtype == otFlux) {
+                    weight = (s->prop.x < K_EPS) ? (s->w0 * pathlen) : ((s->w0 - p->w) / s->prop.x);
+                } else if (g->seed == SEED_FROM_FILE) {
+                    if (g->outputtype == otJacobian) {
+                        uint32_t pid = photon_index(g, idx, f->w) - 1;
+                        weight = g->replayweight[pid] * pathlen;
+                        tshift = gate_of(g, g->replaytof[pid]) +
+                                 ((g->replaydet == -1) ? (((g->replaydetid[pid] & 0xFFFF) - 1) * (int)g->maxgate) : 0);
+                    }
+                } else if (g->outputtype == otL) {
+                    weight = s->w0 * pathlen;
+                }
+
+                if (g->extrasrclen && g->srcid < 0) {
+                    tshift += ((int)s->pp[g->w0offset - 1].f - 1) * ((g->replaydet == -1) ? (int)g->detnum : 1) *
+                              (int)g->maxgate;
+                }
+
+                if (pm_fabs(weight) > 0.f) {
+                    deposit(g, w, idx1dold + (uint32_t)tshift * g->nvox, weight);
+                }
+            }
+
+            s->w0 = p->w;
+            pathlen = 0.f;
+        } else {
+            s->mediaid = mediaidold;
+        }
+
+        if ((s->mediaid == 0 && ((isdet & 0xF) == bcAbsorb || (isdet & 0xF) == bcCyclic ||
+                                 ((isdet & 0xF) == bcReflect && n1 == g->prop[0].w))) || f->y > g->twin1) {
+            if (isdet == bcCyclic) {
+                if (fd->w == 0) {
+                    p->x = mcx_nextafterf(rintf((s->idx1d == OUTSIDE_VOLUME_MIN) ? g->maxidx.x : 0.f),
+                                          (v->x > 0.f) - (v->x < 0.f));
+                    fd->x = (short)floorf(p->x);
+                }
+
+                if (fd->w == 1) {
+                    p->y = mcx_nextafterf(rintf((s->idx1d == OUTSIDE_VOLUME_MIN) ? g->maxidx.y : 0.f),
+                                          (v->y > 0.f) - (v->y < 0.f));
+                    fd->y = (short)floorf(p->y);
+                }
+
+                if (fd->w == 2) {
+                    p->z = mcx_nextafterf(rintf((s->idx1d == OUTSIDE_VOLUME_MIN) ? g->maxidx.z : 0.f),
+                                          (v->z > 0.f) - (v->z < 0.f));
+                    fd->z = (short)floorf(p->z);
+                }
+
+                if (inside(g, fd->x, fd->y, fd->z)) {
+                    s->idx1d = (uint32_t)(fd->z * (int)g->dimxy + fd->y * (int)g->dimx + fd->x);
+                    s->mediaid = g->media[s->idx1d];
+                    isdet = s->mediaid & DET_MASK;
+                    s->mediaid &= MED_MASK;
+                    continue;
+                }
+            }
+
+            uint32_t detflag = (((s->idx1d == OUTSIDE_VOLUME_MAX && g->bc[9 + fd->w]) ||
+                                 (s->idx1d == OUTSIDE_VOLUME_MIN && g->bc[6 + fd->w])) ? OUTSIDE_VOLUME_MIN :
+                                (mediaidold & DET_MASK));
+
+            if (launch_photon(g, w, s, idx, detflag)) {
+                break;
+            }
+
+            isdet = s->mediaid & DET_MASK;
+            s->mediaid &= MED_MASK;
+            continue;
+        }
+
+        if (pm_fabs(p->w) < g->minenergy) {
+            if (rng_next(s->t, w) * ROULETTE_SIZE <= 1.f) {
+                p->w *= ROULETTE_SIZE;
+            } else {
+                if (launch_photon(g, w, s, idx, mediaidold & DET_MASK)) {
+                    break;
+                }
+
+                isdet = s->mediaid & DET_MASK;
+                s->mediaid &= MED_MASK;
+                continue;
+            }
+        }
+
+        if (g->reflectblock) {
+            s->prop = getprop(g, s->mediaid);
+
+            if (((s->mediaid && g->doreflect) ||
+                    (s->mediaid == 0 && (((isdet & 0xF) == bcUnknown && g->doreflect) || ((isdet & 0xF) == bcReflect ||
+                                         (isdet & 0xF) == bcMirror)))) &&
+                    (((isdet & 0xF) == bcMirror) || n1 != s->prop.w)) {
+                float Rtotal = 1.f;
+                float cphi, sphi, stheta, ctheta;
+                w->cnt.nbnd += 1;
+                s->prop = getprop(g, s->mediaid);
+                float tmp0 = n1 * n1;
+                float tmp1 = s->prop.w * s->prop.w;
+                cphi = pm_fabs((fd->w == 0) ? v->x : (fd->w == 1 ? v->y : v->z));
+                sphi = 1.f - cphi * cphi;
+                f->z = 1.f - tmp0 / tmp1 * sphi;
+
+                if (f->z > 0.f && (isdet & 0xF) != bcMirror) {
+                    ctheta = tmp0 * cphi * cphi + tmp1 * f->z;
+                    stheta = 2.f * n1 * s->prop.w * cphi * pm_sqrt(f->z);
+                    Rtotal = (ctheta - stheta) / (ctheta + stheta);
+                    ctheta = tmp1 * cphi * cphi + tmp0 * f->z;
+                    Rtotal = (Rtotal + (ctheta - stheta) / (ctheta + stheta)) * 0.5f;
+                }
+
+                if (Rtotal < 1.f && (!(s->mediaid == 0 && ((isdet & 0xF) == bcMirror))) &&
+                        rng_next(s->t, w) > Rtotal) {
+                    transmit(v, n1, s->prop.w, fd->w);
+
+                    if (s->mediaid == 0) {
+                        uint32_t detflag = (((s->idx1d == OUTSIDE_VOLUME_MAX && g->bc[9 + fd->w]) ||
+                                             (s->idx1d == OUTSIDE_VOLUME_MIN && g->bc[6 + fd->w])) ? OUTSIDE_VOLUME_MIN :
+                                            (mediaidold & DET_MASK));
+
+                        if (launch_photon(g, w, s, idx, detflag)) {
+                            break;
+                        }
+
+                        isdet = s->mediaid & DET_MASK;
+                        s->mediaid &= MED_MASK;
+                        continue;
+                    }
+                } else {
+                    /* the reference passes (v>0)-0.5f as an int direction, which truncates to 0:
+                       the position snaps to the face without a nudge (src/mcx_core.cl:3205-3209) */
+                    if (fd->w == 0) {
+                        v->x = -v->x;
+                        p->x = mcx_nextafterf(rintf(p->x), 0);
+                        fd->x = (short)rintf(p->x);
+                    } else if (fd->w == 1) {
+                        v->y = -v->y;
+                        p->y = mcx_nextafterf(rintf(p->y), 0);
+                        fd->y = (short)rintf(p->y);
+                    } else {
+                        v->z = -v->z;
+                        p->z = mcx_nextafterf(rintf(p->z), 0);
+                        fd->z = (short)rintf(p->z);
+                    }
+
+                    s->idx1d = idx1dold;
+                    s->mediaid = (g->media[s->idx1d] & MED_MASK);
+                    s->prop = getprop(g, s->mediaid);
+                    n1 = s->prop.w;
+                }
+            }
+        }
+    }
+
+    genergy[idx * 2] = s->pp[pd].f;
+    genergy[idx * 2 + 1] = s->pp[pd + 1].f;
+}
+
+/* ---------------- host driver ---------------- */
+typedef struct {
+    OracleCounters cnt;
+    double* rawfield;      /* unnormalised double field, fieldlen entries (caller frees) */
+    size_t fieldlen;
+    float* genergy;        /* per-thread float tallies [2*nthread] (caller frees) */
+    unsigned int nthread;
+    double seconds;        /* wall-clock of the photon loop */
+} OracleResult;
+
+static void set_ieee_fpu(void) {
+    /* clear flush-to-zero and denormals-are-zero so results match IEEE hardware */
+    _MM_SET_FLUSH_ZERO_MODE(_MM_FLUSH_ZERO_OFF);
+    _MM_SET_DENORMALS_ZERO_MODE(_MM_DENORMALS_ZERO_OFF);
+}
+
+static double now_sec(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + ts.tv_nsec * 1e-9;
+}
+
+/*
+ * Run the simulation described by a preprocessed Config on the CPU.
+ * nthread/threadoffset/threadcount select the thread map: the reference's
+ * persistent threads 0..nthread-1 are all seeded (glibc srand/rand), and the
+ * threads [threadoffset, threadoffset+threadcount) are simulated (a bounded
+ * sample for timing); threadcount=0 runs all of them.  nomp is the number of
+ * OpenMP workers (0: all cores).
+ * Outputs follow mcx_run_simulation: cfg->exportfield (normalised float),
+ * cfg->exportdetected/seeddata/detectedcount, cfg->energytot/energyesc/
+ * energyabs and cfg->normalizer.  Returns 0 on success.
+ */
+int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, unsigned int threadcount, int nomp,
+               OracleResult* res) {
+    memset(res, 0, sizeof(*res));
+
+    if (nthread == 0) {
+        return -1;
+    }
+
+    OParam g;
+    memset(&g, 0, sizeof(g));
+    memcpy(&g.srcpos, &cfg->srcpos, 16);
+    memcpy(&g.srcdir, &cfg->srcdir, 16);
+    memcpy(&g.srcparam1, &cfg->srcparam1, 16);
+    memcpy(&g.srcparam2, &cfg->srcparam2, 16);
+    g.maxidx.x = (float)cfg->dim.x;
+    g.maxidx.y = (float)cfg->dim.y;
+    g.maxidx.z = (float)cfg->dim.z;
+    g.dimx = cfg->dim.x;
+    g.dimxy = cfg->dim.x * cfg->dim.y;
+    g.nvox = g.dimxy * cfg->dim.z;
+    unsigned int maxgate = (unsigned int)((cfg->tend - cfg->tstart) / cfg->tstep + 0.5);
+    cfg->maxgate = maxgate;
+    g.maxgate = maxgate;
+    unsigned int nsrcslot = (cfg->srcid < 0) ? cfg->extrasrclen + 1 : 1;
+    unsigned int ndetslot = (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) ? cfg->detnum : 1;
+    size_t fieldlen = (size_t)g.nvox * maxgate * nsrcslot * ndetslot;
+    g.fieldlen = (uint32_t)fieldlen;
+    float minstep = fminf(fminf(cfg->steps.x, cfg->steps.y), cfg->steps.z);
+    g.twin0 = cfg->tstart;
+    g.twin1 = cfg->tend;
+    g.oneoverc0 = R_C0 * cfg->unitinmm;
+    g.Rtstep = 1.f / cfg->tstep;
+    g.minenergy = cfg->minenergy;
+    g.minaccumtime = minstep * R_C0 * cfg->unitinmm;
+    g.maxdetphoton = cfg->maxdetphoton;
+    g.maxmedia = cfg->medianum - 1;
+    g.detnum = cfg->detnum;
+    g.voidtime = (uint32_t)cfg->voidtime;
+    g.srctype = (uint32_t)cfg->srctype;
+    g.maxvoidstep = (uint32_t)cfg->maxvoidstep;
+    g.issaveseed = cfg->issaveseed > 0;
+    g.issaveref = (uint32_t)cfg->issaveref;
+    g.isspecular = cfg->isspecular > 0;
+    g.seed = cfg->seed;
+    g.outputtype = (uint32_t)cfg->outputtype;
+    g.savedetflag = cfg->savedetflag;
+    g.partialdata = mcx_partialdata(cfg);
+    g.reclen = mcx_detreclen(cfg);
+    g.w0offset = g.partialdata + 4;
+    g.mediaformat = cfg->mediabyte;
+    g.gscatter = cfg->gscatter;
+    g.srcnum = cfg->srcnum;
+    g.doreflect = (uint32_t)cfg->isreflect;
+    g.save2pt = (uint32_t)cfg->issave2pt;
+    g.savedet = (uint32_t)cfg->issavedet;
+    g.nphase = cfg->nphase;
+    g.invcdf = cfg->invcdf;
+    g.debuglevel = cfg->debuglevel;
+    g.extrasrclen = cfg->extrasrclen;
+    g.srcid = cfg->srcid;
+    g.srcdata = cfg->srcdata;
+    g.replaydet = cfg->replaydet;
+    g.replayweight = cfg->replay.weight;
+    g.replaytof = cfg->replay.tof;
+    g.replaydetid = cfg->replay.detid;
+    g.replayseed = (const uint64_t*)cfg->replay.seed;
+    memcpy(g.bc, cfg->bc, 12);
+
+    /* the reflection block exists when reflection is on, when the boundary flags
+       differ from "all absorbing" (as C strings), or when a face reflects or
+       mirrors (src/mcx_host.cpp:945-956) */
+    {
+        char bcs[13] = {0}, allabsorb[7] = {bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, 0};
+        memcpy(bcs, cfg->bc, 12);
+        g.reflectblock = cfg->isreflect || (strcmp(bcs, allabsorb) && bcs[0] != 0);
+
+        for (int i = 0; i < 6; i++) {
+            if (cfg->bc[i] == bcReflect || cfg->bc[i] == bcMirror) {
+                g.reflectblock = 1;
+            }
+        }
+    }
+
+    unsigned int is2d = (cfg->dim.x == 1 ? 1 : (cfg->dim.y == 1 ? 2 : (cfg->dim.z == 1 ? 3 : 0)));
+
+    if (is2d) {
+        is2d = is2d * (((cfg->dim.x > 1) + (cfg->dim.y > 1) + (cfg->dim.z > 1)) == 2);
+    }
+
+    g.is2d = is2d;
+
+    F4* props = (F4*)malloc(sizeof(F4) * cfg->medianum);
+    memcpy(props, cfg->prop, sizeof(F4) * cfg->medianum);
+    g.prop = props;
+    g.detpos = (const F4*)cfg->detpos;
+    g.media = cfg->vol;
+
+    size_t nphoton = cfg->nphoton;
+    g.threadphoton = (uint32_t)(nphoton / nthread);
+    g.oddphoton = (int32_t)(nphoton - (size_t)g.threadphoton * nthread);
+
+    /* glibc seeding, one rand() per 32-bit word (src/mcx_host.cpp:696-700, :759-768) */
+    uint32_t* nseed = (uint32_t*)malloc(sizeof(uint32_t) * 4 * (size_t)nthread);
+
+    if (cfg->seed > 0) {
+        srand((unsigned int)cfg->seed);
+    } else {
+        srand((unsigned int)time(0));
+    }
+
+    for (size_t j = 0; j < (size_t)nthread * 4; j++) {
+        nseed[j] = (uint32_t)rand();
+    }
+
+    float* genergy = (float*)calloc((size_t)nthread * 2, sizeof(float));
+    unsigned int t0 = threadoffset, t1 = threadcount ? threadoffset + threadcount : nthread;
+
+    if (t1 > nthread) {
+        t1 = nthread;
+    }
+
+    int nworkers = 1;
+#ifdef _OPENMP
+    nworkers = nomp > 0 ? nomp : omp_get_max_threads();
+#endif
+    OWork* works = (OWork*)calloc(nworkers, sizeof(OWork));
+
+    for (int i = 0; i < nworkers; i++) {
+        works[i].field = (double*)calloc(fieldlen, sizeof(double));
+    }
+
+    double tic = now_sec();
+
+    if (cfg->debuglevel & MCX_DEBUG_RNG) {
+        /* RNG dump: thread idx writes its stream to field[idx], field[idx+nthread], ...
+           (src/mcx_core.cl:2408-2414) */
+        for (unsigned int idx = t0; idx < t1; idx++) {
+            uint64_t t[2];
+            t[0] = (uint64_t)nseed[idx * 4] << 32 | nseed[idx * 4 + 1];
+            t[1] = (uint64_t)nseed[idx * 4 + 2] << 32 | nseed[idx * 4 + 3];
+
+            for (size_t i = idx; i < fieldlen; i += nthread) {
+                works[0].field[i] = rng_next(t, &works[0]);
+            }
+        }
+    } else {
+        #pragma omp parallel num_threads(nworkers)
+        {
+            int wid = 0;
+#ifdef _OPENMP
+            wid = omp_get_thread_num();
+#endif
+            set_ieee_fpu();
+            OWork* w = works + wid;
+            #pragma omp for schedule(dynamic, 64)
+
+            for (long idx = (long)t0; idx < (long)t1; idx++) {
+                if ((uint32_t)idx >= g.threadphoton * nthread + (uint32_t)g.oddphoton) {
+                    continue;
+                }
+
+                run_thread(&g, w, (int)idx, nseed, genergy);
+            }
+        }
+    }
+
+    res->seconds = now_sec() - tic;
+
+    double* raw = (double*)calloc(fieldlen, sizeof(double));
+    size_t ndet = 0;
+
+    for (int i = 0; i < nworkers; i++) {
+        for (size_t j = 0; j < fieldlen; j++) {
+            raw[j] += works[i].field[j];
+        }
+
+        ndet += works[i].ndet;
+        res->cnt.nstep += works[i].cnt.nstep;
+        res->cnt.nscat += works[i].cnt.nscat;
+        res->cnt.ndep += works[i].cnt.ndep;
+        res->cnt.nbnd += works[i].cnt.nbnd;
+        res->cnt.nlaunch += works[i].cnt.nlaunch;
+        res->cnt.nrand += works[i].cnt.nrand;
+    }
+
+    /* detected photons */
+    cfg->detectedcount = 0;
+    cfg->his.detected = (unsigned int)ndet;
+    size_t nsave = ndet < cfg->maxdetphoton ? ndet : cfg->maxdetphoton;
+
+    if (cfg->issavedet) {
+        free(cfg->exportdetected);
+        cfg->exportdetected = (float*)malloc((nsave ? nsave : 1) * g.reclen * sizeof(float));
+
+        if (cfg->issaveseed) {
+            free(cfg->seeddata);
+            cfg->seeddata = malloc((nsave ? nsave : 1) * 16);
+        }
+
+        size_t k = 0;
+
+        for (int i = 0; i < nworkers && k < nsave; i++) {
+            size_t take = works[i].ndet < nsave - k ? works[i].ndet : nsave - k;
+            memcpy(cfg->exportdetected + k * g.reclen, works[i].det, take * g.reclen * sizeof(float));
+
+            if (cfg->issaveseed) {
+                memcpy((char*)cfg->seeddata + k * 16, works[i].seeds, take * 16);
+            }
+
+            k += take;
+        }
+
+        cfg->detectedcount = (unsigned int)nsave;
+    }
+
+    for (int i = 0; i < nworkers; i++) {
+        free(works[i].field);
+        free(works[i].det);
+        free(works[i].seeds);
+    }
+
+    free(works);
+
+    /* energy tallies and normalisation (src/mcx_host.cpp:1299-1306, :1382-1465) */
+    cfg->energyesc = 0;
+    cfg->energytot = 0;
+
+    for (unsigned int i = 0; i < nthread; i++) {
+        cfg->energyesc += genergy[i * 2];
+        cfg->energytot += genergy[i * 2 + 1];
+    }
+
+    cfg->energyabs = cfg->energytot - cfg->energyesc;
+
+    if (cfg->issave2pt) {
+        free(cfg->exportfield);
+        cfg->exportfield = (float*)calloc(fieldlen, sizeof(float));
+
+        for (size_t i = 0; i < fieldlen; i++) {
+            cfg->exportfield[i] = (float)raw[i];
+        }
+
+        if (cfg->isnormalized && !(cfg->debuglevel & MCX_DEBUG_RNG)) {
+            float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
+            float scale = 1.f;
+
+            if (cfg->outputtype == otFlux || cfg->outputtype == otFluence) {
+                scale = cfg->unitinmm / (float)(cfg->energytot * Vvox * cfg->tstep);
+
+                if (cfg->outputtype == otFluence) {
+                    scale *= cfg->tstep;
+                }
+            } else if (cfg->outputtype == otEnergy || cfg->outputtype == otL) {
+                scale = 1.f / (float)cfg->energytot;
+            } else if (cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS) {
+                if (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
+                    for (int d = 1; d <= (int)cfg->detnum; d++) {
+                        float sd = 0.f;
+
+                        for (size_t i = 0; i < cfg->nphoton; i++)
+                            if ((cfg->replay.detid[i] & 0xFFFF) == d) {
+                                sd += cfg->replay.weight[i];
+                            }
+
+                        if (sd > 0.f) {
+                            sd = cfg->unitinmm / sd;
+                        }
+
+                        mcx_normalize(cfg->exportfield + (size_t)(d - 1) * g.nvox * maxgate, sd, (int)(g.nvox * maxgate), 1, 0, 1);
+                    }
+
+                    scale = 0.f;
+                } else {
+                    float sw = 0.f;
+
+                    for (size_t i = 0; i < cfg->nphoton; i++) {
+                        sw += cfg->replay.weight[i];
+                    }
+
+                    scale = sw > 0.f ? cfg->unitinmm / sw : 0.f;
+                }
+            }
+
+            if (cfg->extrasrclen && cfg->srcid < 0) {
+                scale *= (float)(cfg->extrasrclen + 1);
+            }
+
+            if (scale != 0.f) {
+                cfg->normalizer = scale;
+                cfg->his.normalizer = scale;
+                mcx_normalize(cfg->exportfield, scale, (int)fieldlen, cfg->isnormalized, 0, 1);
+            }
+        }
+    }
+
+    if (cfg->exportfield64) {
+        memcpy(cfg->exportfield64, raw, fieldlen * sizeof(double));
+    }
+
+    res->rawfield = raw;
+    res->fieldlen = fieldlen;
+    res->genergy = genergy;
+    res->nthread = nthread;
+    free(nseed);
+    free(props);
+    return 0;
+}
+
+void oracle_free_result(OracleResult* res) {
+    free(res->rawfield);
+    free(res->genergy);
+    memset(res, 0, sizeof(*res));
+}
+
+/* first n words of glibc rand() after srand(seed) (pins the host seeding) */
+void oracle_glibc_rand(unsigned int seed, unsigned int n, uint32_t* out) {
+    srand(seed);
+
+    for (unsigned int i = 0; i < n; i++) {
+        out[i] = (uint32_t)rand();
+    }
+}
+
+/* first n floats of the xorshift128+ stream from 4 seed words */
+void oracle_rng_stream(const uint32_t seed4[4], unsigned int n, float* out, uint64_t* state_out) {
+    OWork w;
+    memset(&w, 0, sizeof(w));
+    uint64_t t[2];
+    t[0] = (uint64_t)seed4[0] << 32 | seed4[1];
+    t[1] = (uint64_t)seed4[2] << 32 | seed4[3];
+
+    for (unsigned int i = 0; i < n; i++) {
+        out[i] = rng_next(t, &w);
+    }
+
+    if (state_out) {
+        state_out[0] = t[0];
+        state_out[1] = t[1];
+    }
+}
+
+/* expose the geometric kernels for brute-force checks */
+float oracle_hitgrid(const float p[3], const float v[3], const short id[3], int* face) {
+    F4 pp = {p[0], p[1], p[2], 0.f}, vv = {v[0], v[1], v[2], 0.f};
+    S4 ii = {id[0], id[1], id[2], 0};
+    float d = hitgrid(&pp, &vv, &ii);
+    *face = ii.w;
+    return d;
+}
+
+float oracle_reflectcoeff(const float v[3], float n1, float n2, int flipdir) {
+    F4 vv = {v[0], v[1], v[2], 0.f};
+    return reflectcoeff(&vv, n1, n2, flipdir);
+}
+
+void oracle_rotatevector(float v[3], float stheta, float ctheta, float sphi, float cphi) {
+    F4 vv = {v[0], v[1], v[2], 0.f};
+    rotatevector(&vv, stheta, ctheta, sphi, cphi);
+    v[0] = vv.x;
+    v[1] = vv.y;
+    v[2] = vv.z;
+}
+
+/* single-precision math functions, for accuracy checks */
+void oracle_fpmath(int which, const float* x, float* y, size_t n) {
+    for (size_t i = 0; i < n; i++) {
+        switch (which) {
+            case 0:
+                y[i] = pm_exp(x[i]);
+                break;
+
+            case 1:
+                y[i] = pm_log(x[i]);
+                break;
+
+            case 2:
+                y[i] = pm_sin(x[i]);
+                break;
+
+            case 3:
+                y[i] = pm_cos(x[i]);
+                break;
+
+            case 4:
+                y[i] = pm_acos(x[i]);
+                break;
+
+            case 5:
+                y[i] = pm_rsqrt(x[i]);
+                break;
+
+            default:
+                y[i] = x[i];
+        }
+    }
+}
+
+size_t oracle_result_sizeof(void) {
+    return sizeof(OracleResult);
+}
