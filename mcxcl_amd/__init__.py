@@ -7,6 +7,6 @@ library has not been built (``make -C mcxcl_amd``).
 """
 
 from .pmcxcl import run, gpuinfo, version  # noqa: F401
-from . import workloads  # noqa: F401
+from . import workloads, anchors  # noqa: F401
 
-__all__ = ["run", "gpuinfo", "version", "workloads"]
+__all__ = ["run", "gpuinfo", "version", "workloads", "anchors"]

@@ -1,0 +1,249 @@
+"""Pins of the CPU oracle against things it does not share with the kernel:
+libc, independent restatements of published algorithms, closed forms, brute
+force and the reference's published absorbed fractions."""
+
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import xorshift128p_floats
+from mcxcl_amd import _native, workloads
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _golden(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+# ---------------- RNG and seeding ----------------
+
+def test_glibc_seed_words_match_libc_and_restatement():
+    """host seeding: the product's own glibc TYPE_3 restatement equals libc rand()"""
+    for seed in (1, 1648335518, 29012392, 0x623F9A9E):
+        w = np.zeros(50000, dtype=np.uint32)
+        _native.lib.mcx_host_seed_words(seed, w.size, w.ctypes.data)
+        assert np.array_equal(w, oracle.glibc_rand(seed, w.size))
+
+
+def test_glibc_seed_golden():
+    g = _golden("rng_vectors.json")
+    for seed, words in g["glibc_rand"].items():
+        assert oracle.glibc_rand(int(seed), len(words)).tolist() == words
+
+
+def test_xorshift_stream_matches_independent_statement():
+    for seed4 in ([1, 2, 3, 4], [0x12345678, 0x9ABCDEF0, 0x0F1E2D3C, 0x4B5A6978], [0xFFFFFFFF, 0, 0x80000000, 7]):
+        a, _ = oracle.rng_stream(seed4, 200)
+        b = xorshift128p_floats(seed4, 200)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_xorshift_stream_golden():
+    g = _golden("rng_vectors.json")
+    for item in g["xorshift128p"]:
+        a, _ = oracle.rng_stream(item["seed4"], len(item["bits"]))
+        assert a.view(np.uint32).tolist() == item["bits"]
+
+
+def test_rng_uniformity():
+    a, _ = oracle.rng_stream([11, 22, 33, 44], 200000)
+    assert a.min() >= 0.0 and a.max() < 1.0
+    hist, _ = np.histogram(a, bins=20, range=(0, 1))
+    expected = a.size / 20
+    chi2 = ((hist - expected) ** 2 / expected).sum()
+    assert chi2 < 50  # 19 dof, p ~ 1e-4
+
+
+# ---------------- single-precision math ----------------
+
+def _ulp_err(y, ref):
+    y = y.astype(np.float64)
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    return np.abs(y - ref) / ulp
+
+
+@pytest.mark.parametrize("which,fn,lo,hi,maxulp", [
+    (0, np.exp, -87.0, 88.0, 4),
+    (2, np.sin, 0.0, 2 * math.pi, 3),
+    (3, np.cos, 0.0, 2 * math.pi, 3),
+    (4, np.arccos, -1.0, 1.0, 4),
+    (5, lambda x: 1.0 / np.sqrt(x), 1e-6, 1e6, 2),
+])
+def test_fpmath_accuracy(which, fn, lo, hi, maxulp):
+    x = np.random.default_rng(which).uniform(lo, hi, 200000).astype(np.float32)
+    y = oracle.fpmath(which, x)
+    ref = fn(x.astype(np.float64))
+    if which in (2, 3):
+        # absolute error near zeros of sin/cos, relative elsewhere
+        assert np.max(np.abs(y - ref)) < 3e-7
+    else:
+        assert np.max(_ulp_err(y, ref)) <= maxulp
+
+
+def test_fpmath_log_accuracy():
+    x = np.exp(np.random.default_rng(5).uniform(np.log(1e-7), np.log(1.0), 200000)).astype(np.float32)
+    y = oracle.fpmath(1, x)
+    ref = np.log(x.astype(np.float64))
+    assert np.max(np.abs(y - ref) / np.maximum(np.abs(ref), 1e-3)) < 4e-7
+    assert oracle.fpmath(1, np.array([1.0], np.float32))[0] == 0.0
+
+
+# ---------------- geometry and optics ----------------
+
+def test_hitgrid_brute_force():
+    """distance to the next voxel face: exact ray-plane intersection in float64"""
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        ijk = rng.integers(0, 50, 3)
+        p = (ijk + rng.uniform(0.01, 0.99, 3)).astype(np.float32)
+        v = rng.normal(size=3)
+        v = (v / np.linalg.norm(v)).astype(np.float32)
+        d, face = oracle.hitgrid(p, v, ijk)
+        planes = ijk + (v > 0)
+        t = (planes - p.astype(np.float64)) / v.astype(np.float64)
+        assert face == int(np.argmin(t))
+        assert abs(d - t.min()) <= 1e-5 * max(1.0, t.min()) + 2e-7 / abs(v[face])
+
+
+def test_fresnel_closed_forms():
+    n1, n2 = 1.37, 1.0
+    # normal incidence: ((n1-n2)/(n1+n2))^2
+    r0 = oracle.reflectcoeff([0, 0, 1], n1, n2, 2)
+    assert abs(r0 - ((n1 - n2) / (n1 + n2)) ** 2) < 1e-6
+    # beyond the critical angle: total internal reflection
+    tc = math.asin(n2 / n1)
+    for th in (tc + 0.01, tc + 0.2, 1.5):
+        v = [math.sin(th), 0, math.cos(th)]
+        assert oracle.reflectcoeff(v, n1, n2, 2) == 1.0
+    # general angle: unpolarised average of the s and p coefficients
+    for th in np.linspace(0.05, tc - 0.02, 20):
+        ci = math.cos(th)
+        ct = math.sqrt(1 - (n1 / n2 * math.sin(th)) ** 2)
+        rs = ((n1 * ci - n2 * ct) / (n1 * ci + n2 * ct)) ** 2
+        rp = ((n1 * ct - n2 * ci) / (n1 * ct + n2 * ci)) ** 2
+        v = [math.sin(th), 0, ci]
+        assert abs(oracle.reflectcoeff(v, n1, n2, 2) - 0.5 * (rs + rp)) < 2e-6
+
+
+def test_rotatevector_geometry():
+    rng = np.random.default_rng(2)
+    for _ in range(1000):
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        theta = rng.uniform(0, math.pi)
+        phi = rng.uniform(0, 2 * math.pi)
+        w = oracle.rotatevector(v, math.sin(theta), math.cos(theta), math.sin(phi), math.cos(phi))
+        assert abs(np.linalg.norm(w) - 1) < 1e-6
+        assert abs(float(np.dot(v, w)) - math.cos(theta)) < 2e-5
+
+
+# ---------------- whole-kernel physics ----------------
+
+def _absorbed(out):
+    st = out["stat"]
+    return st["energyabs"] / st["energytot"]
+
+
+def test_cube60b_absorbed_fraction():
+    """reflecting cube: test/testmcx.sh:65 expects 27.x%; README.md:619 reports 27.22654% at 1e7"""
+    out = oracle.run(workloads.homogeneous_cube_workload(nphoton=200000), nthread=65536)
+    a = _absorbed(out)
+    assert 0.27 <= a < 0.28
+    assert abs(a - 0.2722654) < 0.003
+
+
+def test_cube60_absorbed_fraction():
+    """non-reflecting cube: 17.69% +- 0.5% (mcxlabcl/examples/mcx_gpu_benchmarks.m:65)"""
+    out = oracle.run(workloads.homogeneous_cube_workload(nphoton=200000, isreflect=0), nthread=65536)
+    assert abs(_absorbed(out) - 0.1769) < 0.005
+
+
+def test_cubesph60b_absorbed_fraction():
+    """cube with sphere: 27.02% (speedcontest/mcxcontest:104)"""
+    out = oracle.run(workloads.sphere_cube_workload(nphoton=200000, tstep=5e-9, issavedet=0), nthread=65536)
+    assert abs(_absorbed(out) - 0.2702) < 0.003
+
+
+def test_energy_conservation():
+    """energy output: deposited weight equals launched minus escaped (photons that
+    leave the time window count as escaped and deposit nothing past it)"""
+    cfg = workloads.homogeneous_cube_workload(nphoton=50000, outputtype="energy", isnormalized=0, tend=1e-7,
+                                              tstep=1e-7)
+    out = oracle.run(cfg, nthread=8192)
+    st = out["stat"]
+    dep = out["flux64"].sum()
+    assert abs(dep - st["energyabs"]) / st["energyabs"] < 2e-3
+
+
+def test_isotropic_source_center_absorbed():
+    """isotropic point source in the cube centre: 88.x% absorbed (test/testmcx.sh:89)"""
+    cfg = workloads.homogeneous_cube_workload(nphoton=50000, srctype="isotropic", srcpos=[29, 29, 29], isreflect=0)
+    a = _absorbed(oracle.run(cfg, nthread=8192))
+    assert 0.88 <= a < 0.89
+
+
+def test_hg_mean_cosine():
+    """Henyey-Greenstein sampling: with mus large and a single scattering step the
+    mean cosine of the deflection equals g.  Estimated from the momentum transfer
+    tally (sum of 1-cos(theta)) of detected photons in an all-detecting box."""
+    for gval in (0.01, 0.5, 0.9):
+        prop = [[0, 0, 1, 1], [0.0, 1.0, gval, 1.0]]
+        cfg = workloads.homogeneous_cube_workload(nphoton=20000, prop=prop, isreflect=0, issavedet=1,
+                                                  bc="______111111", savedetflag="DSM", tend=1e-6, tstep=1e-6)
+        out = oracle.run(cfg, nthread=4096)
+        detp = out["detp"]
+        nscat = detp[1].view(np.uint32).astype(np.float64)
+        mom = detp[2].astype(np.float64)
+        mean_cos = 1.0 - mom.sum() / nscat.sum()
+        assert abs(mean_cos - gval) < 0.01, (gval, mean_cos)
+
+
+def test_diffusion_theory_far_field():
+    """homogeneous semi-infinite medium: time-resolved diffusion (utils/tddiffusion.m:27-37 with
+    extrapolated-boundary images, Reff for n=1.37 from demo_validation_homogeneous.m:90) integrated
+    over the gate agrees with the Monte Carlo fluence away from the source and the side walls"""
+    mua, musp, n = 0.005, 1.0 * (1 - 0.01), 1.37
+    cfg = workloads.homogeneous_cube_workload(nphoton=400000, outputtype="fluence", tend=5e-9, tstep=5e-9)
+    out = oracle.run(cfg, nthread=65536)
+    flu = out["flux"][..., 0]
+    D = 1.0 / (3 * (mua + musp))
+    Reff = 0.493
+    zb = 2 * D * (1 + Reff) / (1 - Reff)
+    z0 = 1.0 / (mua + musp)
+    c = 299792458000.0 / n
+    ts = np.linspace(1e-12, 5e-9, 4000)
+    dt = ts[1] - ts[0]
+
+    def phi(r, z):
+        r1 = np.sqrt(r * r + (z - z0) ** 2)
+        r2 = np.sqrt(r * r + (z + z0 + 2 * zb) ** 2)
+        t = ts[:, None]
+        g = c / (4 * np.pi * D * c * t) ** 1.5 * np.exp(-mua * c * t)
+        val = g * (np.exp(-r1 ** 2 / (4 * D * c * t)) - np.exp(-r2 ** 2 / (4 * D * c * t)))
+        return val.sum(axis=0) * dt
+
+    # azimuthal averages over rings around the source axis in the plane z = 8.5 mm,
+    # radii 4-11 mm (away from the source and > 17 mm from every side wall)
+    zc = 8
+    ii, jj = np.meshgrid(np.arange(60), np.arange(60), indexing="ij")
+    rr = np.sqrt((ii + 0.5 - 29.0) ** 2 + (jj + 0.5 - 29.0) ** 2)
+    for r0 in range(4, 12):
+        sel = (rr >= r0) & (rr < r0 + 1)
+        mc = flu[..., zc][sel].mean()
+        th = phi(rr[sel], zc + 0.5).mean()
+        assert abs(mc / th - 1) < 0.08, (r0, mc / th)
+
+
+def test_time_gates_partition_the_cw_result():
+    """10 gates over 0-5 ns sum to the single 0-5 ns gate (same photons, same seeds)"""
+    base = workloads.homogeneous_cube_workload(nphoton=20000, outputtype="energy", isnormalized=0)
+    one = oracle.run(base, nthread=4096)["flux64"]
+    ten = oracle.run(dict(base, tstep=5e-10), nthread=4096)["flux64"]
+    assert ten.shape[-1] == 10
+    np.testing.assert_allclose(ten.sum(axis=-1), one[..., 0], rtol=1e-9, atol=1e-9)
