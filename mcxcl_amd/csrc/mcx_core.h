@@ -23,6 +23,10 @@ struct MCXKParam {
     uint32_t nthread, debugrng, mediaformat;
     int32_t seed, oddphoton, replaydet, srcid;
     uint32_t fieldlen;
+    /* deposit privatisation: the box [x0,x0+nx)x[y0,y0+ny)x[z0,z0+nz) of the first ng
+       gates is accumulated in LDS (f64) per workgroup and flushed once; ng=0 disables */
+    uint32_t box_x0, box_y0, box_z0, box_nx, box_ny, box_nz, box_ng, box_cells;
+    uint32_t nrep;                /* global accumulator replicas (power of two) */
     unsigned char bc[12];
 
     const uint32_t* media;        /* label volume, x fastest; bit 31 marks detector-adjacent voxels */
@@ -35,7 +39,7 @@ struct MCXKParam {
     const int* replaydetid;
     const uint64_t* replayseed;   /* 2 words per replayed photon */
     const uint32_t* nseed;        /* 4 words per thread */
-    double* field;                /* fluence accumulator, fieldlen entries */
+    double* field;                /* fluence accumulator, nrep x fieldlen entries */
     float* genergy;               /* {escaped, launched} per thread */
     float* detrec;                /* detected-photon records, reclen floats each */
     uint64_t* detseed;            /* 2 words per detected photon */
@@ -53,8 +57,8 @@ hipError_t mcx_launch_fast(const MCXKParam* prm, int nblock, int nthreadblock, s
                            int savedet, int reflect);
 /* RNG stream dump: field[i] for i = idx; i < fieldlen; i += nthread */
 hipError_t mcx_launch_rngdump(const MCXKParam* prm, int nblock, int nthreadblock, hipStream_t stream);
-/* out[i] = (float)(in[i]) * scale for the normalised float output */
-hipError_t mcx_launch_finalize(const double* in, float* out, size_t n, float scale, hipStream_t stream);
+/* sum the nrep replicas into in[0..n), then out[i] = (float)in[i] * scale */
+hipError_t mcx_launch_finalize(double* in, float* out, size_t n, unsigned int nrep, float scale, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -261,8 +261,23 @@ struct Photon {
 struct Shared {
     float4* prop;                /* medium table */
     float* invcdf;
+    double* box;                 /* privatised deposit box (box_cells) */
     float* pp;                   /* this lane's scratch: partial data, energies, w0, srcid */
+    size_t repoff;               /* offset of this lane's global accumulator replica */
 };
+
+/* voxel-exit deposit: LDS box when the voxel and gate fall inside it, else global */
+__device__ __forceinline__ void deposit_voxel(const MCXKParam& g, const Shared& sh, int ox, int oy, int oz,
+        uint32_t idx1d, int tshift, float val) {
+    uint32_t dx = (uint32_t)(ox - (int)g.box_x0), dy = (uint32_t)(oy - (int)g.box_y0);
+    uint32_t dz = (uint32_t)(oz - (int)g.box_z0);
+
+    if (dx < g.box_nx && dy < g.box_ny && dz < g.box_nz && (uint32_t)tshift < g.box_ng) {
+        atomicAdd(sh.box + (((uint32_t)tshift * g.box_nz + dz) * g.box_ny + dy) * g.box_nx + dx, (double)val);
+    } else {
+        atomicAdd(g.field + sh.repoff + idx1d + (uint32_t)tshift * g.nvox, (double)val);
+    }
+}
 
 __device__ __forceinline__ float4 getprop(const Shared& sh, uint32_t mediaid) {
     return sh.prop[mediaid & K_MED_MASK];
@@ -370,8 +385,9 @@ __device__ __forceinline__ int gate_of(const MCXKParam& g, float t) {
     return gate > (int)g.maxgate - 1 ? (int)g.maxgate - 1 : gate;
 }
 
-__device__ __forceinline__ void deposit(const MCXKParam& g, uint32_t addr, float val) {
-    atomicAdd(g.field + addr, (double)val);
+/* global deposit into this lane's accumulator replica */
+__device__ __forceinline__ void deposit(const MCXKParam& g, size_t repoff, uint32_t addr, float val) {
+    atomicAdd(g.field + repoff + addr, (double)val);
 }
 
 /* src/mcx_core.cl:838-926 */
@@ -466,7 +482,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
             }
 
             if (t2 >= 0) {
-                deposit(g, s.idx1d + (uint32_t)tshift * g.nvox, -p.w);
+                deposit(g, sh.repoff, s.idx1d + (uint32_t)tshift * g.nvox, -p.w);
             }
         }
 
@@ -760,7 +776,9 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
     Shared sh;
     sh.prop = lds;
     sh.invcdf = (float*)(lds + g.medianum);
-    float* ppbase = sh.invcdf + ((g.nphase + 3) & ~3u);
+    sh.box = (double*)(sh.invcdf + ((g.nphase + 3) & ~3u));
+    float* ppbase = (float*)(sh.box + g.box_cells);
+    sh.repoff = (size_t)(idx & (int)(g.nrep - 1)) * g.fieldlen;
 
     for (uint32_t i = threadIdx.x; i < g.medianum; i += blockDim.x) {
         sh.prop[i] = g.prop[i];
@@ -768,6 +786,10 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
 
     for (uint32_t i = threadIdx.x; i < g.nphase; i += blockDim.x) {
         sh.invcdf[i] = g.invcdf[i];
+    }
+
+    for (uint32_t i = threadIdx.x; i < g.box_cells; i += blockDim.x) {
+        sh.box[i] = 0.0;
     }
 
     /* lane-interleaved scratch: element k of lane t at [k*blockDim + t] would avoid
@@ -781,16 +803,17 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
 
     __syncthreads();
 
-    if ((uint32_t)idx >= g.threadphoton * g.nthread + (uint32_t)g.oddphoton || (uint32_t)idx >= g.nthread) {
-        return;
-    }
-
+    /* lanes without photons skip the transport loop but still join the box flush */
+    const bool active = !((uint32_t)idx >= g.threadphoton * g.nthread + (uint32_t)g.oddphoton ||
+                          (uint32_t)idx >= g.nthread);
     const uint32_t pd = g.partialdata;
-    sh.pp[pd] = g.genergy[idx * 2];
-    sh.pp[pd + 1] = g.genergy[idx * 2 + 1];
+    /* the tail workgroup may hold lanes past nthread: they read nothing */
+    const uint32_t sidx = (uint32_t)idx < g.nthread ? (uint32_t)idx : 0u;
+    sh.pp[pd] = g.genergy[sidx * 2];
+    sh.pp[pd + 1] = g.genergy[sidx * 2 + 1];
     Rng rng;
-    rng.s0 = (uint64_t)g.nseed[idx * 4] << 32 | g.nseed[idx * 4 + 1];
-    rng.s1 = (uint64_t)g.nseed[idx * 4 + 2] << 32 | g.nseed[idx * 4 + 3];
+    rng.s0 = (uint64_t)g.nseed[sidx * 4] << 32 | g.nseed[sidx * 4 + 1];
+    rng.s1 = (uint64_t)g.nseed[sidx * 4 + 2] << 32 | g.nseed[sidx * 4 + 3];
 
     Photon s;
     s.p = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x7FC00000u));
@@ -808,8 +831,13 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
     uint32_t isdet = 0, mediaidold = 0, idx1dold = 0, launchflag = 0;
     float pathlen = 0.f, n1 = 0.f;
     bool relaunch = true;
+    /* coordinates of the voxel idx1dold names.  After a reflection idx1d is restored
+       while the integer position keeps the reflected face's index for one step, so the
+       previous coordinates are kept across that step. */
+    int ox = 0, oy = 0, oz = 0;
+    bool keepold = false;
 
-    while (true) {
+    while (active) {
         if (relaunch) {
             if (launch_photon<SAVEDET, PENCIL>(g, sh, s, rng, idx, launchflag)) {
                 break;
@@ -818,6 +846,7 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
             isdet = s.mediaid & K_DET_MASK;
             s.mediaid &= K_MED_MASK;
             relaunch = false;
+            keepold = false;
         }
 
         float4& p = s.p;
@@ -893,7 +922,7 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
                                   (int)g.maxgate;
                     }
 
-                    deposit(g, s.idx1d + (uint32_t)tshift * g.nvox, tmp0);
+                    deposit(g, sh.repoff, s.idx1d + (uint32_t)tshift * g.nvox, tmp0);
                 }
             }
 
@@ -909,6 +938,13 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
         float slen = f.z * s.prop.y * gfac;
         slen = fminf(slen, f.x);
         f.z = kdiv(slen, s.prop.y * gfac);
+        if (!keepold) {
+            ox = s.fx;
+            oy = s.fy;
+            oz = s.fz;
+        }
+
+        keepold = false;
         pathlen += f.z;
         p.x = p.x + f.z * v.x;
         p.y = p.y + f.z * v.y;
@@ -975,7 +1011,7 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
                 }
 
                 if (fabsf(weight) > 0.f) {
-                    deposit(g, idx1dold + (uint32_t)tshift * g.nvox, weight);
+                    deposit_voxel(g, sh, ox, oy, oz, idx1dold, tshift, weight);
                 }
             }
 
@@ -986,7 +1022,7 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
         }
 
         if ((s.mediaid == 0 && ((isdet & 0xF) == kAbsorb || (isdet & 0xF) == kCyclic ||
-                                ((isdet & 0xF) == kReflect && n1 == sh.prop[0].w))) || f.y > g.twin1) {
+                                ((isdet & 0xF) == kReflect && n1 == sh.prop[0].w))) || !(f.y <= g.twin1)) {
             if (isdet == kCyclic) {
                 if (face == 0) {
                     p.x = nextafter_ref(rintf((s.idx1d == K_OUT_MIN) ? g.maxidx_x : 0.f), (v.x > 0.f) - (v.x < 0.f));
@@ -1076,16 +1112,41 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
                     }
 
                     s.idx1d = idx1dold;
+                    keepold = true;
                     s.mediaid = (g.media[s.idx1d] & K_MED_MASK);
                     s.prop = getprop(sh, s.mediaid);
                     n1 = s.prop.w;
+                    /* drop the boundary code so a later same-voxel step restores
+                       the voxel's own label (the reference ORs the code in) */
+                    isdet &= K_DET_MASK;
                 }
             }
         }
     }
 
-    g.genergy[idx * 2] = sh.pp[pd];
-    g.genergy[idx * 2 + 1] = sh.pp[pd + 1];
+    if (active) {
+        g.genergy[idx * 2] = sh.pp[pd];
+        g.genergy[idx * 2 + 1] = sh.pp[pd + 1];
+    }
+
+    if (g.box_cells) {
+        /* flush the workgroup's private box into its replica of the global grid */
+        __syncthreads();
+        const size_t boff = (size_t)(blockIdx.x & (g.nrep - 1)) * g.fieldlen;
+
+        for (uint32_t i = threadIdx.x; i < g.box_cells; i += blockDim.x) {
+            double val = sh.box[i];
+
+            if (val != 0.0) {
+                uint32_t x = i % g.box_nx, r = i / g.box_nx;
+                uint32_t y = r % g.box_ny;
+                r /= g.box_ny;
+                uint32_t z = r % g.box_nz, gate = r / g.box_nz;
+                uint32_t vox = (g.box_z0 + z) * g.dimxy + (g.box_y0 + y) * g.dimx + (g.box_x0 + x);
+                atomicAdd(g.field + boff + vox + gate * g.nvox, val);
+            }
+        }
+    }
 }
 
 }  // namespace mcxk

@@ -52,18 +52,28 @@ extern "C" hipError_t mcx_launch_rngdump(const MCXKParam* prm, int nblock, int n
     return hipGetLastError();
 }
 
-/* double accumulator -> normalised float output, grid-stride */
-__global__ void finalize_kernel(const double* __restrict__ in, float* __restrict__ out, size_t n, float scale) {
+/* sum the accumulator replicas (fixed order) into replica 0, then write the
+   normalised float output; grid-stride */
+__global__ void finalize_kernel(double* __restrict__ in, float* __restrict__ out, size_t n, unsigned int nrep,
+                                float scale) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        float v = (float)in[i];
+        double acc = in[i];
+
+        for (unsigned int r = 1; r < nrep; r++) {
+            acc += in[(size_t)r * n + i];
+        }
+
+        in[i] = acc;
+        float v = (float)acc;
         out[i] = v * scale;
     }
 }
 
-extern "C" hipError_t mcx_launch_finalize(const double* in, float* out, size_t n, float scale, hipStream_t stream) {
+extern "C" hipError_t mcx_launch_finalize(double* in, float* out, size_t n, unsigned int nrep, float scale,
+                                          hipStream_t stream) {
     int nblock = (int)((n + 255) / 256);
     nblock = nblock > 8192 ? 8192 : (nblock < 1 ? 1 : nblock);
-    hipLaunchKernelGGL(finalize_kernel, dim3(nblock), dim3(256), 0, stream, in, out, n, scale);
+    hipLaunchKernelGGL(finalize_kernel, dim3(nblock), dim3(256), 0, stream, in, out, n, nrep, scale);
     return hipGetLastError();
 }
 

@@ -26,6 +26,16 @@
 #include "mcx_core.h"
 #include "mcx_host.h"
 
+/* lanes per workgroup of the transport kernel (its __launch_bounds__) */
+#define MCX_LAUNCH_BLOCK ((size_t)256)
+/* LDS deposit box: edge in voxels and byte budget per workgroup */
+#define MCX_BOX_EDGE 16u
+#define MCX_BOX_BYTES (32u * 1024u)
+#define MCX_LDS_LIMIT (64u * 1024u)
+/* global accumulator replicas (power of two) and their total byte cap */
+#define MCX_FIELD_REPLICAS 1u
+#define MCX_REPLICA_BYTES ((size_t)1 << 30)
+
 extern "C" void* mcx_push_errjmp(void* jb);
 extern "C" void mcx_pop_errjmp(void* old);
 extern "C" hipError_t mcx_launch_fpmath_parity(int which, const float* x, float* y, size_t n, hipStream_t s);
@@ -259,12 +269,16 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     size_t nblocksize = cfg->autopilot ? gpu[dev].autoblock : cfg->nblocksize;
     size_t nthread = cfg->autopilot ? gpu[dev].autothread : cfg->nthread;
 
-    if (nblocksize == 0 || nblocksize > 256 || (nblocksize & 63)) {
+    if (nblocksize == 0) {
         free(gpu);
-        MCX_ERROR(-4, "block size must be a multiple of 64 and at most 256");
+        MCX_ERROR(-4, "block size must be positive");
     }
 
+    /* the reference rounds the thread count down to whole blocks; the photon map
+       depends only on the thread count, so the transport kernel itself is launched
+       in workgroups of MCX_LAUNCH_BLOCK lanes with the tail lanes idle */
     nthread = (nthread / nblocksize) * nblocksize;
+    const size_t nlaunch = (nthread + MCX_LAUNCH_BLOCK - 1) / MCX_LAUNCH_BLOCK;
 
     if (nthread == 0) {
         free(gpu);
@@ -372,10 +386,84 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
 
     size_t pplen = g.w0offset + g.srcnum;
     size_t shmem = cfg->medianum * sizeof(float4) + ((cfg->nphase + 3) & ~3u) * sizeof(float) +
-                   nblocksize * pplen * sizeof(float);
+                   MCX_LAUNCH_BLOCK * pplen * sizeof(float);
 
     if (shmem > 64 * 1024) {
         MCX_ERROR(-4, "too many media/phase samples for the LDS tables");
+    }
+
+    /* deposit privatisation: most fluence deposits, and every heavily contended
+       cache line, fall in the first gates around the source.  A box of up to
+       MCX_BOX_EDGE^3 voxels centred on the source voxel (clamped to the grid) is
+       accumulated per workgroup in LDS and added to the global grid once at the end. */
+    {
+        size_t budget = MCX_BOX_BYTES;
+        const char* env = getenv("MCXHIP_BOX_BYTES");
+
+        if (env) {
+            budget = (size_t)strtoul(env, NULL, 10);
+        }
+
+        if (g.debugrng || !cfg->issave2pt || budget < sizeof(double)) {
+            budget = 0;
+        }
+
+        unsigned int edge[3] = {cfg->dim.x, cfg->dim.y, cfg->dim.z};
+        float spos[3] = {cfg->srcpos.x, cfg->srcpos.y, cfg->srcpos.z};
+        unsigned int org[3];
+        unsigned int maxedge = MCX_BOX_EDGE;
+
+        while (maxedge > 1 && (size_t)maxedge * maxedge * maxedge * sizeof(double) > budget) {
+            maxedge--;
+        }
+
+        for (int i = 0; i < 3; i++) {
+            edge[i] = edge[i] < maxedge ? edge[i] : maxedge;
+            int c = (int)floorf(spos[i]) - (int)(edge[i] / 2);
+            int hi = (int)((i == 0 ? cfg->dim.x : (i == 1 ? cfg->dim.y : cfg->dim.z)) - edge[i]);
+            org[i] = (unsigned int)(c < 0 ? 0 : (c > hi ? hi : c));
+        }
+
+        size_t pergate = (size_t)edge[0] * edge[1] * edge[2];
+        size_t ng = budget / (pergate * sizeof(double));
+        ng = ng < maxgate ? ng : maxgate;
+
+        if (ng > 0 && shmem + 8 + ng * pergate * sizeof(double) <= MCX_LDS_LIMIT) {
+            g.box_x0 = org[0];
+            g.box_y0 = org[1];
+            g.box_z0 = org[2];
+            g.box_nx = edge[0];
+            g.box_ny = edge[1];
+            g.box_nz = edge[2];
+            g.box_ng = (uint32_t)ng;
+            g.box_cells = (uint32_t)(ng * pergate);
+            /* the box is 8-byte aligned after the float tables */
+            shmem += (size_t)g.box_cells * sizeof(double) + 8;
+        }
+    }
+
+    /* replicas of the global accumulator spread the remaining contended lines */
+    {
+        size_t nr = MCX_FIELD_REPLICAS;
+        const char* env = getenv("MCXHIP_REPLICAS");
+
+        if (env) {
+            nr = (size_t)strtoul(env, NULL, 10);
+        }
+
+        if (g.debugrng) {
+            nr = 1;
+        }
+
+        while (nr > 1 && (nr & (nr - 1))) {
+            nr &= nr - 1;
+        }
+
+        while (nr > 1 && fieldlen * nr * sizeof(double) > MCX_REPLICA_BYTES) {
+            nr >>= 1;
+        }
+
+        g.nrep = nr ? (uint32_t)nr : 1u;
     }
 
     if (cfg->seed == SEED_FROM_FILE &&
@@ -455,8 +543,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     HIP_TRY(dalloc(bufs, &dnseed, nthread * 4));
     HIP_TRY(hipMemcpyAsync(dnseed, hseed, nthread * 4 * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
 
-    HIP_TRY(dalloc(bufs, &dfield, fieldlen));
-    HIP_TRY(hipMemsetAsync(dfield, 0, fieldlen * sizeof(double), stream));
+    HIP_TRY(dalloc(bufs, &dfield, fieldlen * g.nrep));
+    HIP_TRY(hipMemsetAsync(dfield, 0, fieldlen * g.nrep * sizeof(double), stream));
     HIP_TRY(dalloc(bufs, &dgenergy, nthread * 2));
     HIP_TRY(hipMemsetAsync(dgenergy, 0, nthread * 2 * sizeof(float), stream));
     HIP_TRY(dalloc(bufs, &ddetcount, 1));
@@ -491,12 +579,12 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     HIP_TRY(hipEventRecord(ev0, stream));
 
     if (g.debugrng) {
-        HIP_TRY(mcx_launch_rngdump(&g, (int)(nthread / nblocksize), (int)nblocksize, stream));
+        HIP_TRY(mcx_launch_rngdump(&g, (int)nlaunch, (int)MCX_LAUNCH_BLOCK, stream));
     } else if (cfg->mathmode == 1) {
-        HIP_TRY(mcx_launch_parity(&g, (int)(nthread / nblocksize), (int)nblocksize, shmem, stream, cfg->issavedet != 0,
+        HIP_TRY(mcx_launch_parity(&g, (int)nlaunch, (int)MCX_LAUNCH_BLOCK, shmem, stream, cfg->issavedet != 0,
                                   reflect));
     } else {
-        HIP_TRY(mcx_launch_fast(&g, (int)(nthread / nblocksize), (int)nblocksize, shmem, stream, cfg->issavedet != 0,
+        HIP_TRY(mcx_launch_fast(&g, (int)nlaunch, (int)MCX_LAUNCH_BLOCK, shmem, stream, cfg->issavedet != 0,
                                 reflect));
     }
 
@@ -588,12 +676,15 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             }
         }
 
+        HIP_TRY(dalloc(bufs, &dout, fieldlen));
+        HIP_TRY(mcx_launch_finalize(dfield, dout, fieldlen, g.nrep, (normalize && scale != 0.f) ? scale : 1.f,
+                                    stream));
+
         if (cfg->exportfield64) {
-            HIP_TRY(hipMemcpy(cfg->exportfield64, dfield, fieldlen * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpyAsync(cfg->exportfield64, dfield, fieldlen * sizeof(double), hipMemcpyDeviceToHost,
+                                   stream));
         }
 
-        HIP_TRY(dalloc(bufs, &dout, fieldlen));
-        HIP_TRY(mcx_launch_finalize(dfield, dout, fieldlen, (normalize && scale != 0.f) ? scale : 1.f, stream));
         hfield = (float*)malloc(fieldlen * sizeof(float));
         HIP_TRY(hipMemcpyAsync(hfield, dout, fieldlen * sizeof(float), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));

@@ -44,7 +44,13 @@
  *  - replay photons are indexed idx*threadphoton+min(idx,oddphoton)+k, the
  *    same map as forward photons; the reference uses min(idx,oddphoton-1)
  *    (src/mcx_core.cl:1591), which replays one photon twice and reads before
- *    the seed buffer when oddphoton is 0.
+ *    the seed buffer when oddphoton is 0;
+ *  - after a reflection the boundary code is cleared from isdet (its detector
+ *    bit is kept).  The reference keeps it, so the next step that stays in the
+ *    voxel restores mediaid = label | code (src/mcx_core.cl:2796, :2928), e.g.
+ *    1|3 = 3 after a mirror face, and reads past the property table;
+ *  - a photon whose time is NaN leaves the time window (the test is
+ *    !(t <= tend) instead of t > tend), so no lane can loop forever.
  */
 #include <float.h>
 #include <math.h>
@@ -936,7 +942,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
         }
 
         if ((s->mediaid == 0 && ((isdet & 0xF) == bcAbsorb || (isdet & 0xF) == bcCyclic ||
-                                 ((isdet & 0xF) == bcReflect && n1 == g->prop[0].w))) || f->y > g->twin1) {
+                                 ((isdet & 0xF) == bcReflect && n1 == g->prop[0].w))) || !(f->y <= g->twin1)) {
             if (isdet == bcCyclic) {
                 if (fd->w == 0) {
                     p->x = mcx_nextafterf(rintf((s->idx1d == OUTSIDE_VOLUME_MIN) ? g->maxidx.x : 0.f),
@@ -1055,6 +1061,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                     s->mediaid = (g->media[s->idx1d] & MED_MASK);
                     s->prop = getprop(g, s->mediaid);
                     n1 = s->prop.w;
+                    isdet &= DET_MASK;
                 }
             }
         }

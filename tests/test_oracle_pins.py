@@ -247,3 +247,38 @@ def test_time_gates_partition_the_cw_result():
     ten = oracle.run(dict(base, tstep=5e-10), nthread=4096)["flux64"]
     assert ten.shape[-1] == 10
     np.testing.assert_allclose(ten.sum(axis=-1), one[..., 0], rtol=1e-9, atol=1e-9)
+
+
+def test_mirror_box_absorbs_everything():
+    """all six faces mirrors, long time window: nothing escapes, so every launched
+    photon's weight ends in the grid (energy output, unnormalised)"""
+    prop = [[0, 0, 1, 1], [0.05, 1.0, 0.01, 1.37], [0.02, 2.0, 0.5, 1.37]]
+    vol = np.ones((20, 20, 20), dtype=np.uint8)
+    vol[5:15, 5:15, 5:15] = 2
+    cfg = dict(nphoton=2000, vol=vol, prop=prop, srcpos=[10, 10, 0], srcdir=[0, 0, 1], tstart=0, tend=2e-8,
+               tstep=2e-8, bc="mmmmmm", isreflect=1, outputtype="energy", isnormalized=0, seed=12345,
+               autopilot=0, nthread=512, nblocksize=64, issavedet=0)
+    out = oracle.run(cfg, nthread=512)
+    st = out["stat"]
+    assert abs(st["energyabs"] / st["energytot"] - 1.0) < 1e-3
+    assert abs(out["flux64"].sum() - st["energyabs"]) / st["energyabs"] < 1e-3
+    assert np.isfinite(out["flux64"]).all()
+
+
+@pytest.mark.parametrize("bc", ["mmmmmm", "rrrrrr"])
+def test_unused_medium_changes_nothing(bc):
+    """a medium that no voxel carries cannot change the result.  Pins the label
+    restore after a reflection: the reference ORs the boundary code into the
+    label (1|3 = 3 after a mirror face, 2|1 = 3 after a Fresnel face), which
+    would pick up medium 3 here."""
+    prop = [[0, 0, 1, 1], [0.05, 1.0, 0.01, 1.37], [0.02, 2.0, 0.5, 1.37]]
+    vol = np.ones((20, 20, 20), dtype=np.uint8)
+    vol[:, :, 10:] = 2
+    cfg = dict(nphoton=3000, vol=vol, prop=prop, srcpos=[10, 10, 0], srcdir=[0, 0, 1], tstart=0, tend=5e-9,
+               tstep=5e-9, bc=bc, isreflect=1, outputtype="energy", isnormalized=0, seed=777,
+               autopilot=0, nthread=512, nblocksize=64, issavedet=0)
+    a = oracle.run(cfg, nthread=512)
+    b = oracle.run(dict(cfg, prop=prop + [[5.0, 50.0, 0.9, 1.0]]), nthread=512)
+    assert a["stat"]["energyabs"] == b["stat"]["energyabs"]
+    # the worker grids are summed in completion order: rounding-level differences only
+    np.testing.assert_allclose(a["flux64"], b["flux64"], rtol=1e-12, atol=1e-15)
