@@ -1,24 +1,29 @@
 """Benchmark: forward photon transport on one MI355X.
 
-Workload (one "step"): the heterogeneous 60^3 cube with a radius-15 sphere,
-1e8 photons, 10 time gates over 0-5 ns, four 1 mm disk detectors with partial
-path lengths and exit positions saved, float32 fluence output.
+Workload (one "step"): the heterogeneous 60^3 cube with a radius-15 sphere
+(label 2), 1e8 photons, 10 time gates over 0-5 ns, four 1 mm disk detectors with
+partial path lengths and exit positions saved, float32 fluence output.
 
-value = photons per millisecond of the transport kernel (HIP events on the
-kernel's stream; inputs resident in HBM, fluence normalisation included in the
-following finalize kernel).  The JSON line also carries:
-  roofline      VALU lane-instruction roofline of the transport kernel (frozen
-                per-event operation model x oracle event counts, see
-                ROOFLINE_MODEL below) and the float64-atomic deposit rate
+value = photons per millisecond of the whole device job: the transport kernel
+plus the fluence finalisation kernel, bracketed by HIP events on the stream they
+run on, with the volume, media table and RNG seeds already resident in HBM.
+The JSON line also carries:
+  roofline      the transport kernel against the FP32/INT32 vector-issue peak
+                (algorithmic operations per photon from a frozen per-event
+                model times the oracle's event counts, divided by the kernel's
+                HIP-event duration); traffic = HBM bytes per launch read from the
+                committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present
   cpu_baseline  the CPU oracle (C restatement of the reference kernel, OpenMP)
                 timed on this host on a bounded sample of the same workload
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--photons N]
+       python bench.py --copy-probe BYTES   (plain device copy, for counter calibration)
 """
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -28,27 +33,33 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "photons/ms (fwd, frac of roofline); adjoint gradient/s vs finite-diff at ≤2% err"
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "transport_sphere60_1e8_hbm_traffic.txt")
 
-# Frozen per-event operation model (single-precision/integer VALU lane
-# instructions of the reference algorithm, transcendentals counted once here and
-# weighted x4 below because they issue at quarter rate):
-#   step      one main-loop iteration: property fetch, 3-way voxel DDA with 3
-#             divides, step clipping, move, Beer-Lambert exp, time update,
-#             voxel index and boundary tests       ~65 ops, 5 transcendental
-#   deposit   voxel-exit fluence weight and gate  ~17 ops, 1 transcendental (rcp)
+# Frozen per-event operation model: VALU lane operations of the reference
+# algorithm (src/mcx_core.cl mcx_main_loop), counted from its source; divides,
+# square roots and transcendentals are listed separately and weighted x4 because
+# they issue at quarter rate.
+#   step      one main-loop iteration: property fetch, 3-face voxel DDA,
+#             step clipping, move, Beer-Lambert attenuation, time update, voxel
+#             index, inside test, medium lookup, termination/roulette tests
+#                                                   ~65 ops, 5 div/exp
+#   deposit   voxel-exit weight, gate index, address  ~17 ops, 1 div
 #   scatter   scattering length (log), azimuth sincos, Henyey-Greenstein
-#             inversion, acos/sin, rotatevector with 3 rsqrt, 3 RNG draws
-#                                                  ~135 ops, 10 transcendental
-#   boundary  Fresnel coefficient, RNG draw, reflect/transmit ~55 ops, 2 transcendental
+#             inversion, cos/sin of the polar angle, rotation with 3 rsqrt,
+#             3 xorshift128+ draws                    ~135 ops, 10 div/sqrt/trans
+#   boundary  Fresnel coefficient, RNG draw, reflect/transmit  ~55 ops, 2 div/sqrt
 ROOFLINE_MODEL = {
     "step": (65, 5),
     "deposit": (17, 1),
     "scatter": (135, 10),
     "boundary": (55, 2),
 }
-# MI355X published FP32 vector peak 157.3 TFLOP/s counts 2 flops per FMA and
-# 2-wide packed FP32: lane-instruction issue peak = 157.3 / 4 = 39.3 T/s.
-VALU_PEAK_TOPS = 157.3 / 4.0
+# MI355X: 256 CUs x 64 FP32 lanes x 2.4 GHz = 39.3 T lane operations/s.  The
+# published 157.3 TFLOP/s FP32 vector peak is this issue rate x2 (FMA) x2
+# (packed FP32); this kernel's work is compares, selects, integer indexing and
+# single (unpacked) FP32 operations, so the issue rate is its roofline.
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+FP32_PEAK_TFLOPS = 157.3
 
 
 def ops_per_photon(counters):
@@ -61,6 +72,36 @@ def ops_per_photon(counters):
     return tot / n, counters["ndep"] / n
 
 
+def read_traffic():
+    """HBM bytes per transport launch from the committed rocprofv3 summary"""
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    with open(TRAFFIC_SUMMARY) as f:
+        m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def host_threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        n = min(n, int(omp))
+    return max(n, 1)
+
+
+def copy_probe(nbytes):
+    """one plain device-to-device copy of nbytes (FETCH_SIZE/WRITE_SIZE calibration)"""
+    import torch
+
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,14 +109,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=float, default=1e8)
     ap.add_argument("--mathmode", type=int, default=0, help="0: hardware transcendentals, 1: bit-reproducible")
-    ap.add_argument("--cpu-photons", type=float, default=3e5, help="photons in the CPU-baseline sample")
+    ap.add_argument("--cpu-photons", type=float, default=3e6, help="photons in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, help="run one plain device copy of this many bytes and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it (got --gpus %d)"
+        print("bench.py: this build measures one GPU; runs on several GPUs are not part of it (got --gpus %d)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     import mcxcl_amd
     from mcxcl_amd import workloads
@@ -86,19 +132,20 @@ def main():
     for _ in range(args.warmup):
         mcxcl_amd.run(cfg)
 
-    kernel_ms = []
-    wall = []
-    absorbed = []
-    ndet = []
+    job_ms, kernel_ms, wall, absorbed, ndet = [], [], [], [], []
     for _ in range(args.steps):
         t0 = time.perf_counter()
         out = mcxcl_amd.run(cfg)
         wall.append((time.perf_counter() - t0) * 1e3)
-        kernel_ms.append(out["stat"]["kernelms"])
-        absorbed.append(out["stat"]["energyabs"] / out["stat"]["energytot"])
+        st = out["stat"]
+        job_ms.append(st["jobms"])
+        kernel_ms.append(st["kernelms"])
+        absorbed.append(st["energyabs"] / st["energytot"])
         ndet.append(out["detp"].shape[1] if "detp" in out else 0)
+        print("step: job %.2f ms (transport %.2f ms), wall %.1f ms" % (job_ms[-1], kernel_ms[-1], wall[-1]),
+              file=sys.stderr, flush=True)
 
-    ms_per_step = float(np.mean(kernel_ms))
+    ms_per_step = float(np.mean(job_ms))
     value = nphoton / ms_per_step
 
     res = {
@@ -122,6 +169,7 @@ def main():
             "mathmode": "fast" if args.mathmode == 0 else "parity",
             "parallelism": "single GPU, 2560 persistent lanes per CU",
         },
+        "transport_ms_per_step": round(float(np.mean(kernel_ms)), 3),
         "wall_ms_per_step": round(float(np.mean(wall)), 3),
         "absorbed_fraction": round(float(np.mean(absorbed)), 6),
         "detected_per_step": int(np.mean(ndet)),
@@ -131,33 +179,38 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle
 
-        ncpu = os.cpu_count() or 1
+        ncpu = host_threads()
         cpu_n = int(args.cpu_photons)
         ccfg = workloads.sphere_cube_workload(nphoton=cpu_n, autopilot=0, nthread=65536, nblocksize=64)
         ref = oracle.run(ccfg, nthread=65536, nomp=ncpu)
         cpu_rate = cpu_n / (ref["seconds"] * 1e3)
         opp, depp = ops_per_photon(ref["counters"])
-        achieved_tops = value * 1e3 * opp / 1e12
+        transport_ms = float(np.mean(kernel_ms))
+        achieved = nphoton * opp / (transport_ms * 1e-3) / 1e12
         res["roofline"] = {
             "bound": "valu",
-            "achieved": round(achieved_tops, 3),
+            "achieved": round(achieved, 3),
             "peak": round(VALU_PEAK_TOPS, 3),
             "unit": "Tops/s",
-            "frac": round(achieved_tops / VALU_PEAK_TOPS, 4),
-            "traffic": None,
+            "frac": round(achieved / VALU_PEAK_TOPS, 4),
+            "traffic": read_traffic(),
+            "kernel": "transport_kernel<savedet,reflect,pencil>",
+            "kernel_ms": round(transport_ms, 3),
             "ops_per_photon": round(opp, 1),
-            "atomics_per_photon": round(depp, 2),
-            "atomic_rate_per_s": round(value * 1e3 * depp, 0),
+            "frac_of_fp32_vector_peak": round(achieved / FP32_PEAK_TFLOPS, 4),
+            "fluence_atomics_per_photon": round(depp, 2),
+            "fluence_atomic_rate_per_s": round(nphoton * depp / (transport_ms * 1e-3), 0),
         }
         res["cpu_baseline"] = {
             "value": round(cpu_rate, 2),
             "unit": "photons/ms",
             "cores": ncpu,
             "kind": "port",
-            "sample": "%d photons of the same workload (65536-thread map), oracle in C with OpenMP" % cpu_n,
+            "sample": "%d photons of the same workload (65536-thread map), C oracle with OpenMP, %.1f s" %
+                      (cpu_n, ref["seconds"]),
             "absorbed_fraction": round(ref["stat"]["energyabs"] / ref["stat"]["energytot"], 6),
         }
-    print(json.dumps(res))
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

@@ -312,40 +312,45 @@ typedef struct MCXConfig {
     /* ---- MI355X additions (no reference counterpart) ---- */
     int mathmode;            /* 0: fast hardware transcendentals, 1: parity (bit-reproducible) */
     float kernelms;          /* kernel-only time of the last run, from HIP events (ms) */
+    float jobms;             /* transport kernel plus fluence finalisation, from HIP events (ms) */
     double runtimems;        /* wall-clock of the photon loop of the last run (ms) */
     double* exportfield64;   /* optional: unnormalised double-precision field (fieldlen entries, caller-owned) */
 } Config;
 
-/* ---- configuration API (reference src/mcx_utils.h:286-340) ---- */
-void mcx_initcfg(Config* cfg);
-void mcx_clearcfg(Config* cfg);
-void mcx_error(const int id, const char* msg, const char* file, const int linenum);
-void mcx_assess(const int id, const char* msg, const char* file, const int linenum);
-void mcx_readconfig(char* fname, Config* cfg);
-void mcx_loadconfig(FILE* in, Config* cfg);
+/* ---- configuration API; each line names the reference declaration it replaces ---- */
+void mcx_initcfg(Config* cfg);                                              /* src/mcx_utils.h:295 */
+void mcx_clearcfg(Config* cfg);                                             /* src/mcx_utils.h:296 */
+void mcx_error(const int id, const char* msg, const char* file, const int linenum); /* src/mcx_utils.h:288 */
+void mcx_assess(const int id, const char* msg, const char* file, const int linenum); /* src/mcx_utils.h:289 */
+void mcx_readconfig(char* fname, Config* cfg);                              /* src/mcx_utils.h:293 */
+void mcx_loadconfig(FILE* in, Config* cfg);                                 /* src/mcx_utils.h:291 */
+/* replaces int mcx_loadjson(cJSON* root, Config* cfg) (src/mcx_utils.h:313): the
+   parsed-tree type is internal here, so the entry point takes the JSON text */
 int  mcx_loadjson_string(const char* json, Config* cfg);
-void mcx_loadbenchmark(char* key, Config* cfg);
-void mcx_prepdomain(char* filename, Config* cfg);
-void mcx_preprocess(Config* cfg);
-void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte);
-void mcx_replayinit(Config* cfg, float* detps, int dimdetps[2], int seedbyte);
-void mcx_maskdet(Config* cfg);
-void mcx_normalize(float field[], float scale, int fieldlen, int option, int pidx, int srcnum);
-void mcx_kahanSum(float* sum, float* kahanc, float input);
-float mcx_updatemua(unsigned int mediaid, Config* cfg);
-void mcx_createfluence(float** fluence, Config* cfg);
-void mcx_clearfluence(float** fluence);
-void mcx_convertrow2col(unsigned int** vol, MCXUInt4* dim);
-void mcx_cleargpuinfo(GPUInfo** gpuinfo);
-void mcx_flush(Config* cfg);
-int  mcx_lookupindex(char* key, const char* index);
-int  mcx_keylookup(char* key, const char* table[]);
-int  mcx_parsedebugopt(char* debugopt, const char* debugflag);
-void mcx_savedata(float* dat, size_t len, Config* cfg);
-void mcx_savedetphoton(float* ppath, void* seeds, int count, int seedbyte, Config* cfg);
+void mcx_loadbenchmark(char* key, Config* cfg);                             /* src/mcx_utils.h:335 */
+void mcx_prepdomain(char* filename, Config* cfg);                           /* src/mcx_utils.h:305 */
+void mcx_preprocess(Config* cfg);                                           /* src/mcx_utils.c:1521 (not in the reference header) */
+void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte); /* src/mcx_utils.h:337 */
+void mcx_replayinit(Config* cfg, float* detps, int dimdetps[2], int seedbyte); /* src/mcx_utils.h:336 */
+void mcx_maskdet(Config* cfg);                                              /* src/mcx_utils.h:304 */
+void mcx_normalize(float field[], float scale, int fieldlen, int option, int pidx, int srcnum); /* src/mcx_utils.h:300 */
+void mcx_kahanSum(float* sum, float* kahanc, float input);                  /* src/mcx_utils.h:326 */
+float mcx_updatemua(unsigned int mediaid, Config* cfg);                     /* src/mcx_utils.h:327 */
+void mcx_createfluence(float** fluence, Config* cfg);                       /* src/mcx_utils.h:306 */
+void mcx_clearfluence(float** fluence);                                     /* src/mcx_utils.h:307 */
+void mcx_convertrow2col(unsigned int** vol, MCXUInt4* dim);                 /* src/mcx_utils.h:309 */
+void mcx_cleargpuinfo(GPUInfo** gpuinfo);                                   /* src/mcx_utils.h:290 */
+void mcx_flush(Config* cfg);                                                /* src/mcx_utils.h:324 */
+int  mcx_lookupindex(char* key, const char* index);                         /* src/mcx_utils.h:317 */
+int  mcx_keylookup(char* key, const char* table[]);                         /* src/mcx_utils.h:314 */
+int  mcx_parsedebugopt(char* debugopt, const char* debugflag);              /* src/mcx_utils.h:318 */
+void mcx_savedata(float* dat, size_t len, Config* cfg);                     /* src/mcx_utils.h:286 */
+void mcx_savedetphoton(float* ppath, void* seeds, int count, int seedbyte, Config* cfg); /* src/mcx_utils.h:312 */
+/* replaces int mcx_parse_shapestring(Grid3D* g, char* shapedata) (src/mcx_shapes.h:56):
+   rasterises the shapes into cfg->vol/cfg->dim directly */
 int  mcx_parse_shapestring(Config* cfg, const char* shapedata);
-void mcx_parsecmd(int argc, char* argv[], Config* cfg);
-void mcx_usage(Config* cfg, char* exename);
+void mcx_parsecmd(int argc, char* argv[], Config* cfg);                     /* src/mcx_utils.h:297 */
+void mcx_usage(Config* cfg, char* exename);                                 /* src/mcx_utils.h:298 */
 
 /* record geometry of a detected photon (reference src/mcx_host.cpp:494-496) */
 unsigned int mcx_detreclen(const Config* cfg);

@@ -131,7 +131,7 @@ CONFIG_FIELDS = [
     ("nphase", ctypes.c_uint), ("invcdf", ctypes.c_void_p), ("nangle", ctypes.c_uint),
     ("angleinvcdf", ctypes.c_void_p), ("srcid", ctypes.c_int), ("extrasrclen", ctypes.c_uint),
     ("srcdata", ctypes.c_void_p), ("mathmode", ctypes.c_int), ("kernelms", ctypes.c_float),
-    ("runtimems", ctypes.c_double), ("exportfield64", ctypes.c_void_p),
+    ("jobms", ctypes.c_float), ("runtimems", ctypes.c_double), ("exportfield64", ctypes.c_void_p),
 ]
 
 lib.mcx_config_sizeof.restype = ctypes.c_size_t

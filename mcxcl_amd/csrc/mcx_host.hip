@@ -218,7 +218,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     DevBuf bufs;
     bufs.n = 0;
     hipStream_t stream = NULL;
-    hipEvent_t ev0 = NULL, ev1 = NULL;
+    hipEvent_t ev0 = NULL, ev1 = NULL, ev2 = NULL;
     uint32_t* hseed = NULL;
     float* hgenergy = NULL;
     float* hfield = NULL;
@@ -503,6 +503,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     HIP_TRY(hipStreamCreate(&stream));
     HIP_TRY(hipEventCreate(&ev0));
     HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventCreate(&ev2));
 
     HIP_TRY(dalloc(bufs, &dmedia, nvox));
     HIP_TRY(hipMemcpyAsync(dmedia, cfg->vol, nvox * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
@@ -593,6 +594,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     t1 = wall_ms();
     HIP_TRY(hipEventElapsedTime(&kernelms, ev0, ev1));
     cfg->kernelms = kernelms;
+    cfg->jobms = kernelms;
     cfg->runtimems = t1 - t0;
     cfg->runtime = (unsigned int)(t1 - t0 + 0.5);
 
@@ -679,6 +681,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         HIP_TRY(dalloc(bufs, &dout, fieldlen));
         HIP_TRY(mcx_launch_finalize(dfield, dout, fieldlen, g.nrep, (normalize && scale != 0.f) ? scale : 1.f,
                                     stream));
+        HIP_TRY(hipEventRecord(ev2, stream));
 
         if (cfg->exportfield64) {
             HIP_TRY(hipMemcpyAsync(cfg->exportfield64, dfield, fieldlen * sizeof(double), hipMemcpyDeviceToHost,
@@ -688,6 +691,11 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         hfield = (float*)malloc(fieldlen * sizeof(float));
         HIP_TRY(hipMemcpyAsync(hfield, dout, fieldlen * sizeof(float), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        {
+            float jobms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&jobms, ev0, ev2));
+            cfg->jobms = jobms;
+        }
 
         if (normalize && scale == 0.f && cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
             for (int d = 1; d <= (int)cfg->detnum; d++) {
@@ -721,6 +729,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     bufs.release();
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
+    (void)hipEventDestroy(ev2);
     (void)hipStreamDestroy(stream);
     return;
 
@@ -732,6 +741,10 @@ fail:
 
     if (ev0) {
         (void)hipEventDestroy(ev0);
+    }
+
+    if (ev2) {
+        (void)hipEventDestroy(ev2);
     }
 
     if (ev1) {

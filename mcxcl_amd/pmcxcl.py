@@ -338,6 +338,7 @@ def _collect(c, user, ngate, ndet, nsrc):
         "unitinmm": float(c.unitinmm),
         "workload": [1.0],
         "kernelms": float(c.kernelms),
+        "jobms": float(c.jobms),
         "runtimems": float(c.runtimems),
         "detected": int(c.his.detected),
     }
