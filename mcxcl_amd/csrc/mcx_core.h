@@ -27,6 +27,10 @@ struct MCXKParam {
        gates is accumulated in LDS (f64) per workgroup and flushed once; ng=0 disables */
     uint32_t box_x0, box_y0, box_z0, box_nx, box_ny, box_nz, box_ng, box_cells;
     uint32_t nrep;                /* global accumulator replicas (power of two) */
+    uint32_t dynamic;             /* 1: photons are handed out per workgroup at run time
+                                     (threadphoton/oddphoton then split photons over workgroups) */
+    uint32_t simple;              /* 1: pencil beam from a non-void voxel without focus, extra sources,
+                                     replay, 2D, phase table, reflectance tally or replay output types */
     unsigned char bc[12];
 
     const uint32_t* media;        /* label volume, x fastest; bit 31 marks detector-adjacent voxels */
@@ -56,6 +60,8 @@ hipError_t mcx_launch_parity(const MCXKParam* prm, int nblock, int nthreadblock,
 hipError_t mcx_launch_fast(const MCXKParam* prm, int nblock, int nthreadblock, size_t shmem, hipStream_t stream,
                            int savedet, int reflect);
 /* RNG stream dump: field[i] for i = idx; i < fieldlen; i += nthread */
+/* resident workgroups per CU of the fast transport kernel variant for this launch shape */
+int mcx_fast_occupancy(int nthreadblock, size_t shmem, int savedet, int reflect, int simple);
 hipError_t mcx_launch_rngdump(const MCXKParam* prm, int nblock, int nthreadblock, hipStream_t stream);
 /* sum the nrep replicas into in[0..n), then out[i] = (float)in[i] * scale */
 hipError_t mcx_launch_finalize(double* in, float* out, size_t n, unsigned int nrep, float scale, hipStream_t stream);

@@ -8,7 +8,7 @@
 
 template <bool SAVEDET, bool REFLECT>
 static hipError_t launch_variant(const MCXKParam* prm, int nblock, int nthreadblock, size_t shmem, hipStream_t stream) {
-    if (prm->srctype == kPencil) {
+    if (prm->simple) {
         hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, true>), dim3(nblock), dim3(nthreadblock), shmem,
                            stream, *prm);
     } else {
@@ -28,6 +28,38 @@ extern "C" hipError_t mcx_launch_fast(const MCXKParam* prm, int nblock, int nthr
 
     return reflect ? launch_variant<false, true>(prm, nblock, nthreadblock, shmem, stream)
            : launch_variant<false, false>(prm, nblock, nthreadblock, shmem, stream);
+}
+
+template <bool SAVEDET, bool REFLECT, bool SIMPLE>
+static int occupancy_of(int nthreadblock, size_t shmem) {
+    int nb = 0;
+
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxk::transport_kernel<SAVEDET, REFLECT, SIMPLE>,
+            nthreadblock, shmem) != hipSuccess) {
+        return 0;
+    }
+
+    return nb;
+}
+
+extern "C" int mcx_fast_occupancy(int nthreadblock, size_t shmem, int savedet, int reflect, int simple) {
+    if (savedet) {
+        if (reflect) {
+            return simple ? occupancy_of<true, true, true>(nthreadblock, shmem)
+                   : occupancy_of<true, true, false>(nthreadblock, shmem);
+        }
+
+        return simple ? occupancy_of<true, false, true>(nthreadblock, shmem)
+               : occupancy_of<true, false, false>(nthreadblock, shmem);
+    }
+
+    if (reflect) {
+        return simple ? occupancy_of<false, true, true>(nthreadblock, shmem)
+               : occupancy_of<false, true, false>(nthreadblock, shmem);
+    }
+
+    return simple ? occupancy_of<false, false, true>(nthreadblock, shmem)
+           : occupancy_of<false, false, false>(nthreadblock, shmem);
 }
 
 /* device self-check: math in fast (hardware) mode */

@@ -123,6 +123,16 @@ __device__ __forceinline__ float kacos(float x) {
 }
 #endif
 
+/* sin(acos(x)) for |x| <= 1.  Parity mode evaluates it as the reference writes
+   it; fast mode uses the identity sqrt(1 - x^2). */
+__device__ __forceinline__ float ksin_acos(float x) {
+#if MCX_PARITY
+    return ksin(kacos(x));
+#else
+    return __builtin_amdgcn_sqrtf(fmaxf(0.f, 1.f - x * x));
+#endif
+}
+
 /* ---------------- RNG: xorshift128+ (src/mcx_core.cl:684-698) ---------------- */
 struct Rng {
     uint64_t s0, s1;
@@ -185,9 +195,17 @@ __device__ __forceinline__ void rotatevector(float4& v, float stheta, float cthe
         v.z = nz;
     }
 
+#if MCX_PARITY
+    /* three sequential renormalisations, as the reference writes them */
     v.x *= krsqrt(v.x * v.x + v.y * v.y + v.z * v.z);
     v.y *= krsqrt(v.x * v.x + v.y * v.y + v.z * v.z);
     v.z *= krsqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+#else
+    float rn = krsqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    v.x *= rn;
+    v.y *= rn;
+    v.z *= rn;
+#endif
 }
 
 __device__ __forceinline__ void rotatevector2d(float4& v, float stheta, float ctheta, int is2d) {
@@ -264,6 +282,7 @@ struct Shared {
     double* box;                 /* privatised deposit box (box_cells) */
     float* pp;                   /* this lane's scratch: partial data, energies, w0, srcid */
     size_t repoff;               /* offset of this lane's global accumulator replica */
+    uint32_t* pool;              /* dynamic schedule: {next photon, end} of this workgroup */
 };
 
 /* voxel-exit deposit: LDS box when the voxel and gate fall inside it, else global */
@@ -459,7 +478,7 @@ __device__ void save_detected(const MCXKParam& g, const Shared& sh, const Photon
 /* terminate the current photon and launch the next one (src/mcx_core.cl:1466-2275).
    Returns 0 on a launch, 1 when the lane's photon budget is used up, -1 when
    no launch succeeded within maxvoidstep attempts. */
-template <bool SAVEDET, bool PENCIL>
+template <bool SAVEDET, bool SIMPLE>
 __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rng& rng, int idx, uint32_t isdet) {
     float* pp = sh.pp;
     const uint32_t pd = g.partialdata;
@@ -472,7 +491,8 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
     if (fabsf(p.w) >= 0.f) {
         pp[pd] += p.w;
 
-        if (g.issaveref == 1 && s.mediaid == 0 && s.idx1d != K_OUT_MIN && s.idx1d != K_OUT_MAX && p.w > 0.f) {
+        if (!SIMPLE && g.issaveref == 1 && s.mediaid == 0 && s.idx1d != K_OUT_MIN && s.idx1d != K_OUT_MAX &&
+                p.w > 0.f) {
             int tshift = (int)g.maxgate - 1;
             int t2 = (int)floorf((f.y - g.twin0) * g.Rtstep);
             tshift = tshift < t2 ? tshift : t2;
@@ -497,8 +517,38 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
         }
     }
 
-    if (f.w >= (float)(g.threadphoton + (idx < g.oddphoton))) {
+    if (g.dynamic) {
+        /* the workgroup's photons are handed out one at a time from LDS */
+        if (atomicAdd(sh.pool, 1u) >= sh.pool[1]) {
+            return 1;
+        }
+    } else if (f.w >= (float)(g.threadphoton + (idx < g.oddphoton))) {
         return 1;
+    }
+
+    if (SIMPLE) {
+        /* pencil beam from a non-void voxel, no focus, no extra sources, no replay:
+           the general launch below reduces to this (one pass of its loop) */
+        s.seed0 = rng.s0;
+        s.seed1 = rng.s1;
+        p = g.srcpos;
+        v = g.srcdir;
+        f.x = 0.f;
+        f.y = 0.f;
+        f.z = g.minaccumtime;
+        s.idx1d = __float_as_uint(g.srcparam2.z);
+        s.mediaid = __float_as_uint(g.srcparam2.w);
+        s.fx = to_short((int)floorf(p.x));
+        s.fy = to_short((int)floorf(p.y));
+        s.fz = to_short((int)floorf(p.z));
+        f.w += 1.f;
+        s.prop = getprop(sh, s.mediaid);
+        pp[pd + 1] += p.w;
+        s.w0 = p.w;
+        pp[pd + 2] = (g.srcnum > 1) ? pp[pd + 2] : p.w;
+        v.w = K_EPS;
+        s.Lmove = 0.f;
+        return 0;
     }
 
     if (g.seed == K_SEED_FROM_FILE) {
@@ -546,7 +596,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
         s.mediaid = __float_as_uint(lp2.w);
         s.prop = make_float4(lpos.x, lpos.y, lpos.z, 0.f);
 
-        if (!PENCIL) {
+        if (!SIMPLE) {
             const int st = (int)g.srctype;
             bool positioned = false;
 
@@ -768,12 +818,22 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
     return 0;
 }
 
-template <bool SAVEDET, bool REFLECT, bool PENCIL>
-__global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
+template <bool SAVEDET, bool REFLECT, bool SIMPLE>
+#ifndef MCX_MAX_BLOCK
+#define MCX_MAX_BLOCK 256
+#endif
+#ifdef MCX_WAVES_PER_EU
+#define MCX_KATTR __attribute__((amdgpu_waves_per_eu(MCX_WAVES_PER_EU)))
+#else
+#define MCX_KATTR
+#endif
+__global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(const MCXKParam g) {
     extern __shared__ float4 lds[];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t pplen = g.w0offset + g.srcnum;
+    __shared__ uint32_t pool[2];
     Shared sh;
+    sh.pool = pool;
     sh.prop = lds;
     sh.invcdf = (float*)(lds + g.medianum);
     sh.box = (double*)(sh.invcdf + ((g.nphase + 3) & ~3u));
@@ -792,6 +852,14 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
         sh.box[i] = 0.0;
     }
 
+    if (g.dynamic && threadIdx.x == 0) {
+        /* this workgroup's share: threadphoton photons, +1 for the first oddphoton groups */
+        uint32_t b = blockIdx.x;
+        uint32_t start = b * g.threadphoton + (b < (uint32_t)g.oddphoton ? b : (uint32_t)g.oddphoton);
+        pool[0] = start;
+        pool[1] = start + g.threadphoton + (b < (uint32_t)g.oddphoton ? 1u : 0u);
+    }
+
     /* lane-interleaved scratch: element k of lane t at [k*blockDim + t] would avoid
        bank conflicts, but the record copy reads it sequentially; a per-lane block
        keeps the indexing simple and the scratch is touched rarely */
@@ -804,7 +872,8 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
     __syncthreads();
 
     /* lanes without photons skip the transport loop but still join the box flush */
-    const bool active = !((uint32_t)idx >= g.threadphoton * g.nthread + (uint32_t)g.oddphoton ||
+    const bool active = g.dynamic ? (uint32_t)idx < g.nthread :
+                        !((uint32_t)idx >= g.threadphoton * g.nthread + (uint32_t)g.oddphoton ||
                           (uint32_t)idx >= g.nthread);
     const uint32_t pd = g.partialdata;
     /* the tail workgroup may hold lanes past nthread: they read nothing */
@@ -839,7 +908,7 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
 
     while (active) {
         if (relaunch) {
-            if (launch_photon<SAVEDET, PENCIL>(g, sh, s, rng, idx, launchflag)) {
+            if (launch_photon<SAVEDET, SIMPLE>(g, sh, s, rng, idx, launchflag)) {
                 break;
             }
 
@@ -859,19 +928,18 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
             if (v.w != K_EPS) {
                 float cphi = 1.f, sphi = 0.f, theta, stheta, ctheta, tmp0 = 0.f;
 
-                if (!g.is2d) {
+                if (SIMPLE || !g.is2d) {
                     tmp0 = K_TWO_PI * rng.next();
                     ksincos(tmp0, &sphi, &cphi);
                 }
 
-                if (g.nphase > 2) {
+                if (!SIMPLE && g.nphase > 2) {
                     tmp0 = rng.next() * (float)(g.nphase - 1);
                     theta = tmp0 - (float)((int)tmp0);
                     uint32_t i0 = (uint32_t)tmp0 >= g.nphase ? g.nphase - 1 : (uint32_t)(int)tmp0;
                     uint32_t i1 = (uint32_t)tmp0 + 1 >= g.nphase ? g.nphase - 1 : (uint32_t)((int)tmp0 + 1);
                     tmp0 = (1.f - theta) * sh.invcdf[i0] + theta * sh.invcdf[i1];
-                    theta = kacos(tmp0);
-                    stheta = ksin(theta);
+                    stheta = ksin_acos(tmp0);
                     ctheta = tmp0;
                 } else {
                     tmp0 = (v.w > (float)g.gscatter) ? 0.f : s.prop.z;
@@ -881,12 +949,16 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
                         tmp0 *= tmp0;
                         tmp0 = kdiv(1.f + s.prop.z * s.prop.z - tmp0, 2.f * s.prop.z);
                         tmp0 = fmaxf(-1.f, fminf(1.f, tmp0));
-                        theta = kacos(tmp0);
-                        stheta = ksin(theta);
+                        stheta = ksin_acos(tmp0);
                         ctheta = tmp0;
                     } else {
+#if MCX_PARITY
                         theta = kacos(2.f * rng.next() - 1.f);
                         ksincos(theta, &stheta, &ctheta);
+#else
+                        ctheta = 2.f * rng.next() - 1.f;
+                        stheta = ksin_acos(ctheta);
+#endif
                     }
                 }
 
@@ -902,7 +974,7 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
                     }
                 }
 
-                if (g.is2d) {
+                if (!SIMPLE && g.is2d) {
                     rotatevector2d(v, (rng.next() > 0.5f ? stheta : -stheta), ctheta, (int)g.is2d);
                 } else {
                     rotatevector(v, stheta, ctheta, sphi, cphi);
@@ -910,7 +982,7 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
 
                 v.w += 1.f;
 
-                if (g.outputtype == kWP || g.outputtype == kDCS) {
+                if (!SIMPLE && (g.outputtype == kWP || g.outputtype == kDCS)) {
                     uint32_t pid = photon_index(g, idx, f.w) - 1;
                     tmp0 = (g.outputtype == kDCS) ? (1.f - ctheta) : 1.f;
                     tmp0 *= g.replayweight[pid];
@@ -994,7 +1066,7 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
                     weight = s.w0 - p.w;
                 } else if (g.outputtype == kFluence || g.outputtype == kFlux) {
                     weight = (s.prop.x < K_EPS) ? (s.w0 * pathlen) : kdiv(s.w0 - p.w, s.prop.x);
-                } else if (g.seed == K_SEED_FROM_FILE) {
+                } else if (!SIMPLE && g.seed == K_SEED_FROM_FILE) {
                     if (g.outputtype == kJacobian) {
                         uint32_t pid = photon_index(g, idx, f.w) - 1;
                         weight = g.replayweight[pid] * pathlen;
@@ -1005,7 +1077,7 @@ __global__ void __launch_bounds__(256) transport_kernel(const MCXKParam g) {
                     weight = s.w0 * pathlen;
                 }
 
-                if (g.extrasrclen && g.srcid < 0) {
+                if (!SIMPLE && g.extrasrclen && g.srcid < 0) {
                     tshift += ((int)sh.pp[g.w0offset - 1] - 1) * ((g.replaydet == -1) ? (int)g.detnum : 1) *
                               (int)g.maxgate;
                 }

@@ -8,7 +8,7 @@
 
 template <bool SAVEDET, bool REFLECT>
 static hipError_t launch_variant(const MCXKParam* prm, int nblock, int nthreadblock, size_t shmem, hipStream_t stream) {
-    if (prm->srctype == kPencil) {
+    if (prm->simple) {
         hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, true>), dim3(nblock), dim3(nthreadblock), shmem,
                            stream, *prm);
     } else {

@@ -28,6 +28,10 @@
 
 /* lanes per workgroup of the transport kernel (its __launch_bounds__) */
 #define MCX_LAUNCH_BLOCK ((size_t)256)
+#ifndef MCX_MAX_BLOCK
+#define MCX_MAX_BLOCK 256
+#endif
+#define MCX_LAUNCH_BLOCK_MAX ((size_t)MCX_MAX_BLOCK)
 /* LDS deposit box: edge in voxels and byte budget per workgroup */
 #define MCX_BOX_EDGE 16u
 #define MCX_BOX_BYTES (32u * 1024u)
@@ -268,6 +272,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     /* thread map: reference autopilot for AMD devices, or the user's nthread/nblocksize */
     size_t nblocksize = cfg->autopilot ? gpu[dev].autoblock : cfg->nblocksize;
     size_t nthread = cfg->autopilot ? gpu[dev].autothread : cfg->nthread;
+    const int ncu = gpu[dev].sm;
 
     if (nblocksize == 0) {
         free(gpu);
@@ -278,7 +283,20 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
        depends only on the thread count, so the transport kernel itself is launched
        in workgroups of MCX_LAUNCH_BLOCK lanes with the tail lanes idle */
     nthread = (nthread / nblocksize) * nblocksize;
-    const size_t nlaunch = (nthread + MCX_LAUNCH_BLOCK - 1) / MCX_LAUNCH_BLOCK;
+    size_t lblock = MCX_LAUNCH_BLOCK;
+    {
+        const char* env = getenv("MCXHIP_BLOCK");
+
+        if (env) {
+            lblock = (size_t)strtoul(env, NULL, 10);
+        }
+
+        if (lblock < 64 || lblock > MCX_LAUNCH_BLOCK_MAX || (lblock & 63)) {
+            free(gpu);
+            MCX_ERROR(-4, "MCXHIP_BLOCK must be a multiple of 64 within the kernel's launch bounds");
+        }
+    }
+    size_t nlaunch = (nthread + lblock - 1) / lblock;
 
     if (nthread == 0) {
         free(gpu);
@@ -366,6 +384,22 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         g.is2d = is2d;
     }
 
+    /* the specialised launch/scatter/deposit path: a pencil beam starting inside a
+       non-void voxel, no focus or angular launch, no extra sources, no replay and
+       only forward output types */
+    {
+        union {
+            float f;
+            unsigned int u;
+        } lab;
+        lab.f = cfg->srcparam2.w;
+        g.simple = cfg->srctype == MCX_SRC_PENCIL && cfg->srcdir.w == 0.f && (lab.u & MED_MASK) != 0 &&
+                   cfg->extrasrclen == 0 && cfg->seed != SEED_FROM_FILE && cfg->issaveref == 0 && g.is2d == 0 &&
+                   cfg->nphase <= 2 && cfg->srcpos.w > 0.f && cfg->minenergy < cfg->srcpos.w &&
+                   (cfg->outputtype == otFlux || cfg->outputtype == otFluence || cfg->outputtype == otEnergy ||
+                    cfg->outputtype == otL);
+    }
+
     /* photon split across the persistent threads (src/mcx_host.cpp:1101-1102) */
     g.threadphoton = (uint32_t)(cfg->nphoton / nthread);
     g.oddphoton = (int32_t)(cfg->nphoton - (size_t)g.threadphoton * nthread);
@@ -386,7 +420,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
 
     size_t pplen = g.w0offset + g.srcnum;
     size_t shmem = cfg->medianum * sizeof(float4) + ((cfg->nphase + 3) & ~3u) * sizeof(float) +
-                   MCX_LAUNCH_BLOCK * pplen * sizeof(float);
+                   lblock * pplen * sizeof(float);
 
     if (shmem > 64 * 1024) {
         MCX_ERROR(-4, "too many media/phase samples for the LDS tables");
@@ -464,6 +498,26 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         }
 
         g.nrep = nr ? (uint32_t)nr : 1u;
+    }
+
+    /* fast mode: one resident wave of workgroups, each drawing its share of photons
+       one at a time, so lanes stay busy until their workgroup's share is done
+       instead of each running a fixed count (per-lane work varies by +-16% at
+       150 photons per lane and a wave waits for its slowest lane).  Bit-parity
+       mode keeps the reference's fixed photon-per-thread map. */
+    {
+        const char* env = getenv("MCXHIP_DYNAMIC");
+        int want = cfg->mathmode == 0 && !g.debugrng && cfg->seed != SEED_FROM_FILE && !(env && env[0] == '0');
+        int nb = want ? mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, (int)g.simple) : 0;
+
+        if (nb > 0 && ncu > 0) {
+            nlaunch = (size_t)nb * (size_t)ncu;
+            nthread = nlaunch * lblock;
+            g.nthread = (uint32_t)nthread;
+            g.dynamic = 1;
+            g.threadphoton = (uint32_t)(cfg->nphoton / nlaunch);
+            g.oddphoton = (int32_t)(cfg->nphoton - (size_t)g.threadphoton * nlaunch);
+        }
     }
 
     if (cfg->seed == SEED_FROM_FILE &&
@@ -580,12 +634,12 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     HIP_TRY(hipEventRecord(ev0, stream));
 
     if (g.debugrng) {
-        HIP_TRY(mcx_launch_rngdump(&g, (int)nlaunch, (int)MCX_LAUNCH_BLOCK, stream));
+        HIP_TRY(mcx_launch_rngdump(&g, (int)nlaunch, (int)lblock, stream));
     } else if (cfg->mathmode == 1) {
-        HIP_TRY(mcx_launch_parity(&g, (int)nlaunch, (int)MCX_LAUNCH_BLOCK, shmem, stream, cfg->issavedet != 0,
+        HIP_TRY(mcx_launch_parity(&g, (int)nlaunch, (int)lblock, shmem, stream, cfg->issavedet != 0,
                                   reflect));
     } else {
-        HIP_TRY(mcx_launch_fast(&g, (int)nlaunch, (int)MCX_LAUNCH_BLOCK, shmem, stream, cfg->issavedet != 0,
+        HIP_TRY(mcx_launch_fast(&g, (int)nlaunch, (int)lblock, shmem, stream, cfg->issavedet != 0,
                                 reflect));
     }
 

@@ -119,6 +119,31 @@ def test_source_types_parity(srctype, extra):
     np.testing.assert_allclose(gpu["flux64"], ref["flux64"], rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("name,extra", [
+    ("focused pencil", {"srcdir": [0, 0, 1, 20.0]}),
+    ("detector flags on the z=0 face", {"bc": "______001000", "issavedet": 1, "detpos": [[29, 29, 0, 3]]}),
+    ("pencil from outside the grid", {"srcpos": [29.5, 29.5, -3.0]}),
+    ("diffuse reflectance into an air layer", "airlayer"),
+])
+def test_general_pencil_paths_parity(name, extra):
+    """pencil beams that take the general launch/deposit code (not the specialised one)"""
+    if extra == "airlayer":
+        vol = np.ones((40, 40, 40), np.uint8)
+        vol[:, :, :2] = 0
+        cfg = dict(nphoton=20000, vol=vol, prop=[[0, 0, 1, 1], [0.005, 1.0, 0.01, 1.37]], srcpos=[20, 20, 0],
+                   srcdir=[0, 0, 1], tstart=0, tend=5e-9, tstep=5e-9, issaveref=1, isreflect=1, seed=1648335518)
+    else:
+        cfg = workloads.homogeneous_cube_workload(nphoton=20000, **extra)
+    gpu, ref = _parity_pair(cfg, 2048)
+    assert gpu["stat"]["energytot"] == ref["stat"]["energytot"]
+    assert gpu["stat"]["energyabs"] == ref["stat"]["energyabs"]
+    np.testing.assert_allclose(gpu["flux64"], ref["flux64"], rtol=1e-9, atol=1e-12)
+    if "detp" in ref:
+        assert np.array_equal(sorted_records(gpu["detp"]), sorted_records(ref["detp"]))
+    if extra == "airlayer":
+        assert (ref["flux64"] < 0).sum() > 100   # reflectance deposits happened
+
+
 @pytest.mark.parametrize("bc", ["aaaaaa", "rrrrrr", "mmmmmm", "ccrrrr", "______111111"])
 def test_boundary_conditions_parity(bc):
     cfg = workloads.homogeneous_cube_workload(nphoton=10000, bc=bc, issavedet=1 if "1" in bc else 0)
@@ -131,10 +156,13 @@ def test_boundary_conditions_parity(bc):
 
 
 def test_fast_mode_statistics():
-    """hardware-transcendental build: same physics within Monte Carlo error"""
-    cfg = workloads.homogeneous_cube_workload(nphoton=1000000)
+    """hardware-transcendental build with run-time photon scheduling: every photon
+    launched exactly once, same physics within Monte Carlo error"""
+    cfg = workloads.homogeneous_cube_workload(nphoton=1000003)
     fast = mcxcl_amd.run(dict(cfg, mathmode=0))
     par = mcxcl_amd.run(dict(cfg, mathmode=1))
+    assert fast["stat"]["energytot"] == 1000003.0
+    assert par["stat"]["energytot"] == 1000003.0
     af = fast["stat"]["energyabs"] / fast["stat"]["energytot"]
     ap = par["stat"]["energyabs"] / par["stat"]["energytot"]
     assert abs(af - ap) < 3e-3, (af, ap)
