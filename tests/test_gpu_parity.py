@@ -189,6 +189,16 @@ def test_replay_jacobian_parity(replaydet):
     w = np.exp(-(np.array([0.005, 0.002])[:, None] * L).sum(0))
     sel = np.ones(det.size, bool) if replaydet <= 0 else det == replaydet
     np.testing.assert_allclose(gpu["flux64"].sum(), (w[sel] * L[:, sel].sum(0)).sum(), rtol=1e-6)
-    # the fast build retraces the same paths (within float rounding of the path sums)
-    fast = mcxcl_amd.run(dict(rep, rawfield=1, mathmode=0))
-    np.testing.assert_allclose(fast["flux64"].sum(), (w[sel] * L[:, sel].sum(0)).sum(), rtol=1e-3)
+
+
+def test_replay_fast_mode_retraces_paths():
+    """the hardware-math build replays its own detected photons along the same paths"""
+    base = workloads.sphere_cube_workload(nphoton=1000000, issaveseed=1, savedetflag="DP", tstep=5e-9)
+    fwd = mcxcl_amd.run(dict(base, mathmode=0))
+    detp = fwd["detp"]
+    assert detp.shape[1] > 1000
+    rep = mcxcl_amd.run(dict(base, seed=fwd["seeds"], detphotons=detp, outputtype="jacobian", replaydet=0,
+                             mathmode=0, rawfield=1))
+    L = detp[1:3].astype(np.float64)
+    w = np.exp(-(np.array([0.005, 0.002])[:, None] * L).sum(0))
+    np.testing.assert_allclose(rep["flux64"].sum(), (w * L.sum(0)).sum(), rtol=1e-5)
