@@ -315,6 +315,11 @@ typedef struct MCXConfig {
     float jobms;             /* transport kernel plus fluence finalisation, from HIP events (ms) */
     double runtimems;        /* wall-clock of the photon loop of the last run (ms) */
     double* exportfield64;   /* optional: unnormalised double-precision field (fieldlen entries, caller-owned) */
+    float* muamus;           /* optional: per-voxel {mua, mus} in 1/mm (2 x nvox floats, x fastest, caller-owned);
+                                voxels with a non-zero label take mua and mus from here, g and n from the label */
+    float* exportdetweight;  /* optional: exit weight of each detected photon (maxdetphoton floats, caller-owned) */
+    float* replayinweight;   /* optional: replay weight of each row of detphotons, overriding the weight computed
+                                from partial path lengths (needed with muamus) */
 } Config;
 
 /* ---- configuration API; each line names the reference declaration it replaces ---- */

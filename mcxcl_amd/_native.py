@@ -132,6 +132,7 @@ CONFIG_FIELDS = [
     ("angleinvcdf", ctypes.c_void_p), ("srcid", ctypes.c_int), ("extrasrclen", ctypes.c_uint),
     ("srcdata", ctypes.c_void_p), ("mathmode", ctypes.c_int), ("kernelms", ctypes.c_float),
     ("jobms", ctypes.c_float), ("runtimems", ctypes.c_double), ("exportfield64", ctypes.c_void_p),
+    ("muamus", ctypes.c_void_p), ("exportdetweight", ctypes.c_void_p), ("replayinweight", ctypes.c_void_p),
 ]
 
 lib.mcx_config_sizeof.restype = ctypes.c_size_t

@@ -44,6 +44,8 @@ struct MCXKParam {
     const uint64_t* replayseed;   /* 2 words per replayed photon */
     const uint32_t* nseed;        /* 4 words per thread */
     double* field;                /* fluence accumulator, nrep x fieldlen entries */
+    const float2* muamus;         /* optional per-voxel {mua, mus} (1/grid unit); NULL: from the label */
+    float* detweight;             /* optional exit weight of each detected photon */
     float* genergy;               /* {escaped, launched} per thread */
     float* detrec;                /* detected-photon records, reclen floats each */
     uint64_t* detseed;            /* 2 words per detected photon */

@@ -343,8 +343,17 @@ __device__ __forceinline__ void deposit_voxel(const MCXKParam& g, const Shared& 
     }
 }
 
-__device__ __forceinline__ float4 getprop(const Shared& sh, uint32_t mediaid) {
-    return sh.prop[mediaid & K_MED_MASK];
+__device__ __forceinline__ float4 getprop(const MCXKParam& g, const Shared& sh, uint32_t mediaid, uint32_t idx1d) {
+    float4 p = sh.prop[mediaid & K_MED_MASK];
+
+    /* per-voxel absorption and scattering: g and n stay those of the label */
+    if (g.muamus && (mediaid & K_MED_MASK)) {
+        float2 m = g.muamus[idx1d];
+        p.x = m.x;
+        p.y = m.y;
+    }
+
+    return p;
 }
 
 /* src/mcx_core.cl:1350-1455 */
@@ -488,6 +497,10 @@ __device__ void save_detected(const MCXKParam& g, const Shared& sh, const Photon
         g.detseed[(size_t)slot * 2 + 1] = s.seed1;
     }
 
+    if (g.detweight) {
+        g.detweight[slot] = s.p.w;   /* weight at exit: w0 * exp(-sum mua L) */
+    }
+
     float* rec = g.detrec + (size_t)slot * g.reclen;
     uint32_t k = 0;
 
@@ -587,7 +600,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
         s.fy = to_short((int)floorf(p.y));
         s.fz = to_short((int)floorf(p.z));
         f.w += 1.f;
-        s.prop = getprop(sh, s.mediaid);
+        s.prop = getprop(g, sh, s.mediaid, s.idx1d);
         pp[pd + 1] += p.w;
         s.w0 = p.w;
         pp[pd + 2] = (g.srcnum > 1) ? pp[pd + 2] : p.w;
@@ -854,7 +867,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
     } while ((s.mediaid & K_MED_MASK) == 0 || fabsf(p.w) <= g.minenergy);
 
     f.w += 1.f;
-    s.prop = getprop(sh, s.mediaid);
+    s.prop = getprop(g, sh, s.mediaid, s.idx1d);
     ppl[1] += p.w;
     s.w0 = p.w;
     ppl[2] = (g.srcnum > 1) ? ppl[2] : p.w;
@@ -1050,7 +1063,7 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
         }
 
         n1 = s.prop.w;
-        s.prop = getprop(sh, s.mediaid);
+        s.prop = getprop(g, sh, s.mediaid, s.idx1d);
         int face;
         f.z = hitgrid(p, v, s.fx, s.fy, s.fz, &face);
         s.fw = face;
@@ -1185,7 +1198,7 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
         }
 
         if (REFLECT) {
-            s.prop = getprop(sh, s.mediaid);
+            s.prop = getprop(g, sh, s.mediaid, s.idx1d);
 
             if (((s.mediaid && g.doreflect) ||
                     (s.mediaid == 0 && (((isdet & 0xF) == 0 && g.doreflect) || ((isdet & 0xF) == kReflect ||
@@ -1234,7 +1247,7 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
                     s.idx1d = idx1dold;
                     keepold = true;
                     s.mediaid = (g.media[s.idx1d] & K_MED_MASK);
-                    s.prop = getprop(sh, s.mediaid);
+                    s.prop = getprop(g, sh, s.mediaid, s.idx1d);
                     n1 = s.prop.w;
                     /* drop the boundary code so a later same-voxel step restores
                        the voxel's own label (the reference ORs the code in) */

@@ -549,6 +549,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     float* dgenergy = NULL;
     float* ddetrec = NULL;
     uint64_t* ddetseed = NULL;
+    float* ddetw = NULL;
+    float2* dmuamus = NULL;
     uint32_t* ddetcount = NULL;
     float* dout = NULL;
     size_t nrep = 0;
@@ -611,6 +613,33 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         if (cfg->issaveseed) {
             HIP_TRY(dalloc(bufs, &ddetseed, (size_t)cfg->maxdetphoton * 2));
         }
+
+        if (cfg->exportdetweight) {
+            HIP_TRY(dalloc(bufs, &ddetw, (size_t)cfg->maxdetphoton));
+        }
+    }
+
+    if (cfg->muamus) {
+        /* per-voxel mua/mus, converted from 1/mm to 1/grid unit as the label table is */
+        float2* hm = (float2*)malloc(nvox * sizeof(float2));
+
+        if (!hm) {
+            HIP_TRY(hipErrorOutOfMemory);
+        }
+
+        for (size_t i = 0; i < nvox; i++) {
+            hm[i].x = cfg->muamus[2 * i] * cfg->unitinmm;
+            hm[i].y = cfg->muamus[2 * i + 1] * cfg->unitinmm;
+        }
+
+        hipError_t e = dalloc(bufs, &dmuamus, nvox);
+
+        if (e == hipSuccess) {
+            e = hipMemcpy(dmuamus, hm, nvox * sizeof(float2), hipMemcpyHostToDevice);
+        }
+
+        free(hm);
+        HIP_TRY(e);
     }
 
     g.media = dmedia;
@@ -628,6 +657,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     g.detrec = ddetrec;
     g.detseed = ddetseed;
     g.detcount = ddetcount;
+    g.detweight = ddetw;
+    g.muamus = dmuamus;
 
     HIP_TRY(hipStreamSynchronize(stream));
     t0 = wall_ms();
@@ -687,6 +718,10 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             if (nsave) {
                 HIP_TRY(hipMemcpy(cfg->seeddata, ddetseed, (size_t)nsave * 16, hipMemcpyDeviceToHost));
             }
+        }
+
+        if (ddetw && nsave) {
+            HIP_TRY(hipMemcpy(cfg->exportdetweight, ddetw, (size_t)nsave * sizeof(float), hipMemcpyDeviceToHost));
         }
 
         cfg->detectedcount = nsave;

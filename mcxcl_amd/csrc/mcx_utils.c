@@ -1568,6 +1568,10 @@ void mcx_replayinit(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
             tof += plen * R_C0 * cfg->prop[j - hasdetid + 1].n;
         }
 
+        if (cfg->replayinweight) {
+            w = cfg->replayinweight[i];   /* per-voxel media: the recorded exit weight */
+        }
+
         cfg->replay.weight[kept] = w;
         cfg->replay.tof[kept] = tof;
         cfg->replay.detid[kept] = hasdetid ? detword : 1;
@@ -1905,7 +1909,7 @@ static const struct {
     F(faststep), F(normalizer), F(replay), F(seeddata), F(replaydet), F(seedfile), F(jsonfile), F(maxjumpdebug),
     F(debugdatalen), F(gscatter), F(exportdebugdata), F(savedetflag), F(detectedcount), F(runtime), F(parentid),
     F(optlevel), F(mediabyte), F(bc), F(nphase), F(invcdf), F(nangle), F(angleinvcdf), F(srcid), F(extrasrclen),
-    F(srcdata), F(mathmode), F(kernelms), F(jobms), F(runtimems), F(exportfield64)
+    F(srcdata), F(mathmode), F(kernelms), F(jobms), F(runtimems), F(exportfield64), F(muamus), F(exportdetweight), F(replayinweight)
 };
 #undef F
 

@@ -54,8 +54,13 @@ def _parse_volume(user, c):
     if "vol" not in user:
         raise ValueError("Configuration must specify a 2/3/4D volume.")
     vol = np.asarray(user["vol"])
-    if vol.ndim == 4:
-        raise ValueError("4-D (multi-word) volumes are not supported by this build")
+    if vol.ndim == 4 and vol.shape[0] == 2 and vol.dtype == np.float32:
+        # per-voxel {mua, mus} in 1/mm (the reference stores these as two halves,
+        # src/pmcxcl.cpp parseVolume, MEDIA_AS_F2H); here they stay float32, and a
+        # voxel with both zero is background (label 0), any other is label 1
+        vol = (np.any(vol != 0, axis=0)).astype(np.uint8)
+    elif vol.ndim == 4:
+        raise ValueError("4-D volumes must be float32 [2, Nx, Ny, Nz] per-voxel {mua, mus}")
     if vol.ndim != 3:
         raise ValueError("the 'vol' field must be a 3-D array")
     if vol.dtype in (np.uint8, np.int8):
@@ -228,6 +233,7 @@ class PreparedConfig:
         self._keep = []
         try:
             _parse_config(user, self.c)
+            self._set_pervoxel(user)
             detps_ptr, dims, seedbyte = None, None, 0
             if "detphotons" in user:
                 dp = np.asfortranarray(np.asarray(user["detphotons"], dtype=np.float32))
@@ -256,9 +262,41 @@ class PreparedConfig:
             # unnormalised double-precision accumulator, returned as "flux64"
             self.raw = np.zeros(self.fieldlen, dtype=np.float64)
             c.exportfield64 = self.raw.ctypes.data
+        self._set_detweight(user)
+
+    def _set_pervoxel(self, user):
+        """per-voxel media ('vol' as float32 [2,Nx,Ny,Nz], or 'muamus' beside a label
+        volume), replay weights ('replayweight') and exit-weight output ('savedetweight')"""
+        c = self.c
+        mm = None
+        v = np.asarray(user["vol"]) if "vol" in user else None
+        if v is not None and v.ndim == 4 and v.shape[0] == 2 and v.dtype == np.float32:
+            mm = v
+        if "muamus" in user:
+            mm = np.asarray(user["muamus"], dtype=np.float32)
+        if mm is not None:
+            if mm.shape != (2, c.dim.x, c.dim.y, c.dim.z):
+                raise ValueError("per-voxel mua/mus must be float32 [2, Nx, Ny, Nz] matching the volume")
+            if (mm < 0).any() or not np.isfinite(mm).all():
+                raise ValueError("per-voxel mua/mus must be finite and non-negative")
+            flat = np.asfortranarray(mm).ravel(order="F")   # mua, mus interleaved per voxel
+            self._keep.append(flat)
+            c.muamus = flat.ctypes.data
+        if "replayweight" in user:
+            w = np.ascontiguousarray(np.asarray(user["replayweight"], dtype=np.float32).ravel())
+            self._keep.append(w)
+            c.replayinweight = w.ctypes.data
+
+    def _set_detweight(self, user):
+        self.detw = None
+        if user.get("savedetweight") and self.c.issavedet:
+            self.detw = np.zeros(max(1, int(self.c.maxdetphoton)), dtype=np.float32)
+            self.c.exportdetweight = self.detw.ctypes.data
 
     def collect(self):
         out = _collect(self.c, self.user, self.ngate, self.ndet, self.nsrc)
+        if self.detw is not None:
+            out["detw"] = self.detw[:self.c.detectedcount].copy()
         if self.raw is not None:
             dims = [self.c.dim.x, self.c.dim.y, self.c.dim.z, self.ngate * self.ndet * self.nsrc]
             out["flux64"] = self.raw.reshape(dims, order="F").copy()
@@ -267,6 +305,9 @@ class PreparedConfig:
     def close(self):
         if self.c is not None:
             self.c.exportfield64 = None
+            self.c.muamus = None
+            self.c.exportdetweight = None
+            self.c.replayinweight = None
             lib.mcx_clearcfg(ctypes.byref(self.c))
             self.c = None
 
@@ -288,7 +329,11 @@ def run(cfg=None, **kwargs):
 
     Keys beyond the reference's: ``mathmode`` (0: hardware transcendentals,
     1: bit-reproducible arithmetic), ``rawfield`` (also return the unnormalised
-    double-precision accumulator as ``flux64``)."""
+    double-precision accumulator as ``flux64``), ``muamus`` (float32
+    [2, Nx, Ny, Nz] per-voxel mua/mus in 1/mm beside a label ``vol``),
+    ``savedetweight`` (return each detected photon's exit weight as ``detw``) and
+    ``replayweight`` (replay weights per row of ``detphotons``, required with
+    per-voxel media)."""
     with prepare(cfg, **kwargs) as pc:
         c = pc.c
         if c.issave2pt:

@@ -111,12 +111,14 @@ typedef struct {
     const float* replaytof;
     const int* replaydetid;
     const uint64_t* replayseed;
+    const float* muamus;  /* optional per-voxel {mua, mus} in 1/grid unit (2 per voxel) */
 } OParam;
 
 typedef struct {
     double* field;        /* private double field, fieldlen entries */
     float* det;           /* private detected-photon records */
     uint64_t* seeds;      /* private detected-photon seeds (2 words each) */
+    float* detw;          /* private exit weights of the detected photons */
     size_t ndet, capdet;
     OracleCounters cnt;
 } OWork;
@@ -240,6 +242,19 @@ static inline int inside(const OParam* g, short x, short y, short z) {
 
 static inline F4 getprop(const OParam* g, uint32_t mediaid) {
     return g->prop[mediaid & MED_MASK];
+}
+
+/* the properties of the photon's voxel: with per-voxel media the label's
+   mua and mus are replaced by the voxel's own, g and n stay the label's */
+static inline F4 getvoxprop(const OParam* g, uint32_t mediaid, uint32_t idx1d) {
+    F4 p = g->prop[mediaid & MED_MASK];
+
+    if (g->muamus && (mediaid & MED_MASK)) {
+        p.x = g->muamus[2 * (size_t)idx1d];
+        p.y = g->muamus[2 * (size_t)idx1d + 1];
+    }
+
+    return p;
 }
 
 /* src/mcx_core.cl:1350-1455 */
@@ -375,8 +390,10 @@ static void save_detected(const OParam* g, OWork* w, OPhoton* s, uint32_t isdet)
         w->capdet = w->capdet ? w->capdet * 2 : 1024;
         w->det = (float*)realloc(w->det, w->capdet * g->reclen * sizeof(float));
         w->seeds = (uint64_t*)realloc(w->seeds, w->capdet * 2 * sizeof(uint64_t));
+        w->detw = (float*)realloc(w->detw, w->capdet * sizeof(float));
     }
 
+    w->detw[w->ndet] = s->p.w;
     float* rec = w->det + w->ndet * g->reclen;
     w->seeds[w->ndet * 2] = s->photonseed[0];
     w->seeds[w->ndet * 2 + 1] = s->photonseed[1];
@@ -738,7 +755,7 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
     } while ((s->mediaid & MED_MASK) == 0 || pm_fabs(p->w) <= g->minenergy);
 
     f->w += 1.f;
-    s->prop = getprop(g, s->mediaid);
+    s->prop = getvoxprop(g, s->mediaid, s->idx1d);
     pp[1] += p->w;
     s->w0 = p->w;
     pp[2] = (g->srcnum > 1) ? pp[2] : p->w;
@@ -859,7 +876,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
         }
 
         n1 = s->prop.w;
-        s->prop = getprop(g, s->mediaid);
+        s->prop = getvoxprop(g, s->mediaid, s->idx1d);
         f->z = hitgrid(p, v, fd);
         float gfac = (v->w + 1.f > (float)g->gscatter ? (1.f - s->prop.z) : 1.f);
         float slen = f->z * s->prop.y * gfac;
@@ -999,7 +1016,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
         }
 
         if (g->reflectblock) {
-            s->prop = getprop(g, s->mediaid);
+            s->prop = getvoxprop(g, s->mediaid, s->idx1d);
 
             if (((s->mediaid && g->doreflect) ||
                     (s->mediaid == 0 && (((isdet & 0xF) == bcUnknown && g->doreflect) || ((isdet & 0xF) == bcReflect ||
@@ -1008,7 +1025,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                 float Rtotal = 1.f;
                 float cphi, sphi, stheta, ctheta;
                 w->cnt.nbnd += 1;
-                s->prop = getprop(g, s->mediaid);
+                s->prop = getvoxprop(g, s->mediaid, s->idx1d);
                 float tmp0 = n1 * n1;
                 float tmp1 = s->prop.w * s->prop.w;
                 cphi = pm_fabs((fd->w == 0) ? v->x : (fd->w == 1 ? v->y : v->z));
@@ -1059,7 +1076,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
 
                     s->idx1d = idx1dold;
                     s->mediaid = (g->media[s->idx1d] & MED_MASK);
-                    s->prop = getprop(g, s->mediaid);
+                    s->prop = getvoxprop(g, s->mediaid, s->idx1d);
                     n1 = s->prop.w;
                     isdet &= DET_MASK;
                 }
@@ -1200,6 +1217,18 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
     g.prop = props;
     g.detpos = (const F4*)cfg->detpos;
     g.media = cfg->vol;
+    float* muamus = NULL;
+
+    if (cfg->muamus) {
+        size_t nv = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
+        muamus = (float*)malloc(2 * nv * sizeof(float));
+
+        for (size_t i = 0; i < 2 * nv; i++) {
+            muamus[i] = cfg->muamus[i] * cfg->unitinmm;   /* 1/mm -> 1/grid unit, as the label table */
+        }
+    }
+
+    g.muamus = muamus;
 
     size_t nphoton = cfg->nphoton;
     g.threadphoton = (uint32_t)(nphoton / nthread);
@@ -1313,6 +1342,10 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
                 memcpy((char*)cfg->seeddata + k * 16, works[i].seeds, take * 16);
             }
 
+            if (cfg->exportdetweight && take) {
+                memcpy(cfg->exportdetweight + k, works[i].detw, take * sizeof(float));
+            }
+
             k += take;
         }
 
@@ -1323,6 +1356,7 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
         free(works[i].field);
         free(works[i].det);
         free(works[i].seeds);
+        free(works[i].detw);
     }
 
     free(works);
@@ -1409,6 +1443,7 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
     res->nthread = nthread;
     free(nseed);
     free(props);
+    free(muamus);
     return 0;
 }
 

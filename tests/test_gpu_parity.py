@@ -202,3 +202,20 @@ def test_replay_fast_mode_retraces_paths():
     L = detp[1:3].astype(np.float64)
     w = np.exp(-(np.array([0.005, 0.002])[:, None] * L).sum(0))
     np.testing.assert_allclose(rep["flux64"].sum(), (w * L.sum(0)).sum(), rtol=1e-5)
+
+
+def test_pervoxel_media_parity():
+    """per-voxel mua/mus field: device equals oracle, exit weights included"""
+    base = workloads.sphere_cube_workload(nphoton=50000, issaveseed=1, savedetflag="DPX", tstep=5e-10)
+    vol = base["vol"]
+    prop = np.asarray(base["prop"], np.float32)
+    ii, jj, kk = np.meshgrid(np.arange(60), np.arange(60), np.arange(60), indexing="ij")
+    pert = 1 + 0.3 * np.sin(ii / 7.0) * np.cos(jj / 9.0) * np.sin(kk / 5.0)
+    mm = np.stack([prop[vol, 0] * pert, prop[vol, 1] * (2 - pert)]).astype(np.float32)
+    gpu, ref = _parity_pair(dict(base, muamus=mm, savedetweight=1), 4096)
+    assert gpu["stat"]["energytot"] == ref["stat"]["energytot"]
+    assert gpu["stat"]["energyabs"] == ref["stat"]["energyabs"]
+    np.testing.assert_allclose(gpu["flux64"], ref["flux64"], rtol=1e-9, atol=1e-12)
+    a = np.vstack([gpu["detp"], gpu["detw"][None, :]])
+    b = np.vstack([ref["detp"], ref["detw"][None, :]])
+    assert np.array_equal(sorted_records(a), sorted_records(b))
