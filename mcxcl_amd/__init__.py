@@ -10,3 +10,6 @@ from .pmcxcl import run, gpuinfo, version  # noqa: F401
 from . import workloads, anchors  # noqa: F401
 
 __all__ = ["run", "gpuinfo", "version", "workloads", "anchors"]
+
+# mcxcl_amd.autograd (torch.autograd over the replay adjoint) is imported on demand:
+# it needs torch, the rest of the package does not.
