@@ -133,6 +133,14 @@ __device__ __forceinline__ float ksin_acos(float x) {
 #endif
 }
 
+/* LDS deposit box cells: float64 keeps the bit-reproducible build's sums exact
+   to summation order; the fast build halves the footprint with float32 */
+#if MCX_PARITY
+typedef double BoxCell;
+#else
+typedef float BoxCell;
+#endif
+
 /* ---------------- RNG: xorshift128+ (src/mcx_core.cl:684-698) ---------------- */
 struct Rng {
     uint64_t s0, s1;
@@ -279,7 +287,7 @@ struct Photon {
 struct Shared {
     float4* prop;                /* medium table */
     float* invcdf;
-    double* box;                 /* privatised deposit box (box_cells) */
+    BoxCell* box;                /* privatised deposit box (box_cells) */
     float* pp;                   /* this lane's scratch: partial data, energies, w0, srcid */
     size_t repoff;               /* offset of this lane's global accumulator replica */
     uint32_t* pool;              /* dynamic schedule: {next photon, end} of this workgroup */
@@ -337,7 +345,7 @@ __device__ __forceinline__ void deposit_voxel(const MCXKParam& g, const Shared& 
     uint32_t dz = (uint32_t)(oz - (int)g.box_z0);
 
     if (dx < g.box_nx && dy < g.box_ny && dz < g.box_nz && (uint32_t)tshift < g.box_ng) {
-        atomicAdd(sh.box + (((uint32_t)tshift * g.box_nz + dz) * g.box_ny + dy) * g.box_nx + dx, (double)val);
+        atomicAdd(sh.box + (((uint32_t)tshift * g.box_nz + dz) * g.box_ny + dy) * g.box_nx + dx, (BoxCell)val);
     } else {
         depcache_add(g, sh, c, idx1d + (uint32_t)tshift * g.nvox, val);
     }
@@ -894,7 +902,7 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
     sh.pool = pool;
     sh.prop = lds;
     sh.invcdf = (float*)(lds + g.medianum);
-    sh.box = (double*)(sh.invcdf + ((g.nphase + 3) & ~3u));
+    sh.box = (BoxCell*)(sh.invcdf + ((g.nphase + 3) & ~3u));
     float* ppbase = (float*)(sh.box + g.box_cells);
     sh.repoff = (size_t)(idx & (int)(g.nrep - 1)) * g.fieldlen;
 
@@ -907,7 +915,7 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
     }
 
     for (uint32_t i = threadIdx.x; i < g.box_cells; i += blockDim.x) {
-        sh.box[i] = 0.0;
+        sh.box[i] = (BoxCell)0;
     }
 
     if (g.dynamic && threadIdx.x == 0) {
@@ -1269,7 +1277,7 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
         const size_t boff = (size_t)(blockIdx.x & (g.nrep - 1)) * g.fieldlen;
 
         for (uint32_t i = threadIdx.x; i < g.box_cells; i += blockDim.x) {
-            double val = sh.box[i];
+            double val = (double)sh.box[i];
 
             if (val != 0.0) {
                 uint32_t x = i % g.box_nx, r = i / g.box_nx;

@@ -33,7 +33,7 @@
 #endif
 #define MCX_LAUNCH_BLOCK_MAX ((size_t)MCX_MAX_BLOCK)
 /* LDS deposit box: edge in voxels and byte budget per workgroup */
-#define MCX_BOX_EDGE 16u
+#define MCX_BOX_EDGE_MAX 24u
 #define MCX_BOX_BYTES (32u * 1024u)
 #define MCX_LDS_LIMIT (64u * 1024u)
 /* global accumulator replicas (power of two) and their total byte cap */
@@ -427,10 +427,14 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     }
 
     /* deposit privatisation: most fluence deposits, and every heavily contended
-       cache line, fall in the first gates around the source.  A box of up to
-       MCX_BOX_EDGE^3 voxels centred on the source voxel (clamped to the grid) is
-       accumulated per workgroup in LDS and added to the global grid once at the end. */
+       cache line, fall in the first gates around the source.  A box centred on the
+       source voxel (clamped to the grid) is accumulated per workgroup in LDS and
+       added to the global grid once at the end.  Cells are float64 in the
+       bit-reproducible build and float32 in the fast build (the reference
+       accumulates the whole grid in float32); the budget buys two leading gates of
+       16^3 voxels when it can, else one gate of the largest cube that fits. */
     {
+        const size_t elem = (cfg->mathmode == 1) ? sizeof(double) : sizeof(float);
         size_t budget = MCX_BOX_BYTES;
         const char* env = getenv("MCXHIP_BOX_BYTES");
 
@@ -438,18 +442,21 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             budget = (size_t)strtoul(env, NULL, 10);
         }
 
-        if (g.debugrng || !cfg->issave2pt || budget < sizeof(double)) {
+        if (g.debugrng || !cfg->issave2pt || budget < elem) {
             budget = 0;
+        }
+
+        const size_t cells = budget / elem;
+        size_t ng = (cells >= 2u * 16u * 16u * 16u && maxgate >= 2) ? 2 : 1;
+        unsigned int maxedge = 1;
+
+        while (maxedge < MCX_BOX_EDGE_MAX && (size_t)(maxedge + 1) * (maxedge + 1) * (maxedge + 1) * ng <= cells) {
+            maxedge++;
         }
 
         unsigned int edge[3] = {cfg->dim.x, cfg->dim.y, cfg->dim.z};
         float spos[3] = {cfg->srcpos.x, cfg->srcpos.y, cfg->srcpos.z};
         unsigned int org[3];
-        unsigned int maxedge = MCX_BOX_EDGE;
-
-        while (maxedge > 1 && (size_t)maxedge * maxedge * maxedge * sizeof(double) > budget) {
-            maxedge--;
-        }
 
         for (int i = 0; i < 3; i++) {
             edge[i] = edge[i] < maxedge ? edge[i] : maxedge;
@@ -459,10 +466,11 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         }
 
         size_t pergate = (size_t)edge[0] * edge[1] * edge[2];
-        size_t ng = budget / (pergate * sizeof(double));
+        ng = (pergate && cells >= pergate) ? cells / pergate : 0;
+        ng = ng < 2 ? ng : 2;
         ng = ng < maxgate ? ng : maxgate;
 
-        if (ng > 0 && shmem + 8 + ng * pergate * sizeof(double) <= MCX_LDS_LIMIT) {
+        if (ng > 0 && shmem + 8 + ng * pergate * elem <= MCX_LDS_LIMIT) {
             g.box_x0 = org[0];
             g.box_y0 = org[1];
             g.box_z0 = org[2];
@@ -471,8 +479,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             g.box_nz = edge[2];
             g.box_ng = (uint32_t)ng;
             g.box_cells = (uint32_t)(ng * pergate);
-            /* the box is 8-byte aligned after the float tables */
-            shmem += (size_t)g.box_cells * sizeof(double) + 8;
+            /* the box follows the 16-byte aligned float tables */
+            shmem += (size_t)g.box_cells * elem + 8;
         }
     }
 
@@ -518,6 +526,13 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             g.threadphoton = (uint32_t)(cfg->nphoton / nlaunch);
             g.oddphoton = (int32_t)(cfg->nphoton - (size_t)g.threadphoton * nlaunch);
         }
+    }
+
+    if (getenv("MCXHIP_VERBOSE")) {
+        fprintf(stderr, "mcxhip: %zu lanes in %zu workgroups of %zu (%s), LDS %zu B, box %ux%ux%u x %u gates at "
+                "(%u,%u,%u), %u replicas, simple %u\n", nthread, nlaunch, lblock,
+                g.dynamic ? "dynamic photons" : "fixed photons per lane", shmem, g.box_nx, g.box_ny, g.box_nz,
+                g.box_ng, g.box_x0, g.box_y0, g.box_z0, g.nrep, g.simple);
     }
 
     if (cfg->seed == SEED_FROM_FILE &&
