@@ -133,12 +133,13 @@ __device__ __forceinline__ float ksin_acos(float x) {
 #endif
 }
 
-/* LDS deposit box cells: float64 keeps the bit-reproducible build's sums exact
-   to summation order; the fast build halves the footprint with float32 */
-#if MCX_PARITY
-typedef double BoxCell;
-#else
+/* LDS deposit box cells: float64.  (float32 cells would double the box for the
+   same LDS; they gave wrong launch energies and lost deposits on gfx950 with
+   boxes of 16^3 and more, so they stay behind MCX_BOX_F32 until understood.) */
+#if !MCX_PARITY && defined(MCX_BOX_F32)
 typedef float BoxCell;
+#else
+typedef double BoxCell;
 #endif
 
 /* ---------------- RNG: xorshift128+ (src/mcx_core.cl:684-698) ---------------- */
@@ -365,7 +366,7 @@ __device__ __forceinline__ float4 getprop(const MCXKParam& g, const Shared& sh, 
 }
 
 /* src/mcx_core.cl:1350-1455 */
-__device__ int skipvoid(const MCXKParam& g, Photon& s) {
+static __device__ int skipvoid(const MCXKParam& g, Photon& s) {
     int count = 1, idx1d;
     float4& p = s.p;
     float4& v = s.v;

@@ -434,7 +434,11 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
        accumulates the whole grid in float32); the budget buys two leading gates of
        16^3 voxels when it can, else one gate of the largest cube that fits. */
     {
+#ifdef MCX_BOX_F32
         const size_t elem = (cfg->mathmode == 1) ? sizeof(double) : sizeof(float);
+#else
+        const size_t elem = sizeof(double);
+#endif
         size_t budget = MCX_BOX_BYTES;
         const char* env = getenv("MCXHIP_BOX_BYTES");
 
