@@ -12,7 +12,9 @@ The JSON line also carries:
                 (algorithmic operations per photon from a frozen per-event
                 model times the oracle's event counts, divided by the kernel's
                 HIP-event duration); traffic = HBM bytes per launch read from the
-                committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present
+                committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present;
+                "atomic" = the kernel's float64 atomic rate against the ceiling
+                measured live by mcx_device_atomic_ceiling
   cpu_baseline  the CPU oracle (C restatement of the reference kernel, OpenMP)
                 timed on this host on a bounded sample of the same workload
 
@@ -21,6 +23,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--photons N]
 """
 
 import argparse
+import ctypes
 import json
 import os
 import re
@@ -73,12 +76,15 @@ def ops_per_photon(counters):
 
 
 def read_traffic():
-    """HBM bytes per transport launch from the committed rocprofv3 summary"""
+    """per-launch figures of the committed rocprofv3 counter summary (HBM bytes,
+    global atomics per photon)"""
     if not os.path.exists(TRAFFIC_SUMMARY):
         return None
+    out = {}
     with open(TRAFFIC_SUMMARY) as f:
-        m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
-    return float(m.group(1)) if m else None
+        for m in re.finditer(r"^([a-z_]+):\s*([0-9.eE+-]+)\s*$", f.read(), re.M):
+            out[m.group(1)] = float(m.group(2))
+    return out or None
 
 
 def host_threads():
@@ -167,7 +173,7 @@ def main():
             "photons": nphoton,
             "gates": 10,
             "mathmode": "fast" if args.mathmode == 0 else "parity",
-            "parallelism": "single GPU, 2560 persistent lanes per CU",
+            "parallelism": "single GPU; fast math: one resident wave of workgroups drawing photons at run time",
         },
         "transport_ms_per_step": round(float(np.mean(kernel_ms)), 3),
         "wall_ms_per_step": round(float(np.mean(wall)), 3),
@@ -187,20 +193,32 @@ def main():
         opp, depp = ops_per_photon(ref["counters"])
         transport_ms = float(np.mean(kernel_ms))
         achieved = nphoton * opp / (transport_ms * 1e-3) / 1e12
+        traffic = read_traffic()
         res["roofline"] = {
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": round(VALU_PEAK_TOPS, 3),
             "unit": "Tops/s",
             "frac": round(achieved / VALU_PEAK_TOPS, 4),
-            "traffic": read_traffic(),
-            "kernel": "transport_kernel<savedet,reflect,pencil>",
+            "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
+            "kernel": "transport_kernel<savedet,reflect,simple>",
             "kernel_ms": round(transport_ms, 3),
             "ops_per_photon": round(opp, 1),
             "frac_of_fp32_vector_peak": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "fluence_atomics_per_photon": round(depp, 2),
-            "fluence_atomic_rate_per_s": round(nphoton * depp / (transport_ms * 1e-3), 0),
+            "fluence_deposits_per_photon": round(depp, 2),
         }
+        # second ceiling: float64 atomics at scattered addresses, measured live
+        ceiling = ctypes.c_double(0.0)
+        if mcxcl_amd._native.lib.mcx_device_atomic_ceiling(0, 60 * 60 * 60 * 10, ctypes.byref(ceiling)) == 0 and \
+                traffic and traffic.get("global_atomics_per_photon"):
+            rate = nphoton * traffic["global_atomics_per_photon"] / (transport_ms * 1e-3)
+            res["roofline"]["atomic"] = {
+                "achieved": round(rate, 0),
+                "peak": round(ceiling.value, 0),
+                "unit": "atomics/s",
+                "frac": round(rate / ceiling.value, 4),
+                "atomics_per_photon": traffic["global_atomics_per_photon"],
+            }
         res["cpu_baseline"] = {
             "value": round(cpu_rate, 2),
             "unit": "photons/ms",

@@ -49,6 +49,10 @@ int mcx_device_fpmath(int gpuid, int which, int mode, const float* x, float* y, 
 /* first n floats of the device xorshift128+ stream seeded with seed4[4] */
 int mcx_device_rng_stream(int gpuid, const unsigned int seed4[4], unsigned int n, float* out);
 
+/* measured throughput (atomics/s) of float64 atomic adds at hashed addresses of
+   an ncell-cell grid on device gpuid: the ceiling the fluence deposits run against */
+int mcx_device_atomic_ceiling(int gpuid, unsigned int ncell, double* rate);
+
 #ifdef __cplusplus
 }
 #endif

@@ -194,6 +194,8 @@ lib.mcx_device_fpmath.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctyp
 lib.mcx_device_fpmath.restype = ctypes.c_int
 lib.mcx_device_rng_stream.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p]
 lib.mcx_device_rng_stream.restype = ctypes.c_int
+lib.mcx_device_atomic_ceiling.argtypes = [ctypes.c_int, ctypes.c_uint, ctypes.POINTER(ctypes.c_double)]
+lib.mcx_device_atomic_ceiling.restype = ctypes.c_int
 lib.mcx_detreclen.argtypes = [_P(Config)]
 lib.mcx_detreclen.restype = ctypes.c_uint
 lib.mcx_partialdata.argtypes = [_P(Config)]

@@ -64,6 +64,7 @@ hipError_t mcx_launch_fast(const MCXKParam* prm, int nblock, int nthreadblock, s
 /* RNG stream dump: field[i] for i = idx; i < fieldlen; i += nthread */
 /* resident workgroups per CU of the fast transport kernel variant for this launch shape */
 int mcx_fast_occupancy(int nthreadblock, size_t shmem, int savedet, int reflect, int simple);
+hipError_t mcx_launch_atomic_probe(double* grid, uint32_t n, uint32_t nblock, uint32_t k, hipStream_t s);
 hipError_t mcx_launch_rngdump(const MCXKParam* prm, int nblock, int nthreadblock, hipStream_t stream);
 /* sum the nrep replicas into in[0..n), then out[i] = (float)in[i] * scale */
 hipError_t mcx_launch_finalize(double* in, float* out, size_t n, unsigned int nrep, float scale, hipStream_t stream);

@@ -146,6 +146,28 @@ __global__ void rngstream_kernel(const uint32_t* seed4, unsigned int n, float* o
     }
 }
 
+/* throughput ceiling of float64 atomic adds at scattered addresses: every lane
+   adds 1.0 to k hashed cells of an n-cell grid (the fluence grid's size) */
+__global__ void atomic_probe_kernel(double* __restrict__ grid, uint32_t n, uint32_t k) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+
+    for (uint32_t i = 0; i < k; i++) {
+        uint32_t h = t * 2654435761u + i * 40503u;
+        h ^= h >> 16;
+        h *= 0x7FEB352Du;
+        h ^= h >> 15;
+        h *= 0x846CA68Bu;
+        h ^= h >> 16;
+        atomicAdd(grid + (h % n), 1.0);
+    }
+}
+
+extern "C" hipError_t mcx_launch_atomic_probe(double* grid, uint32_t n, uint32_t nblock, uint32_t k,
+        hipStream_t s) {
+    hipLaunchKernelGGL(atomic_probe_kernel, dim3(nblock), dim3(256), 0, s, grid, n, k);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t mcx_launch_rngstream(const uint32_t* seed4, unsigned int n, float* out, hipStream_t s) {
     hipLaunchKernelGGL(rngstream_kernel, dim3(1), dim3(64), 0, s, seed4, n, out);
     hipError_t e = hipGetLastError();

@@ -952,3 +952,75 @@ extern "C" int mcx_device_rng_stream(int gpuid, const unsigned int seed4[4], uns
 
     return (int)e;
 }
+
+extern "C" int mcx_device_atomic_ceiling(int gpuid, unsigned int ncell, double* rate) {
+    double* grid = NULL;
+    hipEvent_t e0 = NULL, e1 = NULL;
+    const uint32_t nblock = 2560, k = 64;
+    float ms = 0.f, best = 1e30f;
+    hipError_t e = hipSetDevice(gpuid);
+
+    *rate = 0.0;
+
+    if (ncell == 0) {
+        return (int)hipErrorInvalidValue;
+    }
+
+    if (e == hipSuccess) {
+        e = hipMalloc((void**)&grid, (size_t)ncell * sizeof(double));
+    }
+
+    if (e == hipSuccess) {
+        e = hipEventCreate(&e0);
+    }
+
+    if (e == hipSuccess) {
+        e = hipEventCreate(&e1);
+    }
+
+    for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+        e = hipMemset(grid, 0, (size_t)ncell * sizeof(double));
+
+        if (e == hipSuccess) {
+            e = hipEventRecord(e0, 0);
+        }
+
+        if (e == hipSuccess) {
+            e = mcx_launch_atomic_probe(grid, ncell, nblock, k, 0);
+        }
+
+        if (e == hipSuccess) {
+            e = hipEventRecord(e1, 0);
+        }
+
+        if (e == hipSuccess) {
+            e = hipEventSynchronize(e1);
+        }
+
+        if (e == hipSuccess) {
+            e = hipEventElapsedTime(&ms, e0, e1);
+        }
+
+        if (e == hipSuccess && ms < best) {
+            best = ms;
+        }
+    }
+
+    if (e == hipSuccess && best > 0.f) {
+        *rate = (double)nblock * 256.0 * k / (best * 1e-3);
+    }
+
+    if (e0) {
+        (void)hipEventDestroy(e0);
+    }
+
+    if (e1) {
+        (void)hipEventDestroy(e1);
+    }
+
+    if (grid) {
+        (void)hipFree(grid);
+    }
+
+    return (int)e;
+}
