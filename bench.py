@@ -108,6 +108,64 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": n * 4}))
 
 
+def adjoint_leg(photons, nfd, seed=0):
+    """Gradient of the 8 detector signals of the 200^3 four-layer workload with
+    respect to per-voxel mua, by replay (mcxcl_amd.autograd), against central
+    finite differences at nfd voxels.  Runs with the fixed photon-per-lane map
+    so that every forward run launches the same photons (exact common random
+    numbers: absorption does not move photons)."""
+    import torch
+    from mcxcl_amd import autograd, workloads
+
+    os.environ["MCXHIP_DYNAMIC"] = "0"
+    try:
+        cfg = workloads.layered_workload(nphoton=int(photons), mathmode=0)
+        vol = cfg["vol"]
+        prop = np.asarray(cfg["prop"], np.float32)
+        mua0 = prop[vol, 0].astype(np.float32)
+        mus0 = prop[vol, 1].astype(np.float32)
+        autograd.detector_signals(torch.tensor(mua0), torch.tensor(mus0), cfg)   # warm-up
+        t0 = time.perf_counter()
+        mua = torch.tensor(mua0, requires_grad=True)
+        S = autograd.detector_signals(mua, torch.tensor(mus0), cfg)
+        t1 = time.perf_counter()
+        S.sum().backward()
+        t2 = time.perf_counter()
+        g = mua.grad.numpy()
+        order = np.argsort(np.abs(g).ravel())[::-1]
+        rng = np.random.default_rng(seed)
+        picks = list(order[:nfd // 2]) + list(rng.choice(order[:20000], nfd - nfd // 2, replace=False))
+        errs, tf = [], []
+        for flat in picks:
+            v = np.unravel_index(flat, g.shape)
+            h = 0.1 * mua0[v]
+            sig = []
+            for s in (1, -1):
+                m = mua0.copy()
+                m[v] += s * h
+                ta = time.perf_counter()
+                sig.append(float(autograd.detector_signals(torch.tensor(m), torch.tensor(mus0), cfg).detach().sum()))
+                tf.append(time.perf_counter() - ta)
+            fd = (sig[0] - sig[1]) / (2 * h)
+            errs.append(abs(fd - g[v]) / abs(g[v]))
+        t_fwd = float(np.median(tf))
+        t_adj = t2 - t0
+        return {
+            "workload": "200^3 4-layer volume, %d photons, 8 detectors, gradient of sum of detector signals "
+                        "w.r.t. per-voxel mua" % int(photons),
+            "forward_s": round(t1 - t0, 4),
+            "backward_s": round(t2 - t1, 4),
+            "gradient_per_s": round(1.0 / t_adj, 4),
+            "fd_64_voxels_s": round(128 * t_fwd, 3),
+            "speedup_vs_fd_64_voxels": round(128 * t_fwd / t_adj, 1),
+            "voxels_checked": nfd,
+            "max_rel_err_vs_fd": round(float(max(errs)), 5),
+            "median_rel_err_vs_fd": round(float(np.median(errs)), 6),
+        }
+    finally:
+        os.environ.pop("MCXHIP_DYNAMIC", None)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +176,9 @@ def main():
     ap.add_argument("--cpu-photons", type=float, default=3e6, help="photons in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="run one plain device copy of this many bytes and exit")
+    ap.add_argument("--adjoint-photons", type=float, default=1e7,
+                    help="photons of the replay-adjoint gradient leg (0 skips it)")
+    ap.add_argument("--adjoint-fd-voxels", type=int, default=16, help="voxels checked by central differences")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -228,6 +289,8 @@ def main():
                       (cpu_n, ref["seconds"]),
             "absorbed_fraction": round(ref["stat"]["energyabs"] / ref["stat"]["energytot"], 6),
         }
+    if args.adjoint_photons > 0:
+        res["adjoint"] = adjoint_leg(args.adjoint_photons, args.adjoint_fd_voxels)
     print(json.dumps(res), flush=True)
 
 

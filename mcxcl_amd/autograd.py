@@ -8,8 +8,11 @@ S_d = sum of the exit weights of the photons detected by detector d.
 The backward pass replays the detected photons from their saved RNG states, so
 no trajectory is stored: each replayed photon retraces its path and deposits
 
-  J_d(v)  = sum_{i in d} w_i L_iv        (outputtype 'jacobian', replaydet -1)
+  J_d(v)  = sum_{i in d} w_i L_iv        (outputtype 'jacobian')
   WP_d(v) = sum_{i in d} w_i n_iv        (outputtype 'wp': one deposit per scattering)
+
+The upstream gradient g_d is folded into the replay weights (w_i g_d(i)), so one
+replay of all detected photons deposits sum_d g_d J_d at once.
 
 with L_iv the photon's path length in voxel v and n_iv its scattering count
 there.  Absorption is continuous weight, w_i = exp(-sum_v mua_v L_iv), so
@@ -74,18 +77,16 @@ class DetectorSignals(torch.autograd.Function):
             return z, z.clone(), None
         run_cfg, seeds, detp, detw = ctx.replay
         g = grad_s.detach().to("cpu", torch.float64).numpy()
-        rep = dict(run_cfg, seed=seeds, detphotons=detp, replayweight=detw, replaydet=-1, rawfield=1,
+        # fold the upstream gradient into the replay weights: one replay of all
+        # detectors then deposits sum_d g_d J_d directly (replaydet 0)
+        detid = detp[0].astype(np.int64) & 0xFFFF
+        wg = (detw.astype(np.float64) * g[detid - 1]).astype(np.float32)
+        rep = dict(run_cfg, seed=seeds, detphotons=detp, replayweight=wg, replaydet=0, rawfield=1,
                    issaveseed=0, savedetweight=0)
         unit = float(run_cfg.get("unitinmm", 1.0))
-        jac = _run(dict(rep, outputtype="jacobian"))["flux64"]
-        wp = _run(dict(rep, outputtype="wp"))["flux64"]
-        # replaydet -1 stacks one block of gates per detector: [Nx, Ny, Nz, gate, detector]
-        ngate = jac.shape[-1] // ctx.ndet
-        J = jac.reshape(shape + (ngate, ctx.ndet), order="F").sum(-2)
-        W = wp.reshape(shape + (ngate, ctx.ndet), order="F").sum(-2)
+        gJ = _run(dict(rep, outputtype="jacobian"))["flux64"].sum(-1).reshape(shape) * unit
+        gW = _run(dict(rep, outputtype="wp"))["flux64"].sum(-1).reshape(shape)
         # J holds path lengths in grid units; per-mm derivatives carry unitinmm
-        gJ = (J * g).sum(-1) * unit
-        gW = (W * g).sum(-1)
         mus = ctx.mus.astype(np.float64)
         with np.errstate(divide="ignore", invalid="ignore"):
             gmus = np.where(mus > 0, gW / mus, 0.0) - gJ

@@ -91,3 +91,62 @@ def sphere_cube_workload(nphoton=100000000, seed=1648335518, **over):
     })
     cfg.update(over)
     return cfg
+
+
+LAYER_PROP = [[0, 0, 1, 1], [0.019, 7.8, 0.89, 1.37], [0.019, 7.8, 0.89, 1.37], [0.0004, 0.009, 0.89, 1.37],
+              [0.02, 9.0, 0.89, 1.37]]
+
+
+def _smooth_surface(rng, n, amplitude, sigma=12.0):
+    """a seeded random surface: white noise low-passed by a Gaussian of width
+    sigma (voxels) in Fourier space, scaled to +-amplitude"""
+    noise = rng.standard_normal((n, n))
+    k = np.fft.fftfreq(n)
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    filt = np.exp(-2.0 * (np.pi * sigma) ** 2 * (kx * kx + ky * ky))
+    s = np.real(np.fft.ifft2(np.fft.fft2(noise) * filt))
+    s -= s.mean()
+    return s * (amplitude / max(np.abs(s).max(), 1e-12))
+
+
+def layered_volume(dim=200, depths=(20, 40, 70), amplitude=8.0, seed=20251202):
+    """dim^3 uint8 volume of 4 layers (labels 1-4 from the top, z = 0) whose
+    interfaces lie at the given mean depths (voxels) displaced by seeded smooth
+    random surfaces of +-amplitude voxels"""
+    rng = np.random.default_rng(seed)
+    z = np.arange(dim, dtype=np.float64)[None, None, :] + 0.5
+    vol = np.ones((dim, dim, dim), dtype=np.uint8)
+    for lab, d in enumerate(depths, start=2):
+        surf = d + _smooth_surface(rng, dim, amplitude)
+        vol[z >= surf[:, :, None]] = lab
+    return vol
+
+
+def layered_workload(nphoton=100000000, dim=200, seed=1648335518, **over):
+    """200^3 4-layer head-like volume (interfaces from seeded Gaussian-filtered
+    noise, 8-voxel amplitude), layer mua/mus/g/n (0.019,7.8,0.89,1.37),
+    (0.019,7.8,0.89,1.37), (0.0004,0.009,0.89,1.37), (0.02,9.0,0.89,1.37) per mm,
+    one pencil source at the centre of the top face and 8 detectors of radius
+    2 mm on a 20-voxel ring around it, 0-5 ns in one gate, partial path lengths
+    saved"""
+    c = dim / 2.0
+    dets = [[c + 20 * np.cos(a), c + 20 * np.sin(a), 0, 2.0] for a in np.arange(8) * (np.pi / 4)]
+    cfg = {
+        "nphoton": int(nphoton),
+        "vol": layered_volume(dim),
+        "prop": LAYER_PROP,
+        "tstart": 0,
+        "tend": 5e-9,
+        "tstep": 5e-9,
+        "srcpos": [c, c, 0],
+        "srcdir": [0, 0, 1],
+        "isreflect": 1,
+        "seed": seed,
+        "issrcfrom0": 1,
+        "detpos": dets,
+        "issavedet": 1,
+        "savedetflag": "DP",
+        "maxdetphoton": 5000000,
+    }
+    cfg.update(over)
+    return cfg
