@@ -350,7 +350,16 @@ int  mcx_lookupindex(char* key, const char* index);                         /* s
 int  mcx_keylookup(char* key, const char* table[]);                         /* src/mcx_utils.h:314 */
 int  mcx_parsedebugopt(char* debugopt, const char* debugflag);              /* src/mcx_utils.h:318 */
 void mcx_savedata(float* dat, size_t len, Config* cfg);                     /* src/mcx_utils.h:286 */
-void mcx_savedetphoton(float* ppath, void* seeds, int count, int seedbyte, Config* cfg); /* src/mcx_utils.h:312 */
+void mcx_savedetphoton(float* ppath, void* seeds, int count, int doappend, Config* cfg); /* src/mcx_utils.h:312 */
+void mcx_savenii(float* dat, size_t len, char* name, int type32bit, int outputformatid, Config* cfg); /* src/mcx_utils.h:287 */
+void mcx_savejnii(float* vol, int ndim, unsigned int* dims, float* voxelsize, char* name, int isfloat, int iscol,
+                  Config* cfg);                                             /* src/mcx_utils.h:330 */
+void mcx_savebnii(float* vol, int ndim, unsigned int* dims, float* voxelsize, char* name, int isfloat, int iscol,
+                  Config* cfg);                                             /* src/mcx_utils.h:331 */
+void mcx_savejdet(float* ppath, void* seeds, unsigned int count, int doappend, Config* cfg); /* src/mcx_utils.h:332 */
+void mcx_replayprep(int* detid, float* ppath, History* his, Config* cfg);   /* src/mcx_utils.h:333 */
+void mcx_loadseedfile(Config* cfg);                                         /* src/mcx_utils.h:325 */
+void mcx_loadseedjdat(char* filename, Config* cfg);                         /* src/mcx_utils.c:3707 (not in the reference header) */
 /* replaces int mcx_parse_shapestring(Grid3D* g, char* shapedata) (src/mcx_shapes.h:56):
    rasterises the shapes into cfg->vol/cfg->dim directly */
 int  mcx_parse_shapestring(Config* cfg, const char* shapedata);
@@ -368,6 +377,9 @@ int mcx_try_loadjson_string(const char* json, Config* cfg);
 int mcx_try_loadbenchmark(const char* key, Config* cfg);
 int mcx_try_readconfig(const char* fname, Config* cfg);
 int mcx_try_parse_shapestring(Config* cfg, const char* shapedata);
+int mcx_try_parsecmd(int argc, char* argv[], Config* cfg);
+int mcx_try_savedata(float* dat, size_t len, Config* cfg);
+int mcx_try_savedetphoton(float* ppath, void* seeds, int count, int doappend, Config* cfg);
 const char* mcx_last_error(void);
 
 /* ABI self-description, used by bindings to verify their Config mirror */

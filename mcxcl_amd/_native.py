@@ -181,6 +181,12 @@ lib.mcx_try_readconfig.argtypes = [ctypes.c_char_p, _P(Config)]
 lib.mcx_try_readconfig.restype = ctypes.c_int
 lib.mcx_try_parse_shapestring.argtypes = [_P(Config), ctypes.c_char_p]
 lib.mcx_try_parse_shapestring.restype = ctypes.c_int
+lib.mcx_try_parsecmd.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _P(Config)]
+lib.mcx_try_parsecmd.restype = ctypes.c_int
+lib.mcx_try_savedata.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _P(Config)]
+lib.mcx_try_savedata.restype = ctypes.c_int
+lib.mcx_try_savedetphoton.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P(Config)]
+lib.mcx_try_savedetphoton.restype = ctypes.c_int
 lib.mcx_try_run_simulation.argtypes = [_P(Config)]
 lib.mcx_try_run_simulation.restype = ctypes.c_int
 lib.mcx_last_error.restype = ctypes.c_char_p

@@ -831,6 +831,23 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         }
     }
 
+    /* a stand-alone caller gets its output files here (src/mcx_host.cpp:1646-1664);
+       the MATLAB/Python bindings take the buffers instead */
+    if (cfg->parentid == mpStandalone) {
+        if (cfg->issave2pt && cfg->exportfield) {
+            mcx_savedata(cfg->exportfield, fieldlen, cfg);
+        }
+
+        if (cfg->issavedet && cfg->exportdetected) {
+            cfg->his.unitinmm = cfg->unitinmm;
+            cfg->his.savedphoton = cfg->detectedcount;
+            cfg->his.totalphoton = (unsigned int)cfg->nphoton;
+            cfg->his.seedbyte = (cfg->issaveseed && cfg->seeddata) ? 16 : 0;
+            cfg->his.detected = cfg->detectedcount;
+            mcx_savedetphoton(cfg->exportdetected, cfg->seeddata, (int)cfg->detectedcount, 0, cfg);
+        }
+    }
+
     free(hseed);
     free(hgenergy);
     free(hfield);

@@ -17,6 +17,7 @@
 #include <string.h>
 #include <time.h>
 
+#include "mcx_jdata.h"
 #include "mcx_json.h"
 #include "mcx_shapes.h"
 #include "mcx_utils.h"
@@ -90,6 +91,7 @@ void mcx_flush(Config* cfg) {
 /* ---------------- defaults (reference src/mcx_utils.c:203-366) ---------------- */
 void mcx_initcfg(Config* cfg) {
     memset(cfg, 0, sizeof(Config));
+    memset(flagset, 0, sizeof(flagset));   /* command-line overrides belong to one configuration */
     cfg->tstart = 0.f;
     cfg->tend = 5e-9f;
     cfg->tstep = 5e-9f;
@@ -847,7 +849,46 @@ static int mcx_loadjson(MJNode* root, Config* cfg) {
     }
 
     if (filename[0] == '\0') {
-        if (Shapes) {
+        if (Shapes && mj_get(Shapes, "_ArraySize_")) {
+            /* a label volume embedded as a JData array (src/mcx_utils.c:3239-3250) */
+            unsigned int dims[4] = {1, 1, 1, 1};
+            int ndim = 0;
+            char type[32];
+            size_t nelem = 0;
+            const char* err = NULL;
+            void* buf = mcx_jdata_decode(Shapes, dims, 4, &ndim, type, sizeof(type), &nelem, &err);
+
+            if (!buf) {
+                MCX_ERROR(-1, err);
+            }
+
+            int eb = mcx_jdata_typebytes(type);
+
+            if (strstr(type, "double") || strstr(type, "int64") || !strcmp(type, "single") || !strcmp(type, "float32")) {
+                free(buf);
+                MCX_ERROR(-1, "8-byte and floating-point JData volumes are not supported");
+            }
+
+            cfg->dim.x = dims[0];
+            cfg->dim.y = dims[1];
+            cfg->dim.z = dims[2];
+            cfg->mediabyte = (unsigned int)eb;
+            free(cfg->vol);
+            cfg->vol = (unsigned int*)malloc((nelem ? nelem : 1) * sizeof(unsigned int));
+
+            for (size_t i = 0; i < nelem; i++) {
+                cfg->vol[i] = (eb == 1) ? ((unsigned char*)buf)[i] : (eb == 2) ? ((unsigned short*)buf)[i] :
+                              ((unsigned int*)buf)[i];
+
+                if (cfg->vol[i] >= cfg->medianum) {
+                    free(buf);
+                    MCX_ERROR(-6, "medium index exceeds the specified medium types");
+                }
+            }
+
+            free(buf);
+            cfg->isrowmajor = 1;   /* JData arrays are row-major */
+        } else if (Shapes) {
             char err[256] = {0};
             int status = mcx_rasterize_shapes(root, &cfg->vol, &cfg->dim, cfg->issrcfrom0, cfg->isrowmajor, err);
 
@@ -1156,6 +1197,14 @@ void mcx_prepdomain(char* filename, Config* cfg) {
         }
     } else if (!cfg->vol) {
         MCX_ERROR(-4, "one must specify a binary volume file in order to run the simulation");
+    }
+
+    if (cfg->seed == SEED_FROM_FILE && cfg->seedfile[0]) {
+        if (strstr(cfg->seedfile, ".jdat")) {
+            mcx_loadseedjdat(cfg->seedfile, cfg);
+        } else {
+            mcx_loadseedfile(cfg);
+        }
     }
 
     mcx_preprocess(cfg);
@@ -1586,6 +1635,233 @@ void mcx_replayinit(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
     cfg->nphoton = kept;
 }
 
+/* ---------------- replay from files (reference src/mcx_utils.c:1432-1466, :3707-3897) ---------------- */
+/* keep the photons of cfg->replaydet (all when <= 0) whose time of flight lies in
+   [tstart, tend], compacting the seeds; ppath holds his->maxmedia path lengths
+   per photon in grid units and the media are still in 1/mm here */
+void mcx_replayprep(int* detid, float* ppath, History* his, Config* cfg) {
+    size_t kept = 0;
+
+    for (size_t i = 0; i < his->savedphoton; i++) {
+        if (cfg->replaydet > 0 && !(detid && cfg->replaydet == (detid[i] & 0xFFFF))) {
+            continue;
+        }
+
+        if (i != kept) {
+            memmove((char*)cfg->replay.seed + kept * his->seedbyte, (char*)cfg->replay.seed + i * his->seedbyte,
+                    his->seedbyte);
+        }
+
+        float w = 1.f, tof = 0.f;
+
+        for (unsigned int j = 0; j < his->maxmedia; j++) {
+            float plen = ppath[i * his->maxmedia + j] * his->unitinmm;
+            w *= expf(-cfg->prop[j + 1].mua * plen);
+            tof += plen * R_C0 * cfg->prop[j + 1].n;
+        }
+
+        cfg->replay.weight[kept] = w;
+        cfg->replay.tof[kept] = tof;
+        cfg->replay.detid[kept] = detid ? detid[i] : 1;
+
+        if (tof < cfg->tstart || tof > cfg->tend) {
+            continue;
+        }
+
+        kept++;
+    }
+
+    cfg->nphoton = kept;
+    cfg->minenergy = 0.f;
+}
+
+static void replay_alloc(Config* cfg, size_t n) {
+    free(cfg->replay.weight);
+    free(cfg->replay.tof);
+    free(cfg->replay.detid);
+    cfg->replay.weight = (float*)malloc((n ? n : 1) * sizeof(float));
+    cfg->replay.tof = (float*)calloc(n ? n : 1, sizeof(float));
+    cfg->replay.detid = (int*)calloc(n ? n : 1, sizeof(int));
+
+    if (!cfg->replay.weight || !cfg->replay.tof || !cfg->replay.detid) {
+        MCX_ERROR(-7, "can not allocate memory");
+    }
+}
+
+/* reference mcx_loadseedfile (src/mcx_utils.c:3806-3897): seeds of an .mch file
+   from a run with -q 1; replay outputs (j, p, r) also take the weights, times of
+   flight and detector ids from its D and P columns */
+void mcx_loadseedfile(Config* cfg) {
+    History his;
+    FILE* fp = fopen(cfg->seedfile, "rb");
+
+    if (!fp) {
+        MCX_ERROR(-7, "can not open the specified history file");
+    }
+
+    if (fread(&his, sizeof(History), 1, fp) != 1) {
+        fclose(fp);
+        MCX_ERROR(-7, "error when reading the history file");
+    }
+
+    if (his.savedphoton == 0 || his.seedbyte == 0) {
+        fclose(fp);
+        MCX_ERROR(-7, "history file does not contain seed data, please re-run your simulation with '-q 1'");
+    }
+
+    if (his.maxmedia != cfg->medianum - 1) {
+        fclose(fp);
+        MCX_ERROR(-7, "the history file was generated with a different media setting");
+    }
+
+    if (his.seedbyte != 16) {
+        fclose(fp);
+        MCX_ERROR(-7, "the history file holds seeds of another random number generator");
+    }
+
+    size_t nrec = his.savedphoton;
+    size_t nval = nrec * his.colcount;
+    float* rec = (float*)malloc((nval > 0 ? nval : 1) * sizeof(float));
+    free(cfg->replay.seed);
+    cfg->replay.seed = malloc(nrec * his.seedbyte);
+
+    if (!rec || !cfg->replay.seed) {
+        free(rec);
+        fclose(fp);
+        MCX_ERROR(-7, "can not allocate memory");
+    }
+
+    if (fread(rec, sizeof(float) * his.colcount, nrec, fp) != nrec ||
+            fread(cfg->replay.seed, his.seedbyte, nrec, fp) != nrec) {
+        free(rec);
+        fclose(fp);
+        MCX_ERROR(-7, "error when reading the seed data");
+    }
+
+    fclose(fp);
+    cfg->seed = SEED_FROM_FILE;
+    cfg->nphoton = nrec;
+    replay_alloc(cfg, nrec);
+
+    int hasdetid = SAVE_DETID(his.savedetflag);
+    int isreplayout = (cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS);
+
+    if (!isreplayout) {
+        /* a plain replay relaunches every saved photon with its own weight */
+        for (size_t i = 0; i < nrec; i++) {
+            cfg->replay.weight[i] = 1.f;
+            cfg->replay.detid[i] = hasdetid ? (int)rec[i * his.colcount] : 1;
+        }
+
+        free(rec);
+        return;
+    }
+
+    if ((!hasdetid && cfg->detnum > 1) || !SAVE_PPATH(his.savedetflag)) {
+        free(rec);
+        MCX_ERROR(-7, "please rerun the baseline simulation and save detector ID (D) and partial-path (P) using '-w DP'");
+    }
+
+    /* gather the detector ids and the partial paths into mcx_replayprep's layout */
+    unsigned int off = hasdetid + SAVE_NSCAT(his.savedetflag) * his.maxmedia;
+    int* detid = (int*)malloc(nrec * sizeof(int));
+    size_t npath = nrec * his.maxmedia;
+    float* pp = (float*)malloc((npath > 0 ? npath : 1) * sizeof(float));
+
+    if (!detid || !pp) {
+        free(detid);
+        free(pp);
+        free(rec);
+        MCX_ERROR(-7, "can not allocate memory");
+    }
+
+    for (size_t i = 0; i < nrec; i++) {
+        detid[i] = hasdetid ? (int)rec[i * his.colcount] : 1;
+        memcpy(pp + i * his.maxmedia, rec + i * his.colcount + off, his.maxmedia * sizeof(float));
+    }
+
+    mcx_replayprep(detid, pp, &his, cfg);
+    free(detid);
+    free(pp);
+    free(rec);
+}
+
+/* reference mcx_loadseedjdat (src/mcx_utils.c:3707-3800): MCXData.PhotonData
+   .seed, .detid and .ppath of a _detp.jdat file */
+void mcx_loadseedjdat(char* filename, Config* cfg) {
+    FILE* fp = fopen(filename, "rb");
+
+    if (!fp) {
+        MCX_ERROR(-6, "fail to open the specified seed jdata file");
+    }
+
+    fseek(fp, 0, SEEK_END);
+    long len = ftell(fp);
+    rewind(fp);
+    char* text = (char*)malloc((size_t)len + 1);
+
+    if (!text || fread(text, 1, (size_t)len, fp) != (size_t)len) {
+        free(text);
+        fclose(fp);
+        MCX_ERROR(-2, "reading input file is terminated");
+    }
+
+    fclose(fp);
+    text[len] = '\0';
+    char perr[256] = {0};
+    MJNode* root = mj_parse(text, perr, sizeof(perr));
+    free(text);
+
+    if (!root) {
+        MCX_ERROR(-1, "invalid jdat file");
+    }
+
+    MJNode* data = mj_get(root, "MCXData");
+    MJNode* info = data ? mj_get(data, "Info") : NULL;
+    MJNode* photon = data ? mj_get(data, "PhotonData") : NULL;
+    MJNode* jseed = photon ? mj_get(photon, "seed") : NULL;
+    MJNode* jdet = photon ? mj_get(photon, "detid") : NULL;
+    MJNode* jppath = photon ? mj_get(photon, "ppath") : NULL;
+
+    if (!jseed || !jppath || !jdet || !info) {
+        mj_free(root);
+        MCX_ERROR(-1, "invalid jdat file, expect MCXData.PhotonData.seed, .detid and .ppath");
+    }
+
+    unsigned int sdims[2] = {0, 0}, pdims[2] = {0, 0}, ddims[2] = {0, 0};
+    int ndim = 0;
+    char type[32];
+    size_t ns = 0, np = 0, nd = 0;
+    const char* err = NULL;
+    History his;
+    memset(&his, 0, sizeof(his));
+    unsigned char* seeds = (unsigned char*)mcx_jdata_decode(jseed, sdims, 2, &ndim, type, sizeof(type), &ns, &err);
+    float* pp = seeds ? (float*)mcx_jdata_decode(jppath, pdims, 2, &ndim, type, sizeof(type), &np, &err) : NULL;
+    int* detid = pp ? (int*)mcx_jdata_decode(jdet, ddims, 2, &ndim, type, sizeof(type), &nd, &err) : NULL;
+    his.unitinmm = (float)mj_num(mj_get(info, "LengthUnit"), 1.0);
+    mj_free(root);
+
+    if (!detid || sdims[1] != 16 || pdims[0] != sdims[0] || ddims[0] != sdims[0] ||
+            pdims[1] != cfg->medianum - 1) {
+        free(seeds);
+        free(pp);
+        free(detid);
+        MCX_ERROR(-1, err ? err : "the jdat seed, detid and ppath arrays do not match this simulation");
+    }
+
+    his.savedphoton = sdims[0];
+    his.seedbyte = sdims[1];
+    his.maxmedia = pdims[1];
+    free(cfg->replay.seed);
+    cfg->replay.seed = seeds;
+    cfg->seed = SEED_FROM_FILE;
+    cfg->nphoton = his.savedphoton;
+    replay_alloc(cfg, his.savedphoton);
+    mcx_replayprep(detid, pp, &his, cfg);
+    free(pp);
+    free(detid);
+}
+
 /* ---------------- validation (reference src/mcx_utils.c:1822-1964) ---------------- */
 void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
     if (!cfg->issrcfrom0) {
@@ -1660,50 +1936,7 @@ void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
     mcx_replayinit(cfg, detps, dimdetps, seedbyte);
 }
 
-/* ---------------- writers (reference src/mcx_utils.c:886-998) ---------------- */
-void mcx_savedata(float* dat, size_t len, Config* cfg) {
-    char name[MAX_FULL_PATH];
-    snprintf(name, sizeof(name), "%s%s%s.mc2", cfg->rootpath, cfg->rootpath[0] ? "/" : "", cfg->session);
-    FILE* fp = fopen(name, "wb");
-
-    if (!fp) {
-        MCX_ERROR(-2, "can not save data to disk");
-    }
-
-    if (fwrite(dat, sizeof(float), len, fp) != len) {
-        fclose(fp);
-        MCX_ERROR(-2, "incomplete write of the fluence file");
-    }
-
-    fclose(fp);
-}
-
-void mcx_savedetphoton(float* ppath, void* seeds, int count, int seedbyte, Config* cfg) {
-    char name[MAX_FULL_PATH];
-    snprintf(name, sizeof(name), "%s%s%s.mch", cfg->rootpath, cfg->rootpath[0] ? "/" : "", cfg->session);
-    FILE* fp = fopen(name, "wb");
-
-    if (!fp) {
-        MCX_ERROR(-2, "can not save data to disk");
-    }
-
-    History his = cfg->his;
-    his.unitinmm = cfg->unitinmm;
-    his.savedphoton = (unsigned int)count;
-    his.detected = (unsigned int)count;
-    his.totalphoton = (unsigned int)cfg->nphoton;
-    his.colcount = mcx_detreclen(cfg);
-    his.seedbyte = seeds ? 16 : 0;
-    fwrite(&his, sizeof(History), 1, fp);
-    fwrite(ppath, sizeof(float), (size_t)count * his.colcount, fp);
-
-    if (seeds && his.seedbyte) {
-        fwrite(seeds, his.seedbyte, (size_t)count, fp);
-    }
-
-    (void)seedbyte;
-    fclose(fp);
-}
+/* writers: mcx_output.c */
 
 /* ---------------- presets (reference src/mcx_bench.h, src/mcx_utils.c:5195-5203) ---------------- */
 extern const char* mcx_preset_json(const char* name);
@@ -1725,7 +1958,7 @@ void mcx_usage(Config* cfg, char* exename) {
            "  -f config     read a .json or .inp input file\n"
            "  --bench name  run a built-in benchmark (cube60, cube60b, cube60planar, cubesph60b, onecube, twocube)\n"
            "  -n photons    number of photons\n"
-           "  -E seed       RNG seed\n"
+           "  -E seed       RNG seed, or a .mch/.jdat file of a -q 1 run to replay its photons\n"
            "  -b 0|1        Fresnel reflection at refractive-index mismatches\n"
            "  -B bcflags    per-face boundary condition letters [_ramc] and detection flags [01]\n"
            "  -s session    output name\n"
@@ -1736,6 +1969,16 @@ void mcx_usage(Config* cfg, char* exename) {
            "  -w flags      detected photon fields [DSPMXVW]\n"
            "  -t N -T N     thread count and block size (disables autopilot)\n"
            "  -S 0|1        save fluence\n"
+           "  -F format     output format: mc2, nii, hdr, ubj, tx3, jnii (default), bnii\n"
+           "  -Z codec      JData compression: zlib (default), gzip, base64\n"
+           "  --root path   output folder\n"
+           "  -P N          replay the photons of detector N (0: all, -1: one output per detector)\n"
+           "  -u mm         voxel edge length in mm\n"
+           "  -r N          repeat the simulation N times\n"
+           "  -z 0|1        source position counted from 0\n"
+           "  -X 0|1 -x 0|1 save diffuse reflectance / exit position and direction\n"
+           "  -V 0|1 -m 0|1 specular reflection at entry / save momentum transfer\n"
+           "  --srcid N     simulate source N only (0: all together, -1: each separately)\n"
            "  --mathmode 0|1  0: hardware transcendentals, 1: bit-reproducible math\n",
            exename);
 }
@@ -1765,7 +2008,15 @@ void mcx_parsecmd(int argc, char* argv[], Config* cfg) {
             flagset['n'] = 1;
         } else if (!strcmp(a, "-E") || !strcmp(a, "--seed")) {
             NEED_ARG();
-            cfg->seed = atoi(next);
+
+            if (strstr(next, ".mch") || strstr(next, ".jdat")) {
+                /* replay the photons saved in a detected-photon file */
+                snprintf(cfg->seedfile, MAX_PATH_LENGTH, "%s", next);
+                cfg->seed = SEED_FROM_FILE;
+            } else {
+                cfg->seed = atoi(next);
+            }
+
             flagset['E'] = 1;
         } else if (!strcmp(a, "-b") || !strcmp(a, "--reflect")) {
             NEED_ARG();
@@ -1832,6 +2083,58 @@ void mcx_parsecmd(int argc, char* argv[], Config* cfg) {
             NEED_ARG();
             cfg->minenergy = (float)atof(next);
             flagset['e'] = 1;
+        } else if (!strcmp(a, "-F") || !strcmp(a, "--outputformat")) {
+            NEED_ARG();
+            int id = mcx_keylookup((char*)next, outputformat_names);
+
+            if (id < 0) {
+                MCX_ERROR(-2, "the specified output data format is not recognized");
+            }
+
+            cfg->outputformat = (char)id;
+            flagset['F'] = 1;
+        } else if (!strcmp(a, "-Z") || !strcmp(a, "--zip")) {
+            NEED_ARG();
+            cfg->zipid = isdigit((unsigned char)next[0]) ? atoi(next) : mcx_zipid(next);
+
+            if (cfg->zipid < 0 || cfg->zipid > mzLz4hc) {
+                MCX_ERROR(-2, "the specified compression method is not recognized");
+            }
+        } else if (!strcmp(a, "--root")) {
+            NEED_ARG();
+            snprintf(cfg->rootpath, MAX_PATH_LENGTH, "%s", next);
+        } else if (!strcmp(a, "-P") || !strcmp(a, "--replaydet")) {
+            NEED_ARG();
+            cfg->replaydet = atoi(next);
+        } else if (!strcmp(a, "-u") || !strcmp(a, "--unitinmm")) {
+            NEED_ARG();
+            cfg->unitinmm = (float)atof(next);
+            flagset['u'] = 1;
+        } else if (!strcmp(a, "-r") || !strcmp(a, "--repeat")) {
+            NEED_ARG();
+            cfg->respin = atoi(next);
+        } else if (!strcmp(a, "-z") || !strcmp(a, "--srcfrom0")) {
+            NEED_ARG();
+            cfg->issrcfrom0 = (char)atoi(next);
+        } else if (!strcmp(a, "-X") || !strcmp(a, "--saveref")) {
+            NEED_ARG();
+            cfg->issaveref = (char)atoi(next);
+            flagset['X'] = 1;
+        } else if (!strcmp(a, "-x") || !strcmp(a, "--saveexit")) {
+            NEED_ARG();
+            cfg->issaveexit = (char)atoi(next);
+            flagset['x'] = 1;
+        } else if (!strcmp(a, "-V") || !strcmp(a, "--specular")) {
+            NEED_ARG();
+            cfg->isspecular = (char)atoi(next);
+            flagset['V'] = 1;
+        } else if (!strcmp(a, "-m") || !strcmp(a, "--momentum")) {
+            NEED_ARG();
+            cfg->ismomentum = (char)atoi(next);
+            flagset['m'] = 1;
+        } else if (!strcmp(a, "--srcid")) {
+            NEED_ARG();
+            cfg->srcid = atoi(next);
         } else {
             mcx_usage(cfg, argv[0]);
             MCX_ERROR(-1, "unsupported command line option");
@@ -1884,6 +2187,18 @@ int mcx_try_readconfig(const char* fname, Config* cfg) {
 
 int mcx_try_parse_shapestring(Config* cfg, const char* shapedata) {
     MCX_TRY(mcx_parse_shapestring(cfg, shapedata));
+}
+
+int mcx_try_parsecmd(int argc, char* argv[], Config* cfg) {
+    MCX_TRY(mcx_parsecmd(argc, argv, cfg));
+}
+
+int mcx_try_savedata(float* dat, size_t len, Config* cfg) {
+    MCX_TRY(mcx_savedata(dat, len, cfg));
+}
+
+int mcx_try_savedetphoton(float* ppath, void* seeds, int count, int doappend, Config* cfg) {
+    MCX_TRY(mcx_savedetphoton(ppath, seeds, count, doappend, cfg));
 }
 
 /* ---------------- ABI description ---------------- */
