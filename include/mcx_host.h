@@ -53,6 +53,39 @@ int mcx_device_rng_stream(int gpuid, const unsigned int seed4[4], unsigned int n
    an ncell-cell grid on device gpuid: the ceiling the fluence deposits run against */
 int mcx_device_atomic_ceiling(int gpuid, unsigned int ncell, double* rate);
 
+/* ---- batched launch (no counterpart in the reference, whose host runs one
+   Config per call): many independent simulations with their own volumes,
+   media, sources and photon counts run in one persistent launch ---- */
+typedef struct MCXBatchOpt {
+    unsigned int blocksize;      /* lanes per workgroup: 64, 128, 192 or 256; 0 picks by autotuning */
+    unsigned int chunk;          /* photons per work item; 0 picks by autotuning */
+    unsigned int usegraph;       /* 1: each step replays a captured HIP graph (prepare, transport, finalize) */
+    unsigned int nqueue;         /* work queues, one per XCD; 0 means 8 */
+    /* filled in by the library */
+    unsigned int nblock;         /* resident workgroups of the launch */
+    unsigned int nitems;         /* work items of a step */
+    float stepms;                /* device time of the last step (prepare + transport + finalize) */
+    float kernelms;              /* device time of the last transport kernel (steps run without a graph) */
+    float tunems;                /* time spent autotuning */
+} MCXBatchOpt;
+
+typedef struct MCXBatch MCXBatch;
+
+/* Validate and stage n prepared Configs (as mcx_run_simulation takes them) on
+   the device of the first; the Configs stay owned by the caller and must
+   outlive the batch.  Restrictions: fast arithmetic (mathmode 0), no detected
+   photons, no replay.  Returns 0, or an error id with mcx_last_error(). */
+int mcx_batch_create(Config** cfgs, unsigned int n, MCXBatchOpt* opt, MCXBatch** out);
+/* nstep steps; each simulates every instance afresh (the RNG streams continue) */
+int mcx_batch_run(MCXBatch* batch, unsigned int nstep);
+/* the last step's fluence into each Config's exportfield (allocated when NULL),
+   its raw grid into exportfield64 when set, and energytot/energyesc/energyabs/
+   normalizer/kernelms */
+int mcx_batch_collect(MCXBatch* batch);
+void mcx_batch_destroy(MCXBatch* batch);
+/* create, one step, collect, destroy */
+int mcx_run_batch(Config** cfgs, unsigned int n, MCXBatchOpt* opt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -211,6 +211,26 @@ lib.mcx_cleargpuinfo.argtypes = [_P(_P(GPUInfo))]
 lib.mcx_cleargpuinfo.restype = None
 
 
+class MCXBatchOpt(ctypes.Structure):
+    """include/mcx_host.h MCXBatchOpt"""
+    _fields_ = [("blocksize", ctypes.c_uint), ("chunk", ctypes.c_uint), ("usegraph", ctypes.c_uint),
+                ("nqueue", ctypes.c_uint), ("nblock", ctypes.c_uint), ("nitems", ctypes.c_uint),
+                ("stepms", ctypes.c_float), ("kernelms", ctypes.c_float), ("tunems", ctypes.c_float)]
+
+
+lib.mcx_batch_create.argtypes = [ctypes.POINTER(_P(Config)), ctypes.c_uint, _P(MCXBatchOpt),
+                                 ctypes.POINTER(ctypes.c_void_p)]
+lib.mcx_batch_create.restype = ctypes.c_int
+lib.mcx_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+lib.mcx_batch_run.restype = ctypes.c_int
+lib.mcx_batch_collect.argtypes = [ctypes.c_void_p]
+lib.mcx_batch_collect.restype = ctypes.c_int
+lib.mcx_batch_destroy.argtypes = [ctypes.c_void_p]
+lib.mcx_batch_destroy.restype = None
+lib.mcx_run_batch.argtypes = [ctypes.POINTER(_P(Config)), ctypes.c_uint, _P(MCXBatchOpt)]
+lib.mcx_run_batch.restype = ctypes.c_int
+
+
 def last_error():
     msg = lib.mcx_last_error()
     return msg.decode(errors="replace") if msg else ""

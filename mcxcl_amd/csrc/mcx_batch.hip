@@ -1,0 +1,749 @@
+/*
+ * mcx_batch.hip - many independent simulations in one persistent launch.
+ *
+ * The reference's host runs one Config per mcx_run_simulation call; a study of
+ * many small domains then pays one launch, one set of uploads and one tail of
+ * idle lanes per simulation.  Here n prepared Configs (own volume, media,
+ * source, photon count, time gates) are staged once and each step runs them
+ * all in one persistent kernel (batch_kernel in mcx_core_impl.h):
+ *   - each instance's photons are cut into work items of `chunk` photons;
+ *   - the items sit in one queue per XCD, instances assigned to queues by
+ *     longest-processing-time first, so an XCD's L2 holds few volumes/grids;
+ *   - a workgroup takes an item, stages the instance's tables in LDS, hands
+ *     its photons to lanes one at a time, and flushes its deposit box;
+ *   - a step is prepare (zero grids and tallies, reset queues) + transport +
+ *     finalize (per-instance normalisation on the device), optionally captured
+ *     once as a HIP graph and replayed.
+ * The launch shape (lanes per workgroup) and the item size are picked by a
+ * short autotuning pass over trial steps with scaled-down photon counts unless
+ * the caller fixes them.
+ *
+ * Device memory of a batch: per instance its labels (uint32 nvox), media
+ * table, phase table, extra sources, float64 grid and float32 output; shared:
+ * kernel parameters, normalisation records, work items and queues, one RNG
+ * state per lane (continued from step to step) and two energy tallies per
+ * instance.
+ */
+#include <math.h>
+#include <setjmp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <hip/hip_runtime.h>
+
+#include "mcx_core.h"
+#include "mcx_host.h"
+
+extern "C" void* mcx_push_errjmp(void* jb);
+extern "C" void mcx_pop_errjmp(void* old);
+void mcx_check_supported(const Config* cfg);
+size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lblock, int* reflect);
+
+#define MCX_BATCH_MAXBUF 16
+#define MCX_BATCH_MAXBLOCK 256
+#define MCX_BATCH_TUNE_PHOTONS 4.0e6
+
+struct MCXBatch {
+    int dev;
+    unsigned int n;
+    Config** cfg;
+    MCXBatchOpt* opt;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1, ev2, ev3;
+    void* buf[MCX_BATCH_MAXBUF];
+    int nbuf;
+    /* per instance */
+    MCXKParam* prm;                 /* host copy of the kernel parameters */
+    MCXBatchNorm* norm;
+    size_t* fieldoff;               /* offset of each grid in dfield / output in dout */
+    uint32_t** dmedia;
+    float4** dprop;
+    float** dinvcdf;
+    float4** dsrcdata;
+    /* shared device buffers */
+    MCXKParam* dprm;
+    MCXBatchNorm* dnorm;
+    uint4* ditems;
+    uint32_t* dqueue;
+    uint32_t* dqueue0;
+    uint32_t* dnseed;
+    double* denergy;
+    double* dfield;
+    float* dout;
+    size_t nfield, maxlen, maxitems, maxlanes;
+    /* schedule */
+    uint4* items;
+    uint32_t* queue0;
+    uint32_t* qof;                  /* queue of each instance */
+    uint32_t* order;                /* instances by decreasing photon count */
+    uint32_t nitems, nqueue;
+    size_t lblock, nblock, shmem;
+    unsigned int chunk;
+    int reflect;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+#define BHIP(call)                                                                    \
+    do {                                                                              \
+        hipError_t e__ = (call);                                                      \
+        if (e__ != hipSuccess) {                                                      \
+            char m__[256];                                                            \
+            snprintf(m__, sizeof(m__), "HIP error %d (%s) in the batched launch",     \
+                     (int)e__, hipGetErrorString(e__));                               \
+            mcx_error(-(int)e__, m__, __FILE__, __LINE__);                            \
+        }                                                                             \
+    } while (0)
+
+template <typename T>
+static T* batch_alloc(MCXBatch* b, size_t n) {
+    void* p = NULL;
+
+    if (b->nbuf >= MCX_BATCH_MAXBUF) {
+        MCX_ERROR(-4, "too many device buffers in a batch");
+    }
+
+    BHIP(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+    b->buf[b->nbuf++] = p;
+    return (T*)p;
+}
+
+static void* host_calloc(size_t n, size_t size) {
+    void* p = calloc(n ? n : 1, size);
+
+    if (!p) {
+        MCX_ERROR(-4, "out of host memory");
+    }
+
+    return p;
+}
+
+static void batch_free(MCXBatch* b) {
+    if (!b) {
+        return;
+    }
+
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+    }
+
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+    }
+
+    for (int i = 0; i < b->nbuf; i++) {
+        (void)hipFree(b->buf[i]);
+    }
+
+    /* per-instance buffers share one allocation each kind; freed above */
+    hipEvent_t ev[4] = {b->ev0, b->ev1, b->ev2, b->ev3};
+
+    for (int i = 0; i < 4; i++) {
+        if (ev[i]) {
+            (void)hipEventDestroy(ev[i]);
+        }
+    }
+
+    if (b->stream) {
+        (void)hipStreamDestroy(b->stream);
+    }
+
+    free(b->prm);
+    free(b->norm);
+    free(b->fieldoff);
+    free(b->dmedia);
+    free(b->dprop);
+    free(b->dinvcdf);
+    free(b->dsrcdata);
+    free(b->items);
+    free(b->queue0);
+    free(b->qof);
+    free(b->order);
+    free(b->cfg);
+    free(b);
+}
+
+/* kernel parameters of every instance for workgroups of lblock lanes; returns
+   the LDS bytes the launch needs (the largest instance's) */
+static size_t batch_params(MCXBatch* b, size_t lblock) {
+    size_t shmem = 0;
+    b->reflect = 0;
+
+    for (unsigned int i = 0; i < b->n; i++) {
+        Config* cfg = b->cfg[i];
+        MCXKParam& g = b->prm[i];
+        int reflect = 0;
+        size_t s = mcx_kparam_setup(cfg, &g, lblock, lblock, &reflect);
+        shmem = s > shmem ? s : shmem;
+        b->reflect |= reflect;
+        g.nrep = 1;
+        g.dynamic = 1;
+        g.media = b->dmedia[i];
+        g.prop = b->dprop[i];
+        g.invcdf = b->dinvcdf[i];
+        g.srcdata = b->dsrcdata[i];
+        g.field = b->dfield + b->fieldoff[i];
+        g.genergy = NULL;
+        g.nseed = NULL;
+    }
+
+    return shmem;
+}
+
+/* work items of `chunk` photons (photon counts scaled by frac) in per-queue
+   order; instances were assigned to queues by batch_assign */
+static void batch_items(MCXBatch* b, unsigned int chunk, double frac, size_t* nphoton) {
+    uint32_t k = 0;
+    size_t total = 0;
+
+    for (uint32_t q = 0; q < b->nqueue; q++) {
+        b->queue0[2 * q] = k;
+
+        for (unsigned int r = 0; r < b->n; r++) {
+            const unsigned int i = b->order[r];
+
+            if (b->qof[i] != q) {
+                continue;
+            }
+
+            size_t np = (size_t)(b->cfg[i]->nphoton * frac + 0.5);
+            np = (np == 0 && b->cfg[i]->nphoton) ? 1 : np;
+            total += np;
+
+            for (size_t s = 0; s < np; s += chunk) {
+                if (k >= b->maxitems) {
+                    MCX_ERROR(-4, "work item table overflow");
+                }
+
+                size_t e = s + chunk < np ? s + chunk : np;
+                b->items[k].x = i;
+                b->items[k].y = (uint32_t)s;
+                b->items[k].z = (uint32_t)e;
+                b->items[k].w = 0;
+                k++;
+            }
+        }
+
+        b->queue0[2 * q + 1] = k;
+    }
+
+    b->nitems = k;
+    *nphoton = total;
+}
+
+/* instances to queues, longest first onto the least-loaded queue; a queue
+   runs its instances longest first */
+static void batch_assign(MCXBatch* b) {
+    uint32_t* order = b->order;
+    double* load = (double*)host_calloc(b->nqueue, sizeof(double));
+
+    for (unsigned int i = 0; i < b->n; i++) {
+        order[i] = i;
+    }
+
+    for (unsigned int i = 1; i < b->n; i++) {   /* insertion sort, decreasing photon count */
+        uint32_t v = order[i];
+        unsigned int j = i;
+
+        while (j > 0 && b->cfg[order[j - 1]]->nphoton < b->cfg[v]->nphoton) {
+            order[j] = order[j - 1];
+            j--;
+        }
+
+        order[j] = v;
+    }
+
+    for (unsigned int r = 0; r < b->n; r++) {
+        unsigned int i = order[r];
+        uint32_t best = 0;
+
+        for (uint32_t q = 1; q < b->nqueue; q++) {
+            best = load[q] < load[best] ? q : best;
+        }
+
+        b->qof[i] = best;
+        load[best] += (double)b->cfg[i]->nphoton;
+    }
+
+    free(load);
+}
+
+static void batch_upload_schedule(MCXBatch* b) {
+    BHIP(hipMemcpyAsync(b->dprm, b->prm, b->n * sizeof(MCXKParam), hipMemcpyHostToDevice, b->stream));
+    BHIP(hipMemcpyAsync(b->ditems, b->items, (size_t)b->nitems * sizeof(uint4), hipMemcpyHostToDevice, b->stream));
+    BHIP(hipMemcpyAsync(b->dqueue0, b->queue0, 2 * (size_t)b->nqueue * sizeof(uint32_t), hipMemcpyHostToDevice,
+                        b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+}
+
+/* one step on the stream: prepare, transport, finalize; ev1/ev2 bracket the
+   transport kernel when timed */
+static void batch_enqueue(MCXBatch* b, int timed) {
+    BHIP(mcx_launch_batch_prep(b->dfield, b->nfield, b->denergy, 2 * b->n, b->dqueue, b->dqueue0, b->nqueue,
+                               b->stream));
+
+    if (timed) {
+        BHIP(hipEventRecord(b->ev1, b->stream));
+    }
+
+    BHIP(mcx_launch_batch(b->dprm, b->ditems, b->nqueue, b->dqueue, b->dnseed, b->denergy, (int)b->nblock,
+                          (int)b->lblock, b->shmem, b->reflect, b->stream));
+
+    if (timed) {
+        BHIP(hipEventRecord(b->ev2, b->stream));
+    }
+
+    BHIP(mcx_launch_batch_finalize(b->dnorm, b->denergy, b->n, b->maxlen, b->stream));
+}
+
+/* resident workgroups for lblock lanes; sets b->lblock, b->nblock, b->shmem */
+static int batch_shape(MCXBatch* b, size_t lblock, int ncu) {
+    size_t shmem = batch_params(b, lblock);
+
+    if (shmem > 64 * 1024) {
+        return 0;
+    }
+
+    int nb = mcx_batch_occupancy((int)lblock, shmem, b->reflect);
+
+    if (nb <= 0 || (size_t)nb * ncu * lblock > b->maxlanes) {
+        return 0;
+    }
+
+    b->lblock = lblock;
+    b->nblock = (size_t)nb * ncu;
+    b->shmem = shmem;
+    return 1;
+}
+
+static double wall_ms_batch(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBatchOpt* opt) {
+    if (n == 0 || !cfgs) {
+        MCX_ERROR(-4, "a batch needs at least one configuration");
+    }
+
+    b->n = n;
+    b->opt = opt;
+    b->cfg = (Config**)host_calloc(n, sizeof(Config*));
+
+    for (unsigned int i = 0; i < n; i++) {
+        Config* cfg = cfgs[i];
+        b->cfg[i] = cfg;
+        mcx_check_supported(cfg);
+
+        if (cfg->mathmode != 0) {
+            MCX_ERROR(-4, "the batched launch runs the fast arithmetic only (mathmode 0)");
+        }
+
+        if (cfg->issavedet || cfg->seed == SEED_FROM_FILE || (cfg->debuglevel & MCX_DEBUG_RNG) || cfg->muamus) {
+            MCX_ERROR(-4, "the batched launch takes no detected photons, replay, RNG dump or per-voxel media");
+        }
+
+        if (cfg->nphoton >= 0xFFFFFFFFull) {
+            MCX_ERROR(-4, "more than 2^32-1 photons in one instance");
+        }
+    }
+
+    if (opt->blocksize && (opt->blocksize < 64 || opt->blocksize > MCX_BATCH_MAXBLOCK || (opt->blocksize & 63))) {
+        MCX_ERROR(-4, "blocksize must be 64, 128, 192 or 256");
+    }
+
+    unsigned int activedev = 0;
+    int devlist[MAX_DEVICE];
+    GPUInfo* gpu = NULL;
+    int ndev = mcx_list_gpu(b->cfg[0], &activedev, devlist, &gpu);
+
+    if (ndev == 0 || activedev == 0) {
+        free(gpu);
+        MCX_ERROR(-99, "No HIP device found (or the specified GPU does not exist)");
+    }
+
+    b->dev = devlist[0];
+    const int ncu = gpu[b->dev].sm;
+    free(gpu);
+    BHIP(hipSetDevice(b->dev));
+    BHIP(hipStreamCreate(&b->stream));
+    BHIP(hipEventCreate(&b->ev0));
+    BHIP(hipEventCreate(&b->ev1));
+    BHIP(hipEventCreate(&b->ev2));
+    BHIP(hipEventCreate(&b->ev3));
+
+    b->prm = (MCXKParam*)host_calloc(n, sizeof(MCXKParam));
+    b->norm = (MCXBatchNorm*)host_calloc(n, sizeof(MCXBatchNorm));
+    b->fieldoff = (size_t*)host_calloc(n + 1, sizeof(size_t));
+    b->dmedia = (uint32_t**)host_calloc(n, sizeof(uint32_t*));
+    b->dprop = (float4**)host_calloc(n, sizeof(float4*));
+    b->dinvcdf = (float**)host_calloc(n, sizeof(float*));
+    b->dsrcdata = (float4**)host_calloc(n, sizeof(float4*));
+    b->qof = (uint32_t*)host_calloc(n, sizeof(uint32_t));
+    b->order = (uint32_t*)host_calloc(n, sizeof(uint32_t));
+    b->nqueue = opt->nqueue ? opt->nqueue : 8;
+
+    /* sizes: grids and per-instance input tables, each kind in one allocation */
+    size_t nvoxall = 0, npropall = 0, nphaseall = 0, nsrcall = 0, maxlen = 0;
+
+    for (unsigned int i = 0; i < n; i++) {
+        Config* cfg = b->cfg[i];
+        MCXKParam g;
+        int reflect;
+        (void)mcx_kparam_setup(cfg, &g, 64, 64, &reflect);
+        b->fieldoff[i + 1] = b->fieldoff[i] + g.fieldlen;
+        maxlen = g.fieldlen > maxlen ? g.fieldlen : maxlen;
+        nvoxall += g.nvox;
+        npropall += cfg->medianum;
+        nphaseall += (cfg->nphase + 3) & ~3u;
+        nsrcall += (size_t)cfg->extrasrclen * 4;
+    }
+
+    b->nfield = b->fieldoff[n];
+    b->maxlen = maxlen;
+    uint32_t* media = batch_alloc<uint32_t>(b, nvoxall);
+    float4* prop = batch_alloc<float4>(b, npropall);
+    float* phase = batch_alloc<float>(b, nphaseall);
+    float4* src = batch_alloc<float4>(b, nsrcall);
+    b->dfield = batch_alloc<double>(b, b->nfield);
+    b->dout = batch_alloc<float>(b, b->nfield);
+
+    for (unsigned int i = 0; i < n; i++) {
+        Config* cfg = b->cfg[i];
+        size_t nvox = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
+        b->dmedia[i] = media;
+        b->dprop[i] = prop;
+        b->dinvcdf[i] = cfg->nphase ? phase : NULL;
+        b->dsrcdata[i] = cfg->extrasrclen ? src : NULL;
+        BHIP(hipMemcpyAsync(media, cfg->vol, nvox * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+        BHIP(hipMemcpyAsync(prop, cfg->prop, cfg->medianum * sizeof(float4), hipMemcpyHostToDevice, b->stream));
+
+        if (cfg->nphase) {
+            BHIP(hipMemcpyAsync(phase, cfg->invcdf, cfg->nphase * sizeof(float), hipMemcpyHostToDevice, b->stream));
+        }
+
+        if (cfg->extrasrclen) {
+            BHIP(hipMemcpyAsync(src, cfg->srcdata, cfg->extrasrclen * sizeof(ExtraSrc), hipMemcpyHostToDevice,
+                                b->stream));
+        }
+
+        media += nvox;
+        prop += cfg->medianum;
+        phase += (cfg->nphase + 3) & ~3u;
+        src += (size_t)cfg->extrasrclen * 4;
+
+        MCXBatchNorm& nm = b->norm[i];
+        nm.field = b->dfield + b->fieldoff[i];
+        nm.out = b->dout + b->fieldoff[i];
+        nm.len = b->fieldoff[i + 1] - b->fieldoff[i];
+        nm.unitinmm = cfg->unitinmm;
+        nm.vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
+        nm.tstep = cfg->tstep;
+        nm.outputtype = (uint32_t)cfg->outputtype;
+        nm.normalize = cfg->isnormalized ? 1u : 0u;
+    }
+
+    /* schedule tables sized for the smallest item the tuner may pick */
+    const unsigned int minchunk = opt->chunk ? opt->chunk : 64u * 4u;
+    size_t maxitems = 0;
+
+    for (unsigned int i = 0; i < n; i++) {
+        maxitems += (b->cfg[i]->nphoton + minchunk - 1) / minchunk + 1;
+    }
+
+    b->maxitems = maxitems;
+    b->items = (uint4*)host_calloc(maxitems, sizeof(uint4));
+    b->queue0 = (uint32_t*)host_calloc(2 * b->nqueue, sizeof(uint32_t));
+    b->maxlanes = (size_t)ncu * 2048;   /* 32 waves of 64 lanes per CU at most */
+    b->dprm = batch_alloc<MCXKParam>(b, n);
+    b->dnorm = batch_alloc<MCXBatchNorm>(b, n);
+    b->ditems = batch_alloc<uint4>(b, maxitems);
+    b->dqueue = batch_alloc<uint32_t>(b, 2 * b->nqueue);
+    b->dqueue0 = batch_alloc<uint32_t>(b, 2 * b->nqueue);
+    b->denergy = batch_alloc<double>(b, 2 * (size_t)n);
+    b->dnseed = batch_alloc<uint32_t>(b, b->maxlanes * 4);
+    BHIP(hipMemcpyAsync(b->dnorm, b->norm, n * sizeof(MCXBatchNorm), hipMemcpyHostToDevice, b->stream));
+
+    /* per-lane RNG seeds from the first instance's seed (src/mcx_host.cpp:696-700) */
+    {
+        unsigned int* hseed = (unsigned int*)host_calloc(b->maxlanes * 4, sizeof(unsigned int));
+        Config* c0 = b->cfg[0];
+        mcx_host_seed_words(c0->seed > 0 ? (unsigned int)c0->seed : (unsigned int)time(0), b->maxlanes * 4, hseed);
+        hipError_t e = hipMemcpy(b->dnseed, hseed, b->maxlanes * 4 * sizeof(unsigned int), hipMemcpyHostToDevice);
+        free(hseed);
+        BHIP(e);
+    }
+
+    batch_assign(b);
+
+    /* autotuning: trial steps at a scaled-down photon count over the candidate
+       launch shapes and item sizes; the best photons/ms wins */
+    const size_t blocks[3] = {64, 128, 256};
+    const unsigned int perlane[3] = {4, 16, 64};
+    size_t total = 0;
+
+    for (unsigned int i = 0; i < n; i++) {
+        total += b->cfg[i]->nphoton;
+    }
+
+    size_t bestblock = opt->blocksize;
+    unsigned int bestchunk = opt->chunk;
+    const double t0 = wall_ms_batch();
+
+    if (!bestblock || !bestchunk) {
+        double frac = total > MCX_BATCH_TUNE_PHOTONS ? MCX_BATCH_TUNE_PHOTONS / (double)total : 1.0;
+        double bestrate = -1.0;
+
+        for (int bi = 0; bi < 3; bi++) {
+            size_t lb = opt->blocksize ? opt->blocksize : blocks[bi];
+
+            if ((opt->blocksize && bi > 0) || !batch_shape(b, lb, ncu)) {
+                continue;
+            }
+
+            for (int ci = 0; ci < 3; ci++) {
+                unsigned int ch = opt->chunk ? opt->chunk : perlane[ci] * (unsigned int)lb;
+
+                if ((opt->chunk && ci > 0) || ch < minchunk) {
+                    continue;
+                }
+
+                size_t np = 0;
+                batch_items(b, ch, frac, &np);
+                batch_upload_schedule(b);
+                batch_enqueue(b, 1);
+                float ms = 0.f;
+                BHIP(hipEventSynchronize(b->ev2));
+                BHIP(hipEventElapsedTime(&ms, b->ev1, b->ev2));
+                double rate = ms > 0.f ? np / (double)ms : 0.0;
+
+                if (rate > bestrate) {
+                    bestrate = rate;
+                    bestblock = lb;
+                    bestchunk = ch;
+                }
+            }
+        }
+
+        if (bestrate < 0.0) {
+            MCX_ERROR(-4, "no launch shape of the batched kernel fits these instances");
+        }
+    }
+
+    opt->tunems = (float)(wall_ms_batch() - t0);
+
+    if (!batch_shape(b, bestblock, ncu)) {
+        MCX_ERROR(-4, "the requested launch shape of the batched kernel does not fit");
+    }
+
+    size_t np = 0;
+    batch_items(b, bestchunk, 1.0, &np);
+    batch_upload_schedule(b);
+    b->chunk = bestchunk;
+    opt->blocksize = (unsigned int)b->lblock;
+    opt->chunk = bestchunk;
+    opt->nblock = (unsigned int)b->nblock;
+    opt->nitems = b->nitems;
+
+    if (getenv("MCXHIP_VERBOSE")) {
+        fprintf(stderr, "mcxhip batch: %u instances, %u items of %u photons, %zu workgroups of %zu lanes, LDS %zu B, "
+                "%u queues, tuning %.1f ms\n", n, b->nitems, bestchunk, b->nblock, b->lblock, b->shmem, b->nqueue,
+                opt->tunems);
+    }
+
+    if (opt->usegraph) {
+        BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        batch_enqueue(b, 0);
+        BHIP(hipStreamEndCapture(b->stream, &b->graph));
+        BHIP(hipGraphInstantiate(&b->exec, b->graph, NULL, NULL, 0));
+    }
+}
+
+static void batch_run_impl(MCXBatch* b, unsigned int nstep) {
+    for (unsigned int s = 0; s < nstep; s++) {
+        const int last = (s + 1 == nstep);
+
+        if (last) {
+            BHIP(hipEventRecord(b->ev0, b->stream));
+        }
+
+        if (b->exec) {
+            BHIP(hipGraphLaunch(b->exec, b->stream));
+        } else {
+            batch_enqueue(b, last);
+        }
+
+        if (last) {
+            BHIP(hipEventRecord(b->ev3, b->stream));
+        }
+    }
+
+    BHIP(hipStreamSynchronize(b->stream));
+
+    if (nstep) {
+        float ms = 0.f;
+        BHIP(hipEventElapsedTime(&ms, b->ev0, b->ev3));
+        b->opt->stepms = ms;
+
+        if (!b->exec) {
+            BHIP(hipEventElapsedTime(&ms, b->ev1, b->ev2));
+            b->opt->kernelms = ms;
+        }
+    }
+}
+
+static void batch_collect_impl(MCXBatch* b) {
+    double* energy = (double*)host_calloc(2 * (size_t)b->n, sizeof(double));
+    float* out = (float*)malloc((b->nfield ? b->nfield : 1) * sizeof(float));
+
+    if (!out) {
+        free(energy);
+        MCX_ERROR(-4, "out of host memory");
+    }
+
+    hipError_t e = hipMemcpy(energy, b->denergy, 2 * (size_t)b->n * sizeof(double), hipMemcpyDeviceToHost);
+
+    if (e == hipSuccess) {
+        e = hipMemcpy(out, b->dout, b->nfield * sizeof(float), hipMemcpyDeviceToHost);
+    }
+
+    for (unsigned int i = 0; i < b->n && e == hipSuccess; i++) {
+        Config* cfg = b->cfg[i];
+        size_t len = b->fieldoff[i + 1] - b->fieldoff[i];
+        cfg->energyesc = energy[2 * i];
+        cfg->energytot = energy[2 * i + 1];
+        cfg->energyabs = cfg->energytot - cfg->energyesc;
+        cfg->kernelms = b->opt->kernelms;
+        cfg->jobms = b->opt->stepms;
+        cfg->runtimems = b->opt->stepms;
+        cfg->runtime = (unsigned int)(b->opt->stepms + 0.5f);
+        const MCXBatchNorm& nm = b->norm[i];
+
+        /* the same float scale as the device finalize */
+        if (nm.normalize && cfg->energytot > 0.0) {
+            float scale = 1.f;
+
+            if (nm.outputtype == otFlux || nm.outputtype == otFluence) {
+                scale = nm.unitinmm / (float)(cfg->energytot * (double)nm.vvox * (double)nm.tstep);
+
+                if (nm.outputtype == otFluence) {
+                    scale *= nm.tstep;
+                }
+            } else if (nm.outputtype == otEnergy || nm.outputtype == otL) {
+                scale = 1.f / (float)cfg->energytot;
+            }
+
+            cfg->normalizer = scale;
+            cfg->his.normalizer = scale;
+        }
+
+        if (cfg->issave2pt) {
+            if (!cfg->exportfield) {
+                cfg->exportfield = (float*)calloc(len * 2, sizeof(float));
+            }
+
+            if (cfg->exportfield) {
+                memcpy(cfg->exportfield, out + b->fieldoff[i], len * sizeof(float));
+            }
+
+            if (cfg->exportfield64) {
+                e = hipMemcpy(cfg->exportfield64, b->dfield + b->fieldoff[i], len * sizeof(double),
+                              hipMemcpyDeviceToHost);
+            }
+        }
+    }
+
+    free(energy);
+    free(out);
+    BHIP(e);
+}
+
+/* ---------------- C entry points: errors return an id, never exit ---------------- */
+extern "C" int mcx_batch_create(Config** cfgs, unsigned int n, MCXBatchOpt* opt, MCXBatch** out) {
+    MCXBatch* volatile b = (MCXBatch*)calloc(1, sizeof(MCXBatch));
+    *out = NULL;
+
+    if (!b) {
+        return -4;
+    }
+
+    jmp_buf jb;
+    void* old = mcx_push_errjmp(&jb);
+    int rc = setjmp(jb);
+
+    if (rc == 0) {
+        batch_create_impl(b, cfgs, n, opt);
+    }
+
+    mcx_pop_errjmp(old);
+
+    if (rc) {
+        batch_free(b);
+        return rc;
+    }
+
+    *out = b;
+    return 0;
+}
+
+extern "C" int mcx_batch_run(MCXBatch* b, unsigned int nstep) {
+    jmp_buf jb;
+    void* old = mcx_push_errjmp(&jb);
+    int rc = setjmp(jb);
+
+    if (rc == 0) {
+        if (!b) {
+            MCX_ERROR(-4, "no batch");
+        }
+
+        BHIP(hipSetDevice(b->dev));
+        batch_run_impl(b, nstep);
+    }
+
+    mcx_pop_errjmp(old);
+    return rc;
+}
+
+extern "C" int mcx_batch_collect(MCXBatch* b) {
+    jmp_buf jb;
+    void* old = mcx_push_errjmp(&jb);
+    int rc = setjmp(jb);
+
+    if (rc == 0) {
+        if (!b) {
+            MCX_ERROR(-4, "no batch");
+        }
+
+        BHIP(hipSetDevice(b->dev));
+        batch_collect_impl(b);
+    }
+
+    mcx_pop_errjmp(old);
+    return rc;
+}
+
+extern "C" void mcx_batch_destroy(MCXBatch* b) {
+    if (b) {
+        (void)hipSetDevice(b->dev);
+        batch_free(b);
+    }
+}
+
+extern "C" int mcx_run_batch(Config** cfgs, unsigned int n, MCXBatchOpt* opt) {
+    MCXBatch* b = NULL;
+    int rc = mcx_batch_create(cfgs, n, opt, &b);
+
+    if (rc == 0) {
+        rc = mcx_batch_run(b, 1);
+    }
+
+    if (rc == 0) {
+        rc = mcx_batch_collect(b);
+    }
+
+    mcx_batch_destroy(b);
+    return rc;
+}

@@ -62,6 +62,96 @@ extern "C" int mcx_fast_occupancy(int nthreadblock, size_t shmem, int savedet, i
            : occupancy_of<false, false, false>(nthreadblock, shmem);
 }
 
+/* ---------------- batched launch ---------------- */
+extern "C" hipError_t mcx_launch_batch(const MCXKParam* prm, const uint4* items, uint32_t nqueue, uint32_t* queue,
+                                       uint32_t* nseed, double* energy, int nblock, int nthreadblock, size_t shmem,
+                                       int reflect, hipStream_t stream) {
+    if (reflect) {
+        hipLaunchKernelGGL(mcxk::batch_kernel<true>, dim3(nblock), dim3(nthreadblock), shmem, stream, prm, items,
+                           nqueue, queue, nseed, energy);
+    } else {
+        hipLaunchKernelGGL(mcxk::batch_kernel<false>, dim3(nblock), dim3(nthreadblock), shmem, stream, prm, items,
+                           nqueue, queue, nseed, energy);
+    }
+
+    return hipGetLastError();
+}
+
+extern "C" int mcx_batch_occupancy(int nthreadblock, size_t shmem, int reflect) {
+    int nb = 0;
+    hipError_t e = reflect ?
+                   hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxk::batch_kernel<true>, nthreadblock, shmem) :
+                   hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxk::batch_kernel<false>, nthreadblock, shmem);
+    return e == hipSuccess ? nb : 0;
+}
+
+/* start of a batched step: zero the grids and energy tallies and reset the
+   work queues (a kernel, not hipMemsetAsync, so the step can be captured in a
+   HIP graph and replayed) */
+__global__ void batch_prep_kernel(double* __restrict__ field, size_t nfield, double* __restrict__ energy,
+                                  uint32_t nenergy, uint32_t* __restrict__ queue, const uint32_t* __restrict__ queue0,
+                                  uint32_t nqueue) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+
+    for (size_t i = i0; i < nfield; i += stride) {
+        field[i] = 0.0;
+    }
+
+    if (i0 < nenergy) {
+        energy[i0] = 0.0;
+    }
+
+    if (i0 < 2 * (size_t)nqueue) {
+        queue[i0] = queue0[i0];
+    }
+}
+
+extern "C" hipError_t mcx_launch_batch_prep(double* field, size_t nfield, double* energy, uint32_t nenergy,
+        uint32_t* queue, const uint32_t* queue0, uint32_t nqueue, hipStream_t stream) {
+    size_t need = nfield > nenergy ? nfield : nenergy;
+    need = need > 2 * (size_t)nqueue ? need : 2 * (size_t)nqueue;
+    size_t nb = (need + 255) / 256;
+    nb = nb < 4096 ? (nb ? nb : 1) : 4096;
+    hipLaunchKernelGGL(batch_prep_kernel, dim3((unsigned int)nb), dim3(256), 0, stream, field, nfield, energy,
+                       nenergy, queue, queue0, nqueue);
+    return hipGetLastError();
+}
+
+/* end of a batched step: each instance's float output from its grid, scaled by
+   the normalisation of its output type with the step's launched energy
+   (the single-run host's formulas, src/mcx_host.cpp:1382-1465); blockIdx.y
+   selects the instance */
+__global__ void batch_finalize_kernel(const MCXBatchNorm* __restrict__ norm, const double* __restrict__ energy) {
+    const MCXBatchNorm& b = norm[blockIdx.y];
+    const double etot = energy[2 * blockIdx.y + 1];
+    float scale = 1.f;
+
+    if (b.normalize && etot > 0.0) {
+        if (b.outputtype == 0 || b.outputtype == 1) {
+            scale = b.unitinmm / (float)(etot * (double)b.vvox * (double)b.tstep);
+
+            if (b.outputtype == 1) {
+                scale *= b.tstep;
+            }
+        } else if (b.outputtype == 2 || b.outputtype == 7) {
+            scale = 1.f / (float)etot;
+        }
+    }
+
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < b.len; i += (size_t)gridDim.x * blockDim.x) {
+        b.out[i] = (float)b.field[i] * scale;
+    }
+}
+
+extern "C" hipError_t mcx_launch_batch_finalize(const MCXBatchNorm* norm, const double* energy, uint32_t ninst,
+        size_t maxlen, hipStream_t stream) {
+    size_t nb = (maxlen + 255) / 256;
+    nb = nb < 1024 ? (nb ? nb : 1) : 1024;
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned int)nb, ninst), dim3(256), 0, stream, norm, energy);
+    return hipGetLastError();
+}
+
 /* device self-check: math in fast (hardware) mode */
 __global__ void fpmath_fast_kernel(int which, const float* __restrict__ x, float* __restrict__ y, size_t n) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;

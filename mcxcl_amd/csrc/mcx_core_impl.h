@@ -885,72 +885,12 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
     return 0;
 }
 
+/* Transport this lane's photons until its share (fixed map) or its
+   workgroup's pool (run-time hand-out) is exhausted.  The photon state starts
+   fresh; the RNG, the LDS tables and the lane's scratch come in, and the
+   lane's pending grid deposits are flushed before returning. */
 template <bool SAVEDET, bool REFLECT, bool SIMPLE>
-#ifndef MCX_MAX_BLOCK
-#define MCX_MAX_BLOCK 256
-#endif
-#ifdef MCX_WAVES_PER_EU
-#define MCX_KATTR __attribute__((amdgpu_waves_per_eu(MCX_WAVES_PER_EU)))
-#else
-#define MCX_KATTR
-#endif
-__global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(const MCXKParam g) {
-    extern __shared__ float4 lds[];
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t pplen = g.w0offset + g.srcnum;
-    __shared__ uint32_t pool[2];
-    Shared sh;
-    sh.pool = pool;
-    sh.prop = lds;
-    sh.invcdf = (float*)(lds + g.medianum);
-    sh.box = (BoxCell*)(sh.invcdf + ((g.nphase + 3) & ~3u));
-    float* ppbase = (float*)(sh.box + g.box_cells);
-    sh.repoff = (size_t)(idx & (int)(g.nrep - 1)) * g.fieldlen;
-
-    for (uint32_t i = threadIdx.x; i < g.medianum; i += blockDim.x) {
-        sh.prop[i] = g.prop[i];
-    }
-
-    for (uint32_t i = threadIdx.x; i < g.nphase; i += blockDim.x) {
-        sh.invcdf[i] = g.invcdf[i];
-    }
-
-    for (uint32_t i = threadIdx.x; i < g.box_cells; i += blockDim.x) {
-        sh.box[i] = (BoxCell)0;
-    }
-
-    if (g.dynamic && threadIdx.x == 0) {
-        /* this workgroup's share: threadphoton photons, +1 for the first oddphoton groups */
-        uint32_t b = blockIdx.x;
-        uint32_t start = b * g.threadphoton + (b < (uint32_t)g.oddphoton ? b : (uint32_t)g.oddphoton);
-        pool[0] = start;
-        pool[1] = start + g.threadphoton + (b < (uint32_t)g.oddphoton ? 1u : 0u);
-    }
-
-    /* lane-interleaved scratch: element k of lane t at [k*blockDim + t] would avoid
-       bank conflicts, but the record copy reads it sequentially; a per-lane block
-       keeps the indexing simple and the scratch is touched rarely */
-    sh.pp = ppbase + threadIdx.x * pplen;
-
-    for (uint32_t i = 0; i < pplen; i++) {
-        sh.pp[i] = 0.f;
-    }
-
-    __syncthreads();
-
-    /* lanes without photons skip the transport loop but still join the box flush */
-    const bool active = g.dynamic ? (uint32_t)idx < g.nthread :
-                        !((uint32_t)idx >= g.threadphoton * g.nthread + (uint32_t)g.oddphoton ||
-                          (uint32_t)idx >= g.nthread);
-    const uint32_t pd = g.partialdata;
-    /* the tail workgroup may hold lanes past nthread: they read nothing */
-    const uint32_t sidx = (uint32_t)idx < g.nthread ? (uint32_t)idx : 0u;
-    sh.pp[pd] = g.genergy[sidx * 2];
-    sh.pp[pd + 1] = g.genergy[sidx * 2 + 1];
-    Rng rng;
-    rng.s0 = (uint64_t)g.nseed[sidx * 4] << 32 | g.nseed[sidx * 4 + 1];
-    rng.s1 = (uint64_t)g.nseed[sidx * 4 + 2] << 32 | g.nseed[sidx * 4 + 3];
-
+__device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared& sh, Rng& rng, const int idx) {
     Photon s;
     s.p = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x7FC00000u));
     s.v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -976,7 +916,7 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
     dcache.a0 = dcache.a1 = 0xFFFFFFFFu;
     dcache.v0 = dcache.v1 = 0.0;
 
-    while (active) {
+    for (;;) {
         if (relaunch) {
             if (launch_photon<SAVEDET, SIMPLE>(g, sh, s, rng, idx, launchflag)) {
                 break;
@@ -1266,8 +1206,77 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
         }
     }
 
+    depcache_flush(g, sh, dcache);
+}
+
+template <bool SAVEDET, bool REFLECT, bool SIMPLE>
+#ifndef MCX_MAX_BLOCK
+#define MCX_MAX_BLOCK 256
+#endif
+#ifdef MCX_WAVES_PER_EU
+#define MCX_KATTR __attribute__((amdgpu_waves_per_eu(MCX_WAVES_PER_EU)))
+#else
+#define MCX_KATTR
+#endif
+__global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(const MCXKParam g) {
+    extern __shared__ float4 lds[];
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t pplen = g.w0offset + g.srcnum;
+    __shared__ uint32_t pool[2];
+    Shared sh;
+    sh.pool = pool;
+    sh.prop = lds;
+    sh.invcdf = (float*)(lds + g.medianum);
+    sh.box = (BoxCell*)(sh.invcdf + ((g.nphase + 3) & ~3u));
+    float* ppbase = (float*)(sh.box + g.box_cells);
+    sh.repoff = (size_t)(idx & (int)(g.nrep - 1)) * g.fieldlen;
+
+    for (uint32_t i = threadIdx.x; i < g.medianum; i += blockDim.x) {
+        sh.prop[i] = g.prop[i];
+    }
+
+    for (uint32_t i = threadIdx.x; i < g.nphase; i += blockDim.x) {
+        sh.invcdf[i] = g.invcdf[i];
+    }
+
+    for (uint32_t i = threadIdx.x; i < g.box_cells; i += blockDim.x) {
+        sh.box[i] = (BoxCell)0;
+    }
+
+    if (g.dynamic && threadIdx.x == 0) {
+        /* this workgroup's share: threadphoton photons, +1 for the first oddphoton groups */
+        uint32_t b = blockIdx.x;
+        uint32_t start = b * g.threadphoton + (b < (uint32_t)g.oddphoton ? b : (uint32_t)g.oddphoton);
+        pool[0] = start;
+        pool[1] = start + g.threadphoton + (b < (uint32_t)g.oddphoton ? 1u : 0u);
+    }
+
+    /* lane-interleaved scratch: element k of lane t at [k*blockDim + t] would avoid
+       bank conflicts, but the record copy reads it sequentially; a per-lane block
+       keeps the indexing simple and the scratch is touched rarely */
+    sh.pp = ppbase + threadIdx.x * pplen;
+
+    for (uint32_t i = 0; i < pplen; i++) {
+        sh.pp[i] = 0.f;
+    }
+
+    __syncthreads();
+
+    /* lanes without photons skip the transport loop but still join the box flush */
+    const bool active = g.dynamic ? (uint32_t)idx < g.nthread :
+                        !((uint32_t)idx >= g.threadphoton * g.nthread + (uint32_t)g.oddphoton ||
+                          (uint32_t)idx >= g.nthread);
+    const uint32_t pd = g.partialdata;
+    /* the tail workgroup may hold lanes past nthread: they read nothing */
+    const uint32_t sidx = (uint32_t)idx < g.nthread ? (uint32_t)idx : 0u;
+    sh.pp[pd] = g.genergy[sidx * 2];
+    sh.pp[pd + 1] = g.genergy[sidx * 2 + 1];
+    Rng rng;
+    rng.s0 = (uint64_t)g.nseed[sidx * 4] << 32 | g.nseed[sidx * 4 + 1];
+    rng.s1 = (uint64_t)g.nseed[sidx * 4 + 2] << 32 | g.nseed[sidx * 4 + 3];
+
     if (active) {
-        depcache_flush(g, sh, dcache);
+        transport_lane<SAVEDET, REFLECT, SIMPLE>(g, sh, rng, idx);
         g.genergy[idx * 2] = sh.pp[pd];
         g.genergy[idx * 2 + 1] = sh.pp[pd + 1];
     }
@@ -1290,6 +1299,135 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(cons
             }
         }
     }
+}
+
+/* sum of a double over the 64 lanes of a wave (result in every lane) */
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off, 64);
+    }
+
+    return v;
+}
+
+/* Batched transport: many simulations in one persistent launch.
+   Work items {instance, first photon, end photon} are grouped in nqueue queues,
+   one per XCD: workgroup b starts at queue b % nqueue, which holds the
+   instances of the XCD it is dispatched to, so one XCD's L2 serves few
+   volumes and grids at a time, and steals from the other queues once its own
+   is empty.  For each item a workgroup stages the instance's tables in LDS,
+   hands the item's photons to its lanes one at a time from an LDS pool,
+   reduces the launched/escaped weight per wave into the instance's energy
+   tally and flushes its deposit box into the instance's grid.  Lanes keep
+   their RNG streams across items and write them back at the end, so the next
+   launch continues them. */
+template <bool REFLECT>
+__global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR batch_kernel(const MCXKParam* __restrict__ prm,
+        const uint4* __restrict__ items, const uint32_t nqueue, uint32_t* __restrict__ queue,
+        uint32_t* __restrict__ nseed, double* __restrict__ energy) {
+    extern __shared__ float4 lds[];
+    __shared__ uint32_t pool[2];
+    __shared__ uint32_t cur;
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    Rng rng;
+    rng.s0 = (uint64_t)nseed[lane * 4] << 32 | nseed[lane * 4 + 1];
+    rng.s1 = (uint64_t)nseed[lane * 4 + 2] << 32 | nseed[lane * 4 + 3];
+    const uint32_t home = blockIdx.x % nqueue;
+
+    for (;;) {
+        if (threadIdx.x == 0) {
+            uint32_t it = 0xFFFFFFFFu;
+
+            for (uint32_t k = 0; k < nqueue && it == 0xFFFFFFFFu; k++) {
+                uint32_t q = home + k < nqueue ? home + k : home + k - nqueue;
+                uint32_t t = atomicAdd(queue + 2 * q, 1u);
+
+                if (t < queue[2 * q + 1]) {
+                    it = t;
+                }
+            }
+
+            cur = it;
+        }
+
+        __syncthreads();
+        const uint32_t it = __builtin_amdgcn_readfirstlane(cur);
+
+        if (it == 0xFFFFFFFFu) {
+            break;
+        }
+
+        const uint4 item = items[it];
+        const MCXKParam& g = prm[item.x];
+        const uint32_t pplen = g.w0offset + g.srcnum;
+        Shared sh;
+        sh.pool = pool;
+        sh.prop = lds;
+        sh.invcdf = (float*)(lds + g.medianum);
+        sh.box = (BoxCell*)(sh.invcdf + ((g.nphase + 3) & ~3u));
+        sh.pp = (float*)(sh.box + g.box_cells) + threadIdx.x * pplen;
+        sh.repoff = 0;
+
+        for (uint32_t i = threadIdx.x; i < g.medianum; i += blockDim.x) {
+            sh.prop[i] = g.prop[i];
+        }
+
+        for (uint32_t i = threadIdx.x; i < g.nphase; i += blockDim.x) {
+            sh.invcdf[i] = g.invcdf[i];
+        }
+
+        for (uint32_t i = threadIdx.x; i < g.box_cells; i += blockDim.x) {
+            sh.box[i] = (BoxCell)0;
+        }
+
+        for (uint32_t i = 0; i < pplen; i++) {
+            sh.pp[i] = 0.f;
+        }
+
+        if (threadIdx.x == 0) {
+            pool[0] = item.y;
+            pool[1] = item.z;
+        }
+
+        __syncthreads();
+
+        if (g.simple) {
+            transport_lane<false, REFLECT, true>(g, sh, rng, lane);
+        } else {
+            transport_lane<false, REFLECT, false>(g, sh, rng, lane);
+        }
+
+        const uint32_t pd = g.partialdata;
+        double esc = wave_sum((double)sh.pp[pd]);
+        double tot = wave_sum((double)sh.pp[pd + 1]);
+
+        if ((threadIdx.x & 63) == 0 && tot != 0.0) {
+            atomicAdd(energy + 2 * item.x, esc);
+            atomicAdd(energy + 2 * item.x + 1, tot);
+        }
+
+        __syncthreads();
+
+        for (uint32_t i = threadIdx.x; i < g.box_cells; i += blockDim.x) {
+            double val = (double)sh.box[i];
+
+            if (val != 0.0) {
+                uint32_t x = i % g.box_nx, r = i / g.box_nx;
+                uint32_t y = r % g.box_ny;
+                r /= g.box_ny;
+                uint32_t z = r % g.box_nz, gate = r / g.box_nz;
+                uint32_t vox = (g.box_z0 + z) * g.dimxy + (g.box_y0 + y) * g.dimx + (g.box_x0 + x);
+                atomicAdd(g.field + vox + gate * g.nvox, val);
+            }
+        }
+
+        __syncthreads();
+    }
+
+    nseed[lane * 4] = (uint32_t)(rng.s0 >> 32);
+    nseed[lane * 4 + 1] = (uint32_t)rng.s0;
+    nseed[lane * 4 + 2] = (uint32_t)(rng.s1 >> 32);
+    nseed[lane * 4 + 3] = (uint32_t)rng.s1;
 }
 
 }  // namespace mcxk

@@ -214,98 +214,38 @@ static hipError_t dalloc(DevBuf& bufs, T** p, size_t n) {
     return e;
 }
 
-extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalenergy) {
-    (void)fluence;
-    (void)totalenergy;
-    hipError_t err = hipSuccess;
-    int errline = 0;
-    DevBuf bufs;
-    bufs.n = 0;
-    hipStream_t stream = NULL;
-    hipEvent_t ev0 = NULL, ev1 = NULL, ev2 = NULL;
-    uint32_t* hseed = NULL;
-    float* hgenergy = NULL;
-    float* hfield = NULL;
-    uint32_t hdetcount = 0;
-    char msg[512];
-
-    unsigned int activedev = 0;
-    int devlist[MAX_DEVICE];
-    GPUInfo* gpu = NULL;
-    int ndev = mcx_list_gpu(cfg, &activedev, devlist, &gpu);
-
-    if (ndev == 0 || activedev == 0) {
-        free(gpu);
-        MCX_ERROR(-99, "No HIP device found (or the specified GPU does not exist)");
-    }
-
-    const int dev = devlist[0];
-
+/* features the transport kernels do not implement stop here, before any
+   device work, instead of running with different semantics */
+void mcx_check_supported(const Config* cfg) {
     if (cfg->mediabyte > 4) {
-        free(gpu);
         MCX_ERROR(-4, "unsupported media format for the GPU kernel");
     }
 
     if (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->srctype == MCX_SRC_FOURIERX ||
             cfg->srctype == MCX_SRC_FOURIERX2D || cfg->srctype == MCX_SRC_HYPERBOLOID_GAUSSIAN) {
-        free(gpu);
         MCX_ERROR(-4, "this source type is not supported by the MI355X kernel");
     }
 
     if (cfg->minenergy >= cfg->srcpos.w) {
-        free(gpu);
         MCX_ERROR(-4, "minenergy must be below the source weight (no photon could ever launch)");
     }
 
     if (cfg->srctype == MCX_SRC_CONE && !(cfg->srcparam1.x > 0.f)) {
-        free(gpu);
         MCX_ERROR(-4, "cone source needs a positive half angle in srcparam1.x");
     }
 
     if (cfg->nangle || cfg->polmedianum || cfg->srcnum > 1 || cfg->issaveref > 1 || cfg->issavedet == FILL_MAXDETPHOTON ||
             (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY))) {
-        free(gpu);
         MCX_ERROR(-4, "angle-CDF launch, polarization, photon sharing, issaveref>1, fill-max mode and trajectory "
                   "saving are not supported by the MI355X kernel");
     }
+}
 
-    /* thread map: reference autopilot for AMD devices, or the user's nthread/nblocksize */
-    size_t nblocksize = cfg->autopilot ? gpu[dev].autoblock : cfg->nblocksize;
-    size_t nthread = cfg->autopilot ? gpu[dev].autothread : cfg->nthread;
-    const int ncu = gpu[dev].sm;
-
-    if (nblocksize == 0) {
-        free(gpu);
-        MCX_ERROR(-4, "block size must be positive");
-    }
-
-    /* the reference rounds the thread count down to whole blocks; the photon map
-       depends only on the thread count, so the transport kernel itself is launched
-       in workgroups of MCX_LAUNCH_BLOCK lanes with the tail lanes idle */
-    nthread = (nthread / nblocksize) * nblocksize;
-    size_t lblock = MCX_LAUNCH_BLOCK;
-    {
-        const char* env = getenv("MCXHIP_BLOCK");
-
-        if (env) {
-            lblock = (size_t)strtoul(env, NULL, 10);
-        }
-
-        if (lblock < 64 || lblock > MCX_LAUNCH_BLOCK_MAX || (lblock & 63)) {
-            free(gpu);
-            MCX_ERROR(-4, "MCXHIP_BLOCK must be a multiple of 64 within the kernel's launch bounds");
-        }
-    }
-    size_t nlaunch = (nthread + lblock - 1) / lblock;
-
-    if (nthread == 0) {
-        free(gpu);
-        MCX_ERROR(-4, "thread number must be at least one block");
-    }
-
-    free(gpu);
-    gpu = NULL;
-
+/* kernel parameters of one simulation on nthread lanes in workgroups of lblock:
+   geometry, time gates, record layout, the specialised-path flag, the LDS deposit
+   box and the accumulator replicas.  Returns the dynamic LDS bytes per workgroup;
+   *reflect receives the reference's rule for the reflection kernel variant. */
+size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lblock, int* reflect) {
     const unsigned int maxgate = (unsigned int)((cfg->tend - cfg->tstart) / cfg->tstep + 0.5);
     cfg->maxgate = maxgate;
     const size_t nvox = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
@@ -319,7 +259,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         MCX_ERROR(-4, "domain too large for 32-bit voxel indexing");
     }
 
-    MCXKParam g;
+    MCXKParam& g = *gp;
     memset(&g, 0, sizeof(g));
     memcpy(&g.srcpos, &cfg->srcpos, 16);
     memcpy(&g.srcdir, &cfg->srcdir, 16);
@@ -400,20 +340,15 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
                     cfg->outputtype == otL);
     }
 
-    /* photon split across the persistent threads (src/mcx_host.cpp:1101-1102) */
-    g.threadphoton = (uint32_t)(cfg->nphoton / nthread);
-    g.oddphoton = (int32_t)(cfg->nphoton - (size_t)g.threadphoton * nthread);
-
     /* reflection block rule (src/mcx_host.cpp:945-956) */
-    int reflect;
     {
         char bcs[13] = {0}, allabsorb[7] = {bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, 0};
         memcpy(bcs, cfg->bc, 12);
-        reflect = cfg->isreflect || (strcmp(bcs, allabsorb) && bcs[0] != 0);
+        *reflect = cfg->isreflect || (strcmp(bcs, allabsorb) && bcs[0] != 0);
 
         for (int i = 0; i < 6; i++) {
             if (cfg->bc[i] == bcReflect || cfg->bc[i] == bcMirror) {
-                reflect = 1;
+                *reflect = 1;
             }
         }
     }
@@ -511,6 +446,87 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
 
         g.nrep = nr ? (uint32_t)nr : 1u;
     }
+
+    return shmem;
+}
+
+extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalenergy) {
+    (void)fluence;
+    (void)totalenergy;
+    hipError_t err = hipSuccess;
+    int errline = 0;
+    DevBuf bufs;
+    bufs.n = 0;
+    hipStream_t stream = NULL;
+    hipEvent_t ev0 = NULL, ev1 = NULL, ev2 = NULL;
+    uint32_t* hseed = NULL;
+    float* hgenergy = NULL;
+    float* hfield = NULL;
+    uint32_t hdetcount = 0;
+    char msg[512];
+
+    mcx_check_supported(cfg);
+
+    unsigned int activedev = 0;
+    int devlist[MAX_DEVICE];
+    GPUInfo* gpu = NULL;
+    int ndev = mcx_list_gpu(cfg, &activedev, devlist, &gpu);
+
+    if (ndev == 0 || activedev == 0) {
+        free(gpu);
+        MCX_ERROR(-99, "No HIP device found (or the specified GPU does not exist)");
+    }
+
+    const int dev = devlist[0];
+
+    /* thread map: reference autopilot for AMD devices, or the user's nthread/nblocksize */
+    size_t nblocksize = cfg->autopilot ? gpu[dev].autoblock : cfg->nblocksize;
+    size_t nthread = cfg->autopilot ? gpu[dev].autothread : cfg->nthread;
+    const int ncu = gpu[dev].sm;
+
+    if (nblocksize == 0) {
+        free(gpu);
+        MCX_ERROR(-4, "block size must be positive");
+    }
+
+    /* the reference rounds the thread count down to whole blocks; the photon map
+       depends only on the thread count, so the transport kernel itself is launched
+       in workgroups of MCX_LAUNCH_BLOCK lanes with the tail lanes idle */
+    nthread = (nthread / nblocksize) * nblocksize;
+    size_t lblock = MCX_LAUNCH_BLOCK;
+    {
+        const char* env = getenv("MCXHIP_BLOCK");
+
+        if (env) {
+            lblock = (size_t)strtoul(env, NULL, 10);
+        }
+
+        if (lblock < 64 || lblock > MCX_LAUNCH_BLOCK_MAX || (lblock & 63)) {
+            free(gpu);
+            MCX_ERROR(-4, "MCXHIP_BLOCK must be a multiple of 64 within the kernel's launch bounds");
+        }
+    }
+    size_t nlaunch = (nthread + lblock - 1) / lblock;
+
+    if (nthread == 0) {
+        free(gpu);
+        MCX_ERROR(-4, "thread number must be at least one block");
+    }
+
+    free(gpu);
+    gpu = NULL;
+
+    MCXKParam g;
+    int reflect = 0;
+    size_t shmem = mcx_kparam_setup(cfg, &g, nthread, lblock, &reflect);
+    const unsigned int maxgate = g.maxgate;
+    const size_t nvox = g.nvox;
+    const size_t fieldlen = g.fieldlen;
+    const unsigned int reclen = g.reclen;
+
+    /* photon split across the persistent threads (src/mcx_host.cpp:1101-1102) */
+    g.threadphoton = (uint32_t)(cfg->nphoton / nthread);
+    g.oddphoton = (int32_t)(cfg->nphoton - (size_t)g.threadphoton * nthread);
 
     /* fast mode: one resident wave of workgroups, each drawing its share of photons
        one at a time, so lanes stay busy until their workgroup's share is done

@@ -344,6 +344,77 @@ def run(cfg=None, **kwargs):
         return pc.collect()
 
 
+class Batch:
+    """Many simulations in one persistent launch (include/mcx_host.h mcx_batch_*).
+
+    ``Batch(cfgs)`` validates and stages every pmcxcl dict in ``cfgs`` (fast
+    arithmetic, no detected photons, no replay); ``run(nstep)`` runs nstep
+    steps, each simulating every instance afresh; ``collect()`` returns one
+    output dict per instance (``flux``, ``stat``, and ``flux64`` with
+    ``rawfield``) from the last step.  ``blocksize``/``chunk`` fix the launch
+    shape and the photons per work item; 0 lets the library autotune them.
+    ``info`` holds the chosen schedule and the last step's device time."""
+
+    def __init__(self, cfgs, blocksize=0, chunk=0, usegraph=True, nqueue=0):
+        self.pcs = []
+        self.handle = ctypes.c_void_p()
+        try:
+            for cfg in cfgs:
+                pc = prepare(cfg)
+                self.pcs.append(pc)
+                if pc.c.issave2pt:
+                    pc.c.exportfield = _native.cmalloc(4 * pc.fieldlen)
+            n = len(self.pcs)
+            self._arr = (ctypes.POINTER(Config) * n)(*[ctypes.pointer(pc.c) for pc in self.pcs])
+            self.opt = _native.MCXBatchOpt(int(blocksize), int(chunk), int(bool(usegraph)), int(nqueue))
+            rc = lib.mcx_batch_create(self._arr, n, ctypes.byref(self.opt), ctypes.byref(self.handle))
+            if rc:
+                raise RuntimeError("MCX terminated due to an exception: " + _native.last_error())
+        except Exception:
+            self.close()
+            raise
+
+    @property
+    def info(self):
+        o = self.opt
+        return {"blocksize": o.blocksize, "chunk": o.chunk, "nblock": o.nblock, "nitems": o.nitems,
+                "stepms": o.stepms, "kernelms": o.kernelms, "tunems": o.tunems, "usegraph": o.usegraph}
+
+    def run(self, nstep=1):
+        rc = lib.mcx_batch_run(self.handle, int(nstep))
+        if rc:
+            raise RuntimeError("MCX terminated due to an exception: " + _native.last_error())
+        return self.info
+
+    def collect(self):
+        rc = lib.mcx_batch_collect(self.handle)
+        if rc:
+            raise RuntimeError("MCX terminated due to an exception: " + _native.last_error())
+        return [pc.collect() for pc in self.pcs]
+
+    def close(self):
+        if self.handle:
+            lib.mcx_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+        for pc in self.pcs:
+            pc.close()
+        self.pcs = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def run_batch(cfgs, nstep=1, **opts):
+    """Run the simulations of ``cfgs`` in one persistent launch; returns
+    (list of output dicts, schedule info).  See ``Batch``."""
+    with Batch(cfgs, **opts) as b:
+        b.run(nstep)
+        return b.collect(), b.info
+
+
 def _collect(c, user, ngate, ndet, nsrc):
     out = {}
     reclen = lib.mcx_detreclen(ctypes.byref(c))

@@ -150,3 +150,69 @@ def layered_workload(nphoton=100000000, dim=200, seed=1648335518, **over):
     }
     cfg.update(over)
     return cfg
+
+
+def _random_rotation(rng):
+    """a uniformly random 3x3 rotation (QR of a Gaussian matrix with the sign fixed)"""
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def ellipsoid_volume(rng, dim=64):
+    """uint8 [dim]^3 label volume: label 1 with one randomly placed, sized and
+    oriented ellipsoid of label 2 (centre in the middle half of the cube,
+    semi-axes 4-16 voxels), Fortran-indexed [x, y, z]"""
+    centre = rng.uniform(0.3 * dim, 0.7 * dim, 3)
+    axes = rng.uniform(4.0, 16.0, 3)
+    rot = _random_rotation(rng)
+    c = np.arange(dim, dtype=np.float32) + np.float32(0.5)
+    x, y, z = np.meshgrid(c - centre[0], c - centre[1], c - centre[2], indexing="ij")
+    pts = np.stack([x, y, z], -1) @ rot.astype(np.float32)   # body-frame coordinates
+    inside = ((pts / axes.astype(np.float32)) ** 2).sum(-1) <= 1.0
+    vol = np.ones((dim, dim, dim), dtype=np.uint8)
+    vol[inside] = 2
+    return vol
+
+
+def batched_workload(ninst=256, dim=64, seed=20251203, photons=(1e4, 1e6), srctypes=("pencil", "isotropic", "disk")):
+    """Many independent simulations: ``ninst`` [dim]^3 volumes, each with a random
+    ellipsoid inclusion; per instance the background and the inclusion draw
+    mua in U[0.001, 0.05]/mm, mus in U[0.5, 20]/mm, g in U[0.7, 0.95] (n = 1.37,
+    boundary reflection on), the photon count is log-uniform in ``photons`` and
+    the source is a pencil beam or a disk (radius 2-8 voxels) entering the z=0
+    face at its centre, or an isotropic point at a random interior voxel.  One
+    0-5 ns gate, fluence-rate output.  Returns a list of pmcxcl dicts."""
+    rng = np.random.default_rng(seed)
+    cfgs = []
+    for i in range(ninst):
+        vol = ellipsoid_volume(rng, dim)
+        prop = [[0, 0, 1, 1]]
+        for _ in range(2):
+            prop.append([float(rng.uniform(0.001, 0.05)), float(rng.uniform(0.5, 20.0)),
+                         float(rng.uniform(0.7, 0.95)), 1.37])
+        kind = srctypes[int(rng.integers(len(srctypes)))]
+        cfg = {
+            "nphoton": int(round(10 ** rng.uniform(np.log10(photons[0]), np.log10(photons[1])))),
+            "vol": vol,
+            "prop": prop,
+            "tstart": 0.0,
+            "tend": 5e-9,
+            "tstep": 5e-9,
+            "srctype": kind,
+            "srcpos": [dim / 2.0, dim / 2.0, 0.0],
+            "srcdir": [0, 0, 1],
+            "isreflect": 1,
+            "issrcfrom0": 1,
+            "issavedet": 0,
+            "seed": int(seed + 7919 * i),
+            "session": "batch%03d" % i,
+        }
+        if kind == "isotropic":
+            cfg["srcpos"] = [float(v) for v in rng.uniform(0.25 * dim, 0.75 * dim, 3)]
+        elif kind == "disk":
+            cfg["srcparam1"] = [float(rng.uniform(2.0, 8.0)), 0, 0, 0]
+        cfgs.append(cfg)
+    return cfgs
