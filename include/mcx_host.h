@@ -57,16 +57,19 @@ int mcx_device_atomic_ceiling(int gpuid, unsigned int ncell, double* rate);
    Config per call): many independent simulations with their own volumes,
    media, sources and photon counts run in one persistent launch ---- */
 typedef struct MCXBatchOpt {
-    unsigned int blocksize;      /* lanes per workgroup: 64, 128, 192 or 256; 0 picks by autotuning */
-    unsigned int chunk;          /* photons per work item; 0 picks by autotuning */
+    unsigned int blocksize;      /* lanes per workgroup: 64, 128, 192 or 256; 0 picks the shape that keeps
+                                    the most lanes resident */
+    unsigned int noprobe;        /* 1: balance the per-XCD instance lists by photon counts instead of the
+                                    costs a probe step measures */
     unsigned int usegraph;       /* 1: each step replays a captured HIP graph (prepare, transport, finalize) */
-    unsigned int nqueue;         /* work queues, one per XCD; 0 means 8 */
+    unsigned int nqueue;         /* instance lists (list q runs on XCD q % 8); 0 picks 64, or fewer for
+                                    small batches */
     /* filled in by the library */
     unsigned int nblock;         /* resident workgroups of the launch */
-    unsigned int nitems;         /* work items of a step */
+    unsigned int reserved;
     float stepms;                /* device time of the last step (prepare + transport + finalize) */
     float kernelms;              /* device time of the last transport kernel (steps run without a graph) */
-    float tunems;                /* time spent autotuning */
+    float tunems;                /* time spent choosing the schedule */
 } MCXBatchOpt;
 
 typedef struct MCXBatch MCXBatch;

@@ -213,8 +213,8 @@ lib.mcx_cleargpuinfo.restype = None
 
 class MCXBatchOpt(ctypes.Structure):
     """include/mcx_host.h MCXBatchOpt"""
-    _fields_ = [("blocksize", ctypes.c_uint), ("chunk", ctypes.c_uint), ("usegraph", ctypes.c_uint),
-                ("nqueue", ctypes.c_uint), ("nblock", ctypes.c_uint), ("nitems", ctypes.c_uint),
+    _fields_ = [("blocksize", ctypes.c_uint), ("noprobe", ctypes.c_uint), ("usegraph", ctypes.c_uint),
+                ("nqueue", ctypes.c_uint), ("nblock", ctypes.c_uint), ("reserved", ctypes.c_uint),
                 ("stepms", ctypes.c_float), ("kernelms", ctypes.c_float), ("tunems", ctypes.c_float)]
 
 

@@ -6,23 +6,23 @@
  * idle lanes per simulation.  Here n prepared Configs (own volume, media,
  * source, photon count, time gates) are staged once and each step runs them
  * all in one persistent kernel (batch_kernel in mcx_core_impl.h):
- *   - each instance's photons are cut into work items of `chunk` photons;
- *   - the items sit in one queue per XCD, instances assigned to queues by
- *     longest-processing-time first, so an XCD's L2 holds few volumes/grids;
- *   - a workgroup takes an item, stages the instance's tables in LDS, hands
- *     its photons to lanes one at a time, and flushes its deposit box;
- *   - a step is prepare (zero grids and tallies, reset queues) + transport +
- *     finalize (per-instance normalisation on the device), optionally captured
- *     once as a HIP graph and replayed.
- * The launch shape (lanes per workgroup) and the item size are picked by a
- * short autotuning pass over trial steps with scaled-down photon counts unless
- * the caller fixes them.
+ *   - lanes draw each instance's photons one at a time from a counter in
+ *     global memory, so workgroups share an instance without item barriers;
+ *   - instances are split into one list per XCD, balanced by predicted cost
+ *     (longest processing time first) and run heaviest first, so an XCD's L2
+ *     holds few volumes and grids at a time;
+ *   - a step is prepare (zero grids and tallies, restore the counters) +
+ *     transport + finalize (per-instance normalisation on the device),
+ *     optionally captured once as a HIP graph and replayed.
+ * The schedule is tuned once: the launch shape that keeps the most lanes
+ * resident, and per-instance costs measured by a probe step at a scaled-down
+ * photon count (the kernel's wall-clock per instance) for the list balance.
  *
  * Device memory of a batch: per instance its labels (uint32 nvox), media
  * table, phase table, extra sources, float64 grid and float32 output; shared:
- * kernel parameters, normalisation records, work items and queues, one RNG
- * state per lane (continued from step to step) and two energy tallies per
- * instance.
+ * kernel parameters, normalisation records, the instance lists and counters,
+ * one RNG state per lane (continued from step to step) and two energy tallies
+ * per instance.
  */
 #include <math.h>
 #include <setjmp.h>
@@ -43,7 +43,8 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
 
 #define MCX_BATCH_MAXBUF 16
 #define MCX_BATCH_MAXBLOCK 256
-#define MCX_BATCH_TUNE_PHOTONS 4.0e6
+#define MCX_BATCH_TUNE_PHOTONS 1.0e6
+#define MCX_BATCH_TUNE_MS 150.0
 
 struct MCXBatch {
     int dev;
@@ -65,22 +66,23 @@ struct MCXBatch {
     /* shared device buffers */
     MCXKParam* dprm;
     MCXBatchNorm* dnorm;
-    uint4* ditems;
-    uint32_t* dqueue;
-    uint32_t* dqueue0;
     uint32_t* dnseed;
     double* denergy;
     double* dfield;
     float* dout;
-    size_t nfield, maxlen, maxitems, maxlanes;
+    size_t nfield, maxlen, maxlanes;
     /* schedule */
-    uint4* items;
-    uint32_t* queue0;
-    uint32_t* qof;                  /* queue of each instance */
-    uint32_t* order;                /* instances by decreasing photon count */
-    uint32_t nitems, nqueue;
+    uint32_t* qlist;                /* instances of list 0, then list 1, ... */
+    uint32_t* state0;               /* initial {head, end} per list, {0, photons} per instance */
+    uint32_t* qof;                  /* list of each instance */
+    uint32_t* order;                /* instances by decreasing predicted cost */
+    double* weight;                 /* predicted cost of each instance */
+    uint32_t* dqlist;
+    uint32_t* dstate;
+    uint32_t* dstate0;
+    unsigned long long* dcost;      /* per-instance workgroup time of a probe step */
+    uint32_t nqueue, nstate;
     size_t lblock, nblock, shmem;
-    unsigned int chunk;
     int reflect;
     hipGraph_t graph;
     hipGraphExec_t exec;
@@ -157,10 +159,11 @@ static void batch_free(MCXBatch* b) {
     free(b->dprop);
     free(b->dinvcdf);
     free(b->dsrcdata);
-    free(b->items);
-    free(b->queue0);
+    free(b->qlist);
+    free(b->state0);
     free(b->qof);
     free(b->order);
+    free(b->weight);
     free(b->cfg);
     free(b);
 }
@@ -192,49 +195,40 @@ static size_t batch_params(MCXBatch* b, size_t lblock) {
     return shmem;
 }
 
-/* work items of `chunk` photons (photon counts scaled by frac) in per-queue
-   order; instances were assigned to queues by batch_assign */
-static void batch_items(MCXBatch* b, unsigned int chunk, double frac, size_t* nphoton) {
+/* the initial schedule state: per list {head, end} into qlist, per instance
+   {0, photons}, photon counts scaled by frac (at least one photon) */
+static size_t batch_state(MCXBatch* b, double frac) {
     uint32_t k = 0;
     size_t total = 0;
+    uint32_t* pool = b->state0 + 2 * b->nqueue;
 
     for (uint32_t q = 0; q < b->nqueue; q++) {
-        b->queue0[2 * q] = k;
+        b->state0[2 * q] = k;
 
         for (unsigned int r = 0; r < b->n; r++) {
             const unsigned int i = b->order[r];
 
-            if (b->qof[i] != q) {
-                continue;
-            }
-
-            size_t np = (size_t)(b->cfg[i]->nphoton * frac + 0.5);
-            np = (np == 0 && b->cfg[i]->nphoton) ? 1 : np;
-            total += np;
-
-            for (size_t s = 0; s < np; s += chunk) {
-                if (k >= b->maxitems) {
-                    MCX_ERROR(-4, "work item table overflow");
-                }
-
-                size_t e = s + chunk < np ? s + chunk : np;
-                b->items[k].x = i;
-                b->items[k].y = (uint32_t)s;
-                b->items[k].z = (uint32_t)e;
-                b->items[k].w = 0;
-                k++;
+            if (b->qof[i] == q) {
+                b->qlist[k++] = i;
             }
         }
 
-        b->queue0[2 * q + 1] = k;
+        b->state0[2 * q + 1] = k;
     }
 
-    b->nitems = k;
-    *nphoton = total;
+    for (unsigned int i = 0; i < b->n; i++) {
+        size_t np = (size_t)(b->cfg[i]->nphoton * frac + 0.5);
+        np = (np == 0 && b->cfg[i]->nphoton) ? 1 : np;
+        pool[2 * i] = 0;
+        pool[2 * i + 1] = (uint32_t)np;
+        total += np;
+    }
+
+    return total;
 }
 
-/* instances to queues, longest first onto the least-loaded queue; a queue
-   runs its instances longest first */
+/* instances to lists by predicted cost (b->weight), heaviest first onto the
+   least-loaded list; a list runs its instances heaviest first */
 static void batch_assign(MCXBatch* b) {
     uint32_t* order = b->order;
     double* load = (double*)host_calloc(b->nqueue, sizeof(double));
@@ -243,11 +237,11 @@ static void batch_assign(MCXBatch* b) {
         order[i] = i;
     }
 
-    for (unsigned int i = 1; i < b->n; i++) {   /* insertion sort, decreasing photon count */
+    for (unsigned int i = 1; i < b->n; i++) {   /* insertion sort, decreasing weight */
         uint32_t v = order[i];
         unsigned int j = i;
 
-        while (j > 0 && b->cfg[order[j - 1]]->nphoton < b->cfg[v]->nphoton) {
+        while (j > 0 && b->weight[order[j - 1]] < b->weight[v]) {
             order[j] = order[j - 1];
             j--;
         }
@@ -264,7 +258,7 @@ static void batch_assign(MCXBatch* b) {
         }
 
         b->qof[i] = best;
-        load[best] += (double)b->cfg[i]->nphoton;
+        load[best] += b->weight[i];
     }
 
     free(load);
@@ -272,24 +266,27 @@ static void batch_assign(MCXBatch* b) {
 
 static void batch_upload_schedule(MCXBatch* b) {
     BHIP(hipMemcpyAsync(b->dprm, b->prm, b->n * sizeof(MCXKParam), hipMemcpyHostToDevice, b->stream));
-    BHIP(hipMemcpyAsync(b->ditems, b->items, (size_t)b->nitems * sizeof(uint4), hipMemcpyHostToDevice, b->stream));
-    BHIP(hipMemcpyAsync(b->dqueue0, b->queue0, 2 * (size_t)b->nqueue * sizeof(uint32_t), hipMemcpyHostToDevice,
-                        b->stream));
+    BHIP(hipMemcpyAsync(b->dqlist, b->qlist, b->n * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    BHIP(hipMemcpyAsync(b->dstate0, b->state0, b->nstate * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
     BHIP(hipStreamSynchronize(b->stream));
 }
 
 /* one step on the stream: prepare, transport, finalize; ev1/ev2 bracket the
-   transport kernel when timed */
-static void batch_enqueue(MCXBatch* b, int timed) {
-    BHIP(mcx_launch_batch_prep(b->dfield, b->nfield, b->denergy, 2 * b->n, b->dqueue, b->dqueue0, b->nqueue,
+   transport kernel when timed; probe also records each instance's cost */
+static void batch_enqueue(MCXBatch* b, int timed, int probe) {
+    BHIP(mcx_launch_batch_prep(b->dfield, b->nfield, b->denergy, 2 * b->n, b->dstate, b->dstate0, b->nstate,
                                b->stream));
+
+    if (probe) {
+        BHIP(hipMemsetAsync(b->dcost, 0, b->n * sizeof(unsigned long long), b->stream));
+    }
 
     if (timed) {
         BHIP(hipEventRecord(b->ev1, b->stream));
     }
 
-    BHIP(mcx_launch_batch(b->dprm, b->ditems, b->nqueue, b->dqueue, b->dnseed, b->denergy, (int)b->nblock,
-                          (int)b->lblock, b->shmem, b->reflect, b->stream));
+    BHIP(mcx_launch_batch(b->dprm, b->dqlist, b->nqueue, b->dstate, b->dnseed, b->denergy, probe ? b->dcost : NULL,
+                          (int)b->nblock, (int)b->lblock, b->shmem, b->reflect, b->stream));
 
     if (timed) {
         BHIP(hipEventRecord(b->ev2, b->stream));
@@ -384,7 +381,12 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
     b->dsrcdata = (float4**)host_calloc(n, sizeof(float4*));
     b->qof = (uint32_t*)host_calloc(n, sizeof(uint32_t));
     b->order = (uint32_t*)host_calloc(n, sizeof(uint32_t));
-    b->nqueue = opt->nqueue ? opt->nqueue : 8;
+    b->weight = (double*)host_calloc(n, sizeof(double));
+    /* lists: a multiple of the 8 XCDs, so list q always runs on XCD q % 8; 64
+       lists (16 workgroups each at full occupancy) measured best on 256 64^3
+       instances, against 2.5x slower with one list and 10% slower with one
+       list per instance */
+    b->nqueue = opt->nqueue ? opt->nqueue : (n >= 64 ? 64 : ((n + 7) / 8) * 8);
 
     /* sizes: grids and per-instance input tables, each kind in one allocation */
     size_t nvoxall = 0, npropall = 0, nphaseall = 0, nsrcall = 0, maxlen = 0;
@@ -446,24 +448,17 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
         nm.normalize = cfg->isnormalized ? 1u : 0u;
     }
 
-    /* schedule tables sized for the smallest item the tuner may pick */
-    const unsigned int minchunk = opt->chunk ? opt->chunk : 64u * 4u;
-    size_t maxitems = 0;
-
-    for (unsigned int i = 0; i < n; i++) {
-        maxitems += (b->cfg[i]->nphoton + minchunk - 1) / minchunk + 1;
-    }
-
-    b->maxitems = maxitems;
-    b->items = (uint4*)host_calloc(maxitems, sizeof(uint4));
-    b->queue0 = (uint32_t*)host_calloc(2 * b->nqueue, sizeof(uint32_t));
+    b->qlist = (uint32_t*)host_calloc(n, sizeof(uint32_t));
+    b->nstate = 2 * b->nqueue + 2 * n;
+    b->state0 = (uint32_t*)host_calloc(b->nstate, sizeof(uint32_t));
     b->maxlanes = (size_t)ncu * 2048;   /* 32 waves of 64 lanes per CU at most */
     b->dprm = batch_alloc<MCXKParam>(b, n);
     b->dnorm = batch_alloc<MCXBatchNorm>(b, n);
-    b->ditems = batch_alloc<uint4>(b, maxitems);
-    b->dqueue = batch_alloc<uint32_t>(b, 2 * b->nqueue);
-    b->dqueue0 = batch_alloc<uint32_t>(b, 2 * b->nqueue);
+    b->dqlist = batch_alloc<uint32_t>(b, n);
+    b->dstate = batch_alloc<uint32_t>(b, b->nstate);
+    b->dstate0 = batch_alloc<uint32_t>(b, b->nstate);
     b->denergy = batch_alloc<double>(b, 2 * (size_t)n);
+    b->dcost = batch_alloc<unsigned long long>(b, n);
     b->dnseed = batch_alloc<uint32_t>(b, b->maxlanes * 4);
     BHIP(hipMemcpyAsync(b->dnorm, b->norm, n * sizeof(MCXBatchNorm), hipMemcpyHostToDevice, b->stream));
 
@@ -477,86 +472,95 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
         BHIP(e);
     }
 
-    batch_assign(b);
-
-    /* autotuning: trial steps at a scaled-down photon count over the candidate
-       launch shapes and item sizes; the best photons/ms wins */
-    const size_t blocks[3] = {64, 128, 256};
-    const unsigned int perlane[3] = {4, 16, 64};
+    /* Schedule.  The launch shape is the one that keeps the most lanes resident
+       (occupancy from the kernel's registers and the LDS its tables need; ties
+       go to the larger workgroup) unless the caller fixes it.  Photon costs
+       differ by orders of magnitude between instances (scattering events per
+       photon scale with mus/mua), so the per-XCD lists are balanced by costs
+       measured in a probe step at a scaled-down photon count (the kernel sums
+       each instance's workgroup wall-clock time), unless opt->noprobe is set
+       (the lists are then balanced by photon counts). */
+    const size_t blocks[3] = {256, 128, 64};
     size_t total = 0;
 
     for (unsigned int i = 0; i < n; i++) {
         total += b->cfg[i]->nphoton;
+        b->weight[i] = (double)b->cfg[i]->nphoton;
     }
 
-    size_t bestblock = opt->blocksize;
-    unsigned int bestchunk = opt->chunk;
     const double t0 = wall_ms_batch();
+    size_t bestblock = opt->blocksize;
 
-    if (!bestblock || !bestchunk) {
-        double frac = total > MCX_BATCH_TUNE_PHOTONS ? MCX_BATCH_TUNE_PHOTONS / (double)total : 1.0;
-        double bestrate = -1.0;
+    if (!bestblock) {
+        size_t bestlanes = 0;
 
         for (int bi = 0; bi < 3; bi++) {
-            size_t lb = opt->blocksize ? opt->blocksize : blocks[bi];
+            if (batch_shape(b, blocks[bi], ncu) && b->nblock * b->lblock > bestlanes) {
+                bestlanes = b->nblock * b->lblock;
+                bestblock = blocks[bi];
+            }
+        }
+    }
 
-            if ((opt->blocksize && bi > 0) || !batch_shape(b, lb, ncu)) {
+    if (!bestblock || !batch_shape(b, bestblock, ncu)) {
+        MCX_ERROR(-4, "no launch shape of the batched kernel fits these instances");
+    }
+
+    batch_assign(b);
+
+    if (!opt->noprobe) {
+        unsigned long long* hcost = (unsigned long long*)host_calloc(n, sizeof(unsigned long long));
+        double frac = total > MCX_BATCH_TUNE_PHOTONS ? MCX_BATCH_TUNE_PHOTONS / (double)total : 1.0;
+
+        for (int pass = 0; pass < 2; pass++) {
+            size_t np = batch_state(b, frac);
+            batch_upload_schedule(b);
+            batch_enqueue(b, 1, 1);
+            float ms = 0.f;
+            BHIP(hipEventSynchronize(b->ev2));
+            BHIP(hipEventElapsedTime(&ms, b->ev1, b->ev2));
+
+            /* a short first probe sizes the photon fraction so the next takes ~MCX_BATCH_TUNE_MS */
+            if (pass == 0 && frac < 1.0 && ms > 0.f && ms < 0.5 * MCX_BATCH_TUNE_MS) {
+                double f = frac * MCX_BATCH_TUNE_MS / ms;
+                frac = f < 1.0 ? f : 1.0;
                 continue;
             }
 
-            for (int ci = 0; ci < 3; ci++) {
-                unsigned int ch = opt->chunk ? opt->chunk : perlane[ci] * (unsigned int)lb;
+            BHIP(hipMemcpy(hcost, b->dcost, n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+            const uint32_t* pool = b->state0 + 2 * b->nqueue;
 
-                if ((opt->chunk && ci > 0) || ch < minchunk) {
-                    continue;
-                }
-
-                size_t np = 0;
-                batch_items(b, ch, frac, &np);
-                batch_upload_schedule(b);
-                batch_enqueue(b, 1);
-                float ms = 0.f;
-                BHIP(hipEventSynchronize(b->ev2));
-                BHIP(hipEventElapsedTime(&ms, b->ev1, b->ev2));
-                double rate = ms > 0.f ? np / (double)ms : 0.0;
-
-                if (rate > bestrate) {
-                    bestrate = rate;
-                    bestblock = lb;
-                    bestchunk = ch;
-                }
+            for (unsigned int i = 0; i < n; i++) {
+                /* ticks per photon times the full photon count */
+                b->weight[i] = pool[2 * i + 1] ? (double)hcost[i] / pool[2 * i + 1] * b->cfg[i]->nphoton : 0.0;
             }
+
+            if (getenv("MCXHIP_VERBOSE")) {
+                fprintf(stderr, "mcxhip batch tuning: cost probe over %zu photons in %.1f ms\n", np, ms);
+            }
+
+            break;
         }
 
-        if (bestrate < 0.0) {
-            MCX_ERROR(-4, "no launch shape of the batched kernel fits these instances");
-        }
+        free(hcost);
+        batch_assign(b);
     }
 
     opt->tunems = (float)(wall_ms_batch() - t0);
-
-    if (!batch_shape(b, bestblock, ncu)) {
-        MCX_ERROR(-4, "the requested launch shape of the batched kernel does not fit");
-    }
-
-    size_t np = 0;
-    batch_items(b, bestchunk, 1.0, &np);
+    batch_state(b, 1.0);
     batch_upload_schedule(b);
-    b->chunk = bestchunk;
     opt->blocksize = (unsigned int)b->lblock;
-    opt->chunk = bestchunk;
     opt->nblock = (unsigned int)b->nblock;
-    opt->nitems = b->nitems;
+    opt->nqueue = b->nqueue;
 
     if (getenv("MCXHIP_VERBOSE")) {
-        fprintf(stderr, "mcxhip batch: %u instances, %u items of %u photons, %zu workgroups of %zu lanes, LDS %zu B, "
-                "%u queues, tuning %.1f ms\n", n, b->nitems, bestchunk, b->nblock, b->lblock, b->shmem, b->nqueue,
-                opt->tunems);
+        fprintf(stderr, "mcxhip batch: %u instances in %u lists, %zu workgroups of %zu lanes, LDS %zu B, tuning "
+                "%.1f ms\n", n, b->nqueue, b->nblock, b->lblock, b->shmem, opt->tunems);
     }
 
     if (opt->usegraph) {
         BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        batch_enqueue(b, 0);
+        batch_enqueue(b, 0, 0);
         BHIP(hipStreamEndCapture(b->stream, &b->graph));
         BHIP(hipGraphInstantiate(&b->exec, b->graph, NULL, NULL, 0));
     }
@@ -573,7 +577,7 @@ static void batch_run_impl(MCXBatch* b, unsigned int nstep) {
         if (b->exec) {
             BHIP(hipGraphLaunch(b->exec, b->stream));
         } else {
-            batch_enqueue(b, last);
+            batch_enqueue(b, last, 0);
         }
 
         if (last) {

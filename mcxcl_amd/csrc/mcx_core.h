@@ -76,12 +76,12 @@ int mcx_fast_occupancy(int nthreadblock, size_t shmem, int savedet, int reflect,
 hipError_t mcx_launch_atomic_probe(double* grid, uint32_t n, uint32_t nblock, uint32_t k, hipStream_t s);
 hipError_t mcx_launch_rngdump(const MCXKParam* prm, int nblock, int nthreadblock, hipStream_t stream);
 /* batched launch: many simulations in one persistent kernel (fast arithmetic) */
-hipError_t mcx_launch_batch(const MCXKParam* prm, const uint4* items, uint32_t nqueue, uint32_t* queue,
-                            uint32_t* nseed, double* energy, int nblock, int nthreadblock, size_t shmem, int reflect,
-                            hipStream_t stream);
+hipError_t mcx_launch_batch(const MCXKParam* prm, const uint32_t* qlist, uint32_t nqueue, uint32_t* state,
+                            uint32_t* nseed, double* energy, unsigned long long* cost, int nblock, int nthreadblock,
+                            size_t shmem, int reflect, hipStream_t stream);
 int mcx_batch_occupancy(int nthreadblock, size_t shmem, int reflect);
-hipError_t mcx_launch_batch_prep(double* field, size_t nfield, double* energy, uint32_t nenergy, uint32_t* queue,
-                                 const uint32_t* queue0, uint32_t nqueue, hipStream_t stream);
+hipError_t mcx_launch_batch_prep(double* field, size_t nfield, double* energy, uint32_t nenergy, uint32_t* state,
+                                 const uint32_t* state0, uint32_t nstate, hipStream_t stream);
 hipError_t mcx_launch_batch_finalize(const MCXBatchNorm* norm, const double* energy, uint32_t ninst, size_t maxlen,
                                      hipStream_t stream);
 /* sum the nrep replicas into in[0..n), then out[i] = (float)in[i] * scale */

@@ -63,15 +63,15 @@ extern "C" int mcx_fast_occupancy(int nthreadblock, size_t shmem, int savedet, i
 }
 
 /* ---------------- batched launch ---------------- */
-extern "C" hipError_t mcx_launch_batch(const MCXKParam* prm, const uint4* items, uint32_t nqueue, uint32_t* queue,
-                                       uint32_t* nseed, double* energy, int nblock, int nthreadblock, size_t shmem,
-                                       int reflect, hipStream_t stream) {
+extern "C" hipError_t mcx_launch_batch(const MCXKParam* prm, const uint32_t* qlist, uint32_t nqueue, uint32_t* state,
+                                       uint32_t* nseed, double* energy, unsigned long long* cost, int nblock,
+                                       int nthreadblock, size_t shmem, int reflect, hipStream_t stream) {
     if (reflect) {
-        hipLaunchKernelGGL(mcxk::batch_kernel<true>, dim3(nblock), dim3(nthreadblock), shmem, stream, prm, items,
-                           nqueue, queue, nseed, energy);
+        hipLaunchKernelGGL(mcxk::batch_kernel<true>, dim3(nblock), dim3(nthreadblock), shmem, stream, prm, qlist,
+                           nqueue, state, nseed, energy, cost);
     } else {
-        hipLaunchKernelGGL(mcxk::batch_kernel<false>, dim3(nblock), dim3(nthreadblock), shmem, stream, prm, items,
-                           nqueue, queue, nseed, energy);
+        hipLaunchKernelGGL(mcxk::batch_kernel<false>, dim3(nblock), dim3(nthreadblock), shmem, stream, prm, qlist,
+                           nqueue, state, nseed, energy, cost);
     }
 
     return hipGetLastError();
@@ -85,12 +85,13 @@ extern "C" int mcx_batch_occupancy(int nthreadblock, size_t shmem, int reflect) 
     return e == hipSuccess ? nb : 0;
 }
 
-/* start of a batched step: zero the grids and energy tallies and reset the
-   work queues (a kernel, not hipMemsetAsync, so the step can be captured in a
-   HIP graph and replayed) */
+/* start of a batched step: zero the grids and energy tallies and restore the
+   schedule state (list heads and photon counters) from its initial copy; a
+   kernel, not hipMemsetAsync, so the step can be captured in a HIP graph and
+   replayed */
 __global__ void batch_prep_kernel(double* __restrict__ field, size_t nfield, double* __restrict__ energy,
-                                  uint32_t nenergy, uint32_t* __restrict__ queue, const uint32_t* __restrict__ queue0,
-                                  uint32_t nqueue) {
+                                  uint32_t nenergy, uint32_t* __restrict__ state, const uint32_t* __restrict__ state0,
+                                  uint32_t nstate) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
 
@@ -98,23 +99,23 @@ __global__ void batch_prep_kernel(double* __restrict__ field, size_t nfield, dou
         field[i] = 0.0;
     }
 
-    if (i0 < nenergy) {
-        energy[i0] = 0.0;
+    for (size_t i = i0; i < nenergy; i += stride) {
+        energy[i] = 0.0;
     }
 
-    if (i0 < 2 * (size_t)nqueue) {
-        queue[i0] = queue0[i0];
+    for (size_t i = i0; i < nstate; i += stride) {
+        state[i] = state0[i];
     }
 }
 
 extern "C" hipError_t mcx_launch_batch_prep(double* field, size_t nfield, double* energy, uint32_t nenergy,
-        uint32_t* queue, const uint32_t* queue0, uint32_t nqueue, hipStream_t stream) {
+        uint32_t* state, const uint32_t* state0, uint32_t nstate, hipStream_t stream) {
     size_t need = nfield > nenergy ? nfield : nenergy;
-    need = need > 2 * (size_t)nqueue ? need : 2 * (size_t)nqueue;
+    need = need > nstate ? need : nstate;
     size_t nb = (need + 255) / 256;
     nb = nb < 4096 ? (nb ? nb : 1) : 4096;
     hipLaunchKernelGGL(batch_prep_kernel, dim3((unsigned int)nb), dim3(256), 0, stream, field, nfield, energy,
-                       nenergy, queue, queue0, nqueue);
+                       nenergy, state, state0, nstate);
     return hipGetLastError();
 }
 

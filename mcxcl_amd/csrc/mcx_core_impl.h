@@ -291,7 +291,8 @@ struct Shared {
     BoxCell* box;                /* privatised deposit box (box_cells) */
     float* pp;                   /* this lane's scratch: partial data, energies, w0, srcid */
     size_t repoff;               /* offset of this lane's global accumulator replica */
-    uint32_t* pool;              /* dynamic schedule: {next photon, end} of this workgroup */
+    uint32_t* pool;              /* dynamic schedule: {next photon, end} of this workgroup (LDS)
+                                    or of this instance (global memory, batched launch) */
 };
 
 /* Per-lane write-combining buffer for grid deposits: the two most recent grid
@@ -1311,23 +1312,32 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 /* Batched transport: many simulations in one persistent launch.
-   Work items {instance, first photon, end photon} are grouped in nqueue queues,
-   one per XCD: workgroup b starts at queue b % nqueue, which holds the
-   instances of the XCD it is dispatched to, so one XCD's L2 serves few
-   volumes and grids at a time, and steals from the other queues once its own
-   is empty.  For each item a workgroup stages the instance's tables in LDS,
-   hands the item's photons to its lanes one at a time from an LDS pool,
-   reduces the launched/escaped weight per wave into the instance's energy
-   tally and flushes its deposit box into the instance's grid.  Lanes keep
-   their RNG streams across items and write them back at the end, so the next
-   launch continues them. */
+   Each instance has a photon counter {next, end} in global memory; lanes draw
+   its photons one at a time with an atomic add, so any number of workgroups
+   can share an instance and none waits for another at item boundaries.  The
+   instances are split into nqueue lists, one per XCD: workgroup b starts at
+   list b % nqueue (the XCD it is dispatched to), works on the list's first
+   instance with photons left, and moves on when that one is drained; once its
+   own list is drained it helps the others.  So an XCD's L2 holds few volumes
+   and grids at a time.  On each instance a workgroup stages the tables in LDS,
+   runs its lanes until the counter is exhausted, reduces the launched/escaped
+   weight per wave into the instance's energy tally and flushes its deposit
+   box.  Lanes keep their RNG streams across instances and write them back at
+   the end, so the next launch continues them.  With cost != NULL the
+   workgroup's wall-clock time per instance is summed per instance (the
+   scheduler's cost probe).
+
+   state: 2*nqueue words {head, end} of the lists in qlist, then 2 words
+   {next photon, end} per instance. */
 template <bool REFLECT>
-__global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR batch_kernel(const MCXKParam* __restrict__ prm,
-        const uint4* __restrict__ items, const uint32_t nqueue, uint32_t* __restrict__ queue,
-        uint32_t* __restrict__ nseed, double* __restrict__ energy) {
+/* four waves per SIMD: at most 128 VGPRs (the general transport path alone
+   would take 140 and three waves) */
+__global__ void __launch_bounds__(MCX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) batch_kernel(const MCXKParam* __restrict__ prm,
+        const uint32_t* __restrict__ qlist, const uint32_t nqueue, uint32_t* __restrict__ state,
+        uint32_t* __restrict__ nseed, double* __restrict__ energy, unsigned long long* __restrict__ cost) {
     extern __shared__ float4 lds[];
-    __shared__ uint32_t pool[2];
     __shared__ uint32_t cur;
+    uint32_t* const pools = state + 2 * nqueue;
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     Rng rng;
     rng.s0 = (uint64_t)nseed[lane * 4] << 32 | nseed[lane * 4 + 1];
@@ -1336,32 +1346,47 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR batch_kernel(const MC
 
     for (;;) {
         if (threadIdx.x == 0) {
-            uint32_t it = 0xFFFFFFFFu;
+            uint32_t found = 0xFFFFFFFFu;
 
-            for (uint32_t k = 0; k < nqueue && it == 0xFFFFFFFFu; k++) {
+            /* every pass either finds an instance with photons left or advances a
+               list head, so the search ends */
+            for (uint32_t k = 0; k < nqueue && found == 0xFFFFFFFFu; k++) {
                 uint32_t q = home + k < nqueue ? home + k : home + k - nqueue;
-                uint32_t t = atomicAdd(queue + 2 * q, 1u);
 
-                if (t < queue[2 * q + 1]) {
-                    it = t;
+                for (;;) {
+                    uint32_t h = __hip_atomic_load(state + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+                    if (h >= state[2 * q + 1]) {
+                        break;
+                    }
+
+                    uint32_t inst = qlist[h];
+
+                    if (__hip_atomic_load(pools + 2 * inst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                            pools[2 * inst + 1]) {
+                        found = inst;
+                        break;
+                    }
+
+                    atomicCAS(state + 2 * q, h, h + 1);
                 }
             }
 
-            cur = it;
+            cur = found;
         }
 
         __syncthreads();
-        const uint32_t it = __builtin_amdgcn_readfirstlane(cur);
+        const uint32_t inst = __builtin_amdgcn_readfirstlane(cur);
 
-        if (it == 0xFFFFFFFFu) {
+        if (inst == 0xFFFFFFFFu) {
             break;
         }
 
-        const uint4 item = items[it];
-        const MCXKParam& g = prm[item.x];
+        const MCXKParam& g = prm[inst];
+        const long long tstart = cost ? wall_clock64() : 0;
         const uint32_t pplen = g.w0offset + g.srcnum;
         Shared sh;
-        sh.pool = pool;
+        sh.pool = pools + 2 * inst;
         sh.prop = lds;
         sh.invcdf = (float*)(lds + g.medianum);
         sh.box = (BoxCell*)(sh.invcdf + ((g.nphase + 3) & ~3u));
@@ -1384,26 +1409,19 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR batch_kernel(const MC
             sh.pp[i] = 0.f;
         }
 
-        if (threadIdx.x == 0) {
-            pool[0] = item.y;
-            pool[1] = item.z;
-        }
-
         __syncthreads();
 
-        if (g.simple) {
-            transport_lane<false, REFLECT, true>(g, sh, rng, lane);
-        } else {
-            transport_lane<false, REFLECT, false>(g, sh, rng, lane);
-        }
+        /* the general path only: with the pencil-beam special path inlined beside
+           it the kernel needs more registers and runs fewer waves per SIMD */
+        transport_lane<false, REFLECT, false>(g, sh, rng, lane);
 
         const uint32_t pd = g.partialdata;
         double esc = wave_sum((double)sh.pp[pd]);
         double tot = wave_sum((double)sh.pp[pd + 1]);
 
         if ((threadIdx.x & 63) == 0 && tot != 0.0) {
-            atomicAdd(energy + 2 * item.x, esc);
-            atomicAdd(energy + 2 * item.x + 1, tot);
+            atomicAdd(energy + 2 * inst, esc);
+            atomicAdd(energy + 2 * inst + 1, tot);
         }
 
         __syncthreads();
@@ -1422,6 +1440,10 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR batch_kernel(const MC
         }
 
         __syncthreads();
+
+        if (cost && threadIdx.x == 0) {
+            atomicAdd(cost + inst, (unsigned long long)(wall_clock64() - tstart));
+        }
     }
 
     nseed[lane * 4] = (uint32_t)(rng.s0 >> 32);

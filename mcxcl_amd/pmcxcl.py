@@ -351,11 +351,13 @@ class Batch:
     arithmetic, no detected photons, no replay); ``run(nstep)`` runs nstep
     steps, each simulating every instance afresh; ``collect()`` returns one
     output dict per instance (``flux``, ``stat``, and ``flux64`` with
-    ``rawfield``) from the last step.  ``blocksize``/``chunk`` fix the launch
-    shape and the photons per work item; 0 lets the library autotune them.
-    ``info`` holds the chosen schedule and the last step's device time."""
+    ``rawfield``) from the last step.  ``blocksize`` fixes the lanes per
+    workgroup (0: the shape that keeps the most lanes resident); ``probe``
+    balances the per-XCD instance lists by costs measured in a probe step
+    (else by photon counts).  ``info`` holds the chosen schedule and the last
+    step's device time."""
 
-    def __init__(self, cfgs, blocksize=0, chunk=0, usegraph=True, nqueue=0):
+    def __init__(self, cfgs, blocksize=0, probe=True, usegraph=True, nqueue=0):
         self.pcs = []
         self.handle = ctypes.c_void_p()
         try:
@@ -366,7 +368,7 @@ class Batch:
                     pc.c.exportfield = _native.cmalloc(4 * pc.fieldlen)
             n = len(self.pcs)
             self._arr = (ctypes.POINTER(Config) * n)(*[ctypes.pointer(pc.c) for pc in self.pcs])
-            self.opt = _native.MCXBatchOpt(int(blocksize), int(chunk), int(bool(usegraph)), int(nqueue))
+            self.opt = _native.MCXBatchOpt(int(blocksize), int(not probe), int(bool(usegraph)), int(nqueue))
             rc = lib.mcx_batch_create(self._arr, n, ctypes.byref(self.opt), ctypes.byref(self.handle))
             if rc:
                 raise RuntimeError("MCX terminated due to an exception: " + _native.last_error())
@@ -377,7 +379,7 @@ class Batch:
     @property
     def info(self):
         o = self.opt
-        return {"blocksize": o.blocksize, "chunk": o.chunk, "nblock": o.nblock, "nitems": o.nitems,
+        return {"blocksize": o.blocksize, "probe": not o.noprobe, "nblock": o.nblock, "nqueue": o.nqueue,
                 "stepms": o.stepms, "kernelms": o.kernelms, "tunems": o.tunems, "usegraph": o.usegraph}
 
     def run(self, nstep=1):

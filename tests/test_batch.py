@@ -72,15 +72,15 @@ def test_batch_matches_single_runs():
     cfgs = _small_batch()
     outs, info = pmcxcl.run_batch(cfgs)
     print("schedule", info, flush=True)
-    assert info["nitems"] >= len(cfgs) and info["nblock"] > 0
+    assert info["nblock"] > 0 and info["probe"]
     _check_against_single(outs, cfgs)
 
 
 @pytest.mark.gpu
 def test_batch_fixed_schedule_without_graph():
     cfgs = _small_batch(n=6, seed=11)
-    outs, info = pmcxcl.run_batch(cfgs, blocksize=128, chunk=2048, usegraph=False)
-    assert info["blocksize"] == 128 and info["chunk"] == 2048 and info["kernelms"] > 0
+    outs, info = pmcxcl.run_batch(cfgs, blocksize=128, probe=False, usegraph=False)
+    assert info["blocksize"] == 128 and not info["probe"] and info["kernelms"] > 0
     _check_against_single(outs, cfgs)
 
 
