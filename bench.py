@@ -19,6 +19,8 @@ The JSON line also carries:
                 timed on this host on a bounded sample of the same workload
   adjoint       replay-adjoint gradient of detector signals on the 200^3
                 four-layer volume against central finite differences
+  inverse       16 sources x 16 detectors on 128^3 per-voxel media: forward,
+                folded backward, all 256 signals' gradients, finite-difference check
   batch         256 independent 64^3 simulations in one persistent launch,
                 against the same instances run one call each
 
@@ -170,6 +172,65 @@ def adjoint_leg(photons, nfd, seed=0):
         os.environ.pop("MCXHIP_DYNAMIC", None)
 
 
+def inverse_leg(photons_per_source, nfd, seed=0):
+    """The multi-source differentiable step: 128^3 per-voxel mua/mus (two layers
+    with smooth +-30% perturbations), 16 sources x 16 detectors in one launch.
+    Times the forward run, the folded backward of a scalar loss over the 256
+    signals, and the all-pairs gradients (every signal's dS/dmua and dS/dmus
+    from two replays with one slice per source and detector); checks single
+    signals against central differences at nfd voxels.  Fixed photon-per-lane
+    map, so finite differences use common random numbers."""
+    import torch
+    from mcxcl_amd import autograd, workloads
+
+    os.environ["MCXHIP_DYNAMIC"] = "0"
+    try:
+        cfg, mua0, mus0 = workloads.inverse_workload(photons_per_source=int(photons_per_source))
+        wts = torch.tensor(np.random.default_rng(seed).uniform(0.5, 1.5, (16, 16)))
+        autograd.detector_signals(torch.tensor(mua0), torch.tensor(mus0), cfg)   # warm-up
+        t0 = time.perf_counter()
+        mua = torch.tensor(mua0, requires_grad=True)
+        mus = torch.tensor(mus0, requires_grad=True)
+        S = autograd.detector_signals(mua, mus, cfg)
+        t1 = time.perf_counter()
+        (S * wts).sum().backward()
+        t2 = time.perf_counter()
+        _, dmua, _ = autograd.signal_jacobians(mua0, mus0, cfg)
+        t3 = time.perf_counter()
+        rng = np.random.default_rng(seed)
+        errs, tf = [], []
+        for k in range(nfd):
+            s_, d_ = int(rng.integers(16)), int(rng.integers(16))
+            g = dmua[s_, d_]
+            order = np.argsort(np.abs(g).ravel())[::-1]
+            v = np.unravel_index(order[int(rng.integers(0, 50))], g.shape)
+            h = 0.1 * mua0[v]
+            sig = []
+            for sgn in (1, -1):
+                m = mua0.copy()
+                m[v] += sgn * h
+                ta = time.perf_counter()
+                sig.append(float(autograd.detector_signals(torch.tensor(m), torch.tensor(mus0), cfg)[s_, d_]))
+                tf.append(time.perf_counter() - ta)
+            fd = (sig[0] - sig[1]) / (2 * h)
+            errs.append(abs(fd - g[v]) / abs(g[v]))
+        t_fwd = float(np.median(tf))
+        return {
+            "workload": "128^3 per-voxel mua/mus (2 layers, +-30%% smooth perturbations), 16 sources x 16 "
+                        "detectors in one launch, %d photons per source" % int(photons_per_source),
+            "signals": int(S.numel()),
+            "forward_s": round(t1 - t0, 4),
+            "backward_scalar_loss_s": round(t2 - t1, 4),
+            "all_pairs_gradients_s": round(t3 - t2, 4),
+            "fd_one_voxel_one_signal_s": round(2 * t_fwd, 3),
+            "voxels_checked": nfd,
+            "max_rel_err_vs_fd": round(float(max(errs)), 5) if errs else None,
+            "median_rel_err_vs_fd": round(float(np.median(errs)), 6) if errs else None,
+        }
+    finally:
+        os.environ.pop("MCXHIP_DYNAMIC", None)
+
+
 def batch_leg(ninst, steps):
     """Many independent simulations in one persistent launch: ninst 64^3 volumes
     with a random ellipsoid each, per-instance media, log-uniform photon counts
@@ -229,6 +290,9 @@ def main():
     ap.add_argument("--adjoint-photons", type=float, default=1e7,
                     help="photons of the replay-adjoint gradient leg (0 skips it)")
     ap.add_argument("--adjoint-fd-voxels", type=int, default=16, help="voxels checked by central differences")
+    ap.add_argument("--inverse-photons", type=float, default=1e6,
+                    help="photons per source of the 16x16 multi-source gradient leg (0 skips it)")
+    ap.add_argument("--inverse-fd-voxels", type=int, default=4)
     ap.add_argument("--batch-instances", type=int, default=256,
                     help="simulations of the batched-launch leg (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -344,6 +408,8 @@ def main():
         }
     if args.adjoint_photons > 0:
         res["adjoint"] = adjoint_leg(args.adjoint_photons, args.adjoint_fd_voxels)
+    if args.inverse_photons > 0:
+        res["inverse"] = inverse_leg(args.inverse_photons, args.inverse_fd_voxels)
     if args.batch_instances > 0:
         res["batch"] = batch_leg(args.batch_instances, args.batch_steps)
     print(json.dumps(res), flush=True)

@@ -84,7 +84,7 @@ hipError_t mcx_launch_batch_prep(double* field, size_t nfield, double* energy, u
                                  const uint32_t* state0, uint32_t nstate, hipStream_t stream);
 hipError_t mcx_launch_batch_finalize(const MCXBatchNorm* norm, const double* energy, uint32_t ninst, size_t maxlen,
                                      hipStream_t stream);
-/* sum the nrep replicas into in[0..n), then out[i] = (float)in[i] * scale */
+/* sum the nrep replicas into in[0..n), then out[i] = (float)in[i] * scale (out may be NULL) */
 hipError_t mcx_launch_finalize(double* in, float* out, size_t n, unsigned int nrep, float scale, hipStream_t stream);
 
 #ifdef __cplusplus

@@ -64,8 +64,11 @@ __global__ void finalize_kernel(double* __restrict__ in, float* __restrict__ out
         }
 
         in[i] = acc;
-        float v = (float)acc;
-        out[i] = v * scale;
+
+        if (out) {   /* NULL: only fold the replicas */
+            float v = (float)acc;
+            out[i] = v * scale;
+        }
     }
 }
 

@@ -802,9 +802,20 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             }
         }
 
-        HIP_TRY(dalloc(bufs, &dout, fieldlen));
-        HIP_TRY(mcx_launch_finalize(dfield, dout, fieldlen, g.nrep, (normalize && scale != 0.f) ? scale : 1.f,
-                                    stream));
+        /* a caller that hands in only the float64 grid (exportfield64 set,
+           exportfield NULL) gets no float32 copy: large replay grids are then
+           read back once */
+        const int wantfloat = cfg->exportfield != NULL || cfg->exportfield64 == NULL || cfg->parentid == mpStandalone;
+
+        if (wantfloat) {
+            HIP_TRY(dalloc(bufs, &dout, fieldlen));
+            HIP_TRY(mcx_launch_finalize(dfield, dout, fieldlen, g.nrep, (normalize && scale != 0.f) ? scale : 1.f,
+                                        stream));
+        } else if (g.nrep > 1) {
+            /* fold the replicas into replica 0 without a float output */
+            HIP_TRY(mcx_launch_finalize(dfield, NULL, fieldlen, g.nrep, 1.f, stream));
+        }
+
         HIP_TRY(hipEventRecord(ev2, stream));
 
         if (cfg->exportfield64) {
@@ -812,8 +823,16 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
                                    stream));
         }
 
-        hfield = (float*)malloc(fieldlen * sizeof(float));
-        HIP_TRY(hipMemcpyAsync(hfield, dout, fieldlen * sizeof(float), hipMemcpyDeviceToHost, stream));
+        if (wantfloat) {
+            hfield = (float*)malloc(fieldlen * sizeof(float));
+
+            if (!hfield) {
+                HIP_TRY(hipErrorOutOfMemory);
+            }
+
+            HIP_TRY(hipMemcpyAsync(hfield, dout, fieldlen * sizeof(float), hipMemcpyDeviceToHost, stream));
+        }
+
         HIP_TRY(hipStreamSynchronize(stream));
         {
             float jobms = 0.f;
@@ -821,7 +840,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             cfg->jobms = jobms;
         }
 
-        if (normalize && scale == 0.f && cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
+        if (wantfloat && normalize && scale == 0.f && cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
             for (int d = 1; d <= (int)cfg->detnum; d++) {
                 float sd = 0.f;
 
@@ -838,12 +857,14 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             }
         }
 
-        if (!cfg->exportfield) {
-            cfg->exportfield = (float*)calloc(fieldlen * 2, sizeof(float));
-        }
+        if (wantfloat) {
+            if (!cfg->exportfield) {
+                cfg->exportfield = (float*)calloc(fieldlen * 2, sizeof(float));
+            }
 
-        for (size_t i = 0; i < fieldlen; i++) {
-            cfg->exportfield[i] += hfield[i];
+            for (size_t i = 0; i < fieldlen; i++) {
+                cfg->exportfield[i] += hfield[i];
+            }
         }
     }
 

@@ -299,7 +299,7 @@ class PreparedConfig:
             out["detw"] = self.detw[:self.c.detectedcount].copy()
         if self.raw is not None:
             dims = [self.c.dim.x, self.c.dim.y, self.c.dim.z, self.ngate * self.ndet * self.nsrc]
-            out["flux64"] = self.raw.reshape(dims, order="F").copy()
+            out["flux64"] = self.raw.reshape(dims, order="F")   # the array the run filled, no copy
         return out
 
     def close(self):
@@ -331,12 +331,13 @@ def run(cfg=None, **kwargs):
     1: bit-reproducible arithmetic), ``rawfield`` (also return the unnormalised
     double-precision accumulator as ``flux64``), ``muamus`` (float32
     [2, Nx, Ny, Nz] per-voxel mua/mus in 1/mm beside a label ``vol``),
-    ``savedetweight`` (return each detected photon's exit weight as ``detw``) and
+    ``savedetweight`` (return each detected photon's exit weight as ``detw``),
     ``replayweight`` (replay weights per row of ``detphotons``, required with
-    per-voxel media)."""
+    per-voxel media) and ``rawonly`` (with ``rawfield``: return only ``flux64``,
+    skipping the float32 copy of large replay grids)."""
     with prepare(cfg, **kwargs) as pc:
         c = pc.c
-        if c.issave2pt:
+        if c.issave2pt and not (pc.user.get("rawfield") and pc.user.get("rawonly")):
             c.exportfield = _native.cmalloc(4 * pc.fieldlen)
         rc = lib.mcx_try_run_simulation(ctypes.byref(c))
         if rc:

@@ -216,3 +216,51 @@ def batched_workload(ninst=256, dim=64, seed=20251203, photons=(1e4, 1e6), srcty
             cfg["srcparam1"] = [float(rng.uniform(2.0, 8.0)), 0, 0, 0]
         cfgs.append(cfg)
     return cfgs
+
+
+def _smooth_field(rng, shape, sigma=10.0):
+    """seeded white noise low-passed by a Gaussian of width sigma (voxels) in
+    Fourier space, scaled to [-1, 1]"""
+    noise = rng.standard_normal(shape)
+    k = [np.fft.fftfreq(n) for n in shape]
+    kk = sum(np.meshgrid(*[ki ** 2 for ki in k], indexing="ij"))
+    smooth = np.real(np.fft.ifftn(np.fft.fftn(noise) * np.exp(-2.0 * (np.pi * sigma) ** 2 * kk)))
+    return (smooth / np.abs(smooth).max()).astype(np.float32)
+
+
+def inverse_workload(dim=128, photons_per_source=10000000, nside=4, top=30, seed=20251204, **over):
+    """Multi-source differentiable step: a [dim]^3 two-layer background (top
+    ``top`` voxels mua 0.015/mus 8, below mua 0.03/mus 12 per mm, g 0.9, n 1.37)
+    with smooth random per-voxel perturbations of +-30% on mua and on mus,
+    nside x nside pencil sources and nside x nside r=2 mm detectors interleaved
+    on the z=0 face, photons_per_source photons per source in one launch.
+    Returns (cfg, mua, mus) with mua/mus float32 [dim]^3 in 1/mm."""
+    rng = np.random.default_rng(seed)
+    vol = np.ones((dim, dim, dim), dtype=np.uint8)
+    vol[:, :, top:] = 2
+    prop = [[0, 0, 1, 1], [0.015, 8.0, 0.9, 1.37], [0.03, 12.0, 0.9, 1.37]]
+    base = np.asarray(prop, np.float32)
+    mua = base[vol, 0] * (1 + 0.3 * _smooth_field(rng, vol.shape))
+    mus = base[vol, 1] * (1 + 0.3 * _smooth_field(rng, vol.shape))
+    step = dim / (nside + 0.5)
+    src = [[step * (i + 0.5), step * (j + 0.5), 0] for j in range(nside) for i in range(nside)]
+    det = [[step * (i + 1.0), step * (j + 1.0), 0, 2] for j in range(nside) for i in range(nside)]
+    cfg = {
+        "nphoton": int(photons_per_source) * len(src),
+        "vol": vol,
+        "prop": prop,
+        "tstart": 0.0,
+        "tend": 5e-9,
+        "tstep": 5e-9,
+        "srcpos": src,
+        "srcdir": [[0, 0, 1, 0]] * len(src),
+        "detpos": det,
+        "isreflect": 1,
+        "issrcfrom0": 1,
+        "savedetflag": "DP",
+        "maxdetphoton": 20000000,
+        "seed": int(seed),
+        "session": "inverse",
+    }
+    cfg.update(over)
+    return cfg, mua.astype(np.float32), mus.astype(np.float32)

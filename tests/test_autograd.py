@@ -117,3 +117,106 @@ def test_mua_gradient_on_device_matches_finite_differences():
             sig.append(float(autograd.detector_signals(torch.tensor(m), torch.tensor(mus0), base).detach().sum()))
         fd = (sig[0] - sig[1]) / (2 * h)
         assert abs(fd - g[v]) <= 0.02 * abs(g[v]), (v, fd, g[v])
+
+
+def _multisource_case(nphoton=30000):
+    """3 pencil sources and 2 detectors on a 40^3 two-layer volume with per-voxel media"""
+    vol = np.ones((40, 40, 40), np.uint8)
+    vol[:, :, 15:] = 2
+    cfg = dict(nphoton=nphoton, vol=vol, prop=[[0, 0, 1, 1], [0.01, 3.0, 0.8, 1.37], [0.02, 5.0, 0.8, 1.37]],
+               srcpos=[[12, 20, 0], [20, 20, 0], [28, 20, 0]], srcdir=[[0, 0, 1, 0]] * 3, detpos=[[16, 20, 0, 2],
+               [24, 20, 0, 2]], tstart=0, tend=5e-9, tstep=5e-9, issrcfrom0=1, isreflect=1, seed=77,
+               savedetflag="DP", maxdetphoton=200000)
+    prop = np.asarray(cfg["prop"], np.float32)
+    rng = np.random.default_rng(1)
+    pert = 1 + 0.2 * rng.uniform(-1, 1, vol.shape).astype(np.float32)
+    return cfg, (prop[vol, 0] * pert).astype(np.float32), (prop[vol, 1] * (2 - pert)).astype(np.float32)
+
+
+def test_multisource_signals_are_per_source_and_detector(oracle_engine):
+    cfg, mua0, mus0 = _multisource_case()
+    S = autograd.detector_signals(torch.tensor(mua0), torch.tensor(mus0), cfg)
+    assert tuple(S.shape) == (3, 2)
+    # the source beside each detector gives it the most light
+    s = S.numpy()
+    assert s[0, 0] > s[2, 0] and s[2, 1] > s[0, 1] and (s > 0).all()
+
+
+def test_multisource_gradient_of_one_pair_is_its_replay_jacobian(oracle_engine):
+    """d S_sd / d mua = -J_sd: the replay Jacobian of the photons of source s
+    seen by detector d (the reference's replay with srcid=s, replaydet=d,
+    src/mcx_utils.c:1355-1421, src/mcx_core.cl:1602-1616)"""
+    cfg, mua0, mus0 = _multisource_case()
+    for s_, d_ in [(0, 1), (2, 0)]:
+        mua = torch.tensor(mua0, requires_grad=True)
+        S = autograd.detector_signals(mua, torch.tensor(mus0), cfg)
+        S[s_, d_].backward()
+        fwd = _oracle_run(dict(cfg, muamus=np.stack([mua0, mus0]), issavedet=1, issaveseed=1, savedetweight=1,
+                               issave2pt=0, srcid=0))
+        rep = dict(cfg, muamus=np.stack([mua0, mus0]), seed=fwd["seeds"], detphotons=fwd["detp"],
+                   replayweight=fwd["detw"], replaydet=d_ + 1, srcid=s_ + 1, rawfield=1, outputtype="jacobian")
+        jac = _oracle_run(rep)["flux64"][..., 0]
+        np.testing.assert_allclose(mua.grad.numpy(), -jac, rtol=1e-5, atol=1e-6 * np.abs(jac).max())
+
+
+def test_multisource_mua_gradient_matches_finite_differences(oracle_engine):
+    cfg, mua0, mus0 = _multisource_case(nphoton=20000)
+    wts = torch.tensor([[1.0, -0.5], [2.0, 0.3], [-1.0, 0.7]], dtype=torch.float64)
+    mua = torch.tensor(mua0, requires_grad=True)
+    S = autograd.detector_signals(mua, torch.tensor(mus0), cfg)
+    (S * wts).sum().backward()
+    g = mua.grad.numpy()
+    order = np.argsort(np.abs(g).ravel())[::-1]
+    for flat in list(order[:4]) + list(order[300:302]):
+        v = np.unravel_index(flat, g.shape)
+        h = 0.05 * mua0[v]
+        sig = []
+        for sgn in (1, -1):
+            m = mua0.copy()
+            m[v] += sgn * h
+            sig.append(float((autograd.detector_signals(torch.tensor(m), torch.tensor(mus0), cfg) * wts).sum()))
+        fd = (sig[0] - sig[1]) / (2 * h)
+        assert abs(fd - g[v]) <= 2e-3 * abs(g[v]) + 1e-4, (v, fd, g[v])
+
+
+def test_signal_jacobians_match_the_folded_backward(oracle_engine):
+    """every (source, detector) slice of the all-pairs replay equals the
+    gradient of that one signal through the folded backward, and the slices
+    weighted by an upstream gradient sum to the folded result"""
+    cfg, mua0, mus0 = _multisource_case(nphoton=20000)
+    S, dmua, dmus = autograd.signal_jacobians(mua0, mus0, cfg)
+    assert dmua.shape == (3, 2, 40, 40, 40) and dmus.shape == dmua.shape
+    S2 = autograd.detector_signals(torch.tensor(mua0), torch.tensor(mus0), cfg).numpy()
+    np.testing.assert_allclose(S, S2, rtol=1e-12)
+    wts = np.array([[1.0, -0.5], [2.0, 0.3], [-1.0, 0.7]])
+    mua = torch.tensor(mua0, requires_grad=True)
+    mus = torch.tensor(mus0, requires_grad=True)
+    (autograd.detector_signals(mua, mus, cfg) * torch.tensor(wts)).sum().backward()
+    ga = np.tensordot(wts, dmua.astype(np.float64), axes=([0, 1], [0, 1]))
+    gs = np.tensordot(wts, dmus.astype(np.float64), axes=([0, 1], [0, 1]))
+    np.testing.assert_allclose(mua.grad.numpy(), ga, rtol=1e-4, atol=1e-5 * np.abs(ga).max())
+    np.testing.assert_allclose(mus.grad.numpy(), gs, rtol=1e-4, atol=1e-5 * np.abs(gs).max())
+
+
+@pytest.mark.gpu
+def test_multisource_jacobians_on_device_match_finite_differences():
+    """the inverse-step workload at reduced size on the device: all-pairs
+    gradients against central differences of single signals at 6 voxels each
+    (bit-reproducible mode: every forward run launches the same photons)"""
+    cfg, mua0, mus0 = workloads.inverse_workload(dim=48, photons_per_source=100000, nside=2, top=12, mathmode=1)
+    S, dmua, dmus = autograd.signal_jacobians(mua0, mus0, cfg)
+    assert S.shape == (4, 4) and (S > 0).all() and dmua.shape == (4, 4, 48, 48, 48)
+    rng = np.random.default_rng(3)
+    for s_, d_ in [(0, 0), (3, 1)]:
+        g = dmua[s_, d_]
+        order = np.argsort(np.abs(g).ravel())[::-1]
+        for flat in list(order[:3]) + list(rng.choice(order[:2000], 3, replace=False)):
+            v = np.unravel_index(flat, g.shape)
+            h = 0.1 * mua0[v]
+            sig = []
+            for sgn in (1, -1):
+                m = mua0.copy()
+                m[v] += sgn * h
+                sig.append(float(autograd.detector_signals(torch.tensor(m), torch.tensor(mus0), cfg)[s_, d_]))
+            fd = (sig[0] - sig[1]) / (2 * h)
+            assert abs(fd - g[v]) <= 0.02 * abs(g[v]), (s_, d_, v, fd, g[v])
