@@ -1,7 +1,10 @@
 /*
  * mcx_core_fast.hip - transport kernel instances using gfx950 hardware
  * transcendentals and reciprocal-based division (the analogue of the
- * reference's -cl-mad-enable -DMCX_USE_NATIVE build).
+ * reference's -DMCX_USE_NATIVE build).  Built without floating-point
+ * contraction: the compiler fuses multiply-adds differently in each template
+ * instance, and a replay instance must round the forward instance's path
+ * arithmetic identically to retrace its photons.
  */
 #define MCX_PARITY 0
 #include "mcx_core_impl.h"

@@ -50,7 +50,17 @@ enum {kPencil = 0, kIsotropic = 1, kCone = 2, kGaussian = 3, kPlanar = 4, kFouri
       kZGaussian = 11, kLine = 12, kSlit = 13, kPencilArray = 14, kRing = 17
      };
 
-namespace mcxk {
+/* Each build (parity, fast) gets its own namespace: the kernels are templates
+   instantiated with the same arguments in both translation units, and under
+   one name their host launch stubs would be merged at link time, so a launch
+   from one build could run the other build's code object. */
+#if MCX_PARITY
+#define MCXK_NS mcxk_parity
+#else
+#define MCXK_NS mcxk_fast
+#endif
+
+namespace MCXK_NS {
 
 #if MCX_PARITY
 __device__ __forceinline__ float kexp(float x) {
@@ -133,13 +143,13 @@ __device__ __forceinline__ float ksin_acos(float x) {
 #endif
 }
 
-/* LDS deposit box cells: float64.  (float32 cells would double the box for the
-   same LDS; they gave wrong launch energies and lost deposits on gfx950 with
-   boxes of 16^3 and more, so they stay behind MCX_BOX_F32 until understood.) */
-#if !MCX_PARITY && defined(MCX_BOX_F32)
-typedef float BoxCell;
-#else
+/* LDS deposit box cells: float64 in the parity build, float32 in the fast
+   build (twice the cells for the same LDS; the reference accumulates its whole
+   grid in float32).  The host sizes the box with the same element size. */
+#if MCX_PARITY
 typedef double BoxCell;
+#else
+typedef float BoxCell;
 #endif
 
 /* ---------------- RNG: xorshift128+ (src/mcx_core.cl:684-698) ---------------- */
@@ -1452,4 +1462,6 @@ __global__ void __launch_bounds__(MCX_MAX_BLOCK) __attribute__((amdgpu_waves_per
     nseed[lane * 4 + 3] = (uint32_t)rng.s1;
 }
 
-}  // namespace mcxk
+}  // namespace MCXK_NS
+
+namespace mcxk = MCXK_NS;

@@ -369,11 +369,7 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
        accumulates the whole grid in float32); the budget buys two leading gates of
        16^3 voxels when it can, else one gate of the largest cube that fits. */
     {
-#ifdef MCX_BOX_F32
         const size_t elem = (cfg->mathmode == 1) ? sizeof(double) : sizeof(float);
-#else
-        const size_t elem = sizeof(double);
-#endif
         size_t budget = MCX_BOX_BYTES;
         const char* env = getenv("MCXHIP_BOX_BYTES");
 
