@@ -39,10 +39,12 @@
 extern "C" void* mcx_push_errjmp(void* jb);
 extern "C" void mcx_pop_errjmp(void* old);
 void mcx_check_supported(const Config* cfg);
-size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lblock, int* reflect);
+size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lblock, int* reflect, size_t boxbytes);
 
 #define MCX_BATCH_MAXBUF 16
 #define MCX_BATCH_MAXBLOCK 256
+/* LDS deposit box budget per workgroup (bytes) */
+#define MCX_BATCH_BOX_BYTES (32u * 1024u)
 #define MCX_BATCH_TUNE_PHOTONS 1.0e6
 #define MCX_BATCH_TUNE_MS 150.0
 
@@ -178,7 +180,7 @@ static size_t batch_params(MCXBatch* b, size_t lblock) {
         Config* cfg = b->cfg[i];
         MCXKParam& g = b->prm[i];
         int reflect = 0;
-        size_t s = mcx_kparam_setup(cfg, &g, lblock, lblock, &reflect);
+        size_t s = mcx_kparam_setup(cfg, &g, lblock, lblock, &reflect, MCX_BATCH_BOX_BYTES);
         shmem = s > shmem ? s : shmem;
         b->reflect |= reflect;
         g.nrep = 1;
@@ -395,7 +397,7 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
         Config* cfg = b->cfg[i];
         MCXKParam g;
         int reflect;
-        (void)mcx_kparam_setup(cfg, &g, 64, 64, &reflect);
+        (void)mcx_kparam_setup(cfg, &g, 64, 64, &reflect, MCX_BATCH_BOX_BYTES);
         b->fieldoff[i + 1] = b->fieldoff[i] + g.fieldlen;
         maxlen = g.fieldlen > maxlen ? g.fieldlen : maxlen;
         nvoxall += g.nvox;

@@ -9,6 +9,16 @@
 
 #include <hip/hip_runtime.h>
 
+/* Workgroup size bounds of the transport kernel.  The bit-reproducible build
+   launches 256-lane workgroups.  The fast build runs one workgroup per CU,
+   as wide as the kernel's registers allow: 1024 lanes (16 waves, <=128 VGPRs)
+   for the pencil-beam specialisation and 768 (12 waves, <=168 VGPRs) for the
+   general path, so that a single LDS deposit box can use most of the CU's
+   LDS instead of being split between several workgroups. */
+#define MCX_MAX_BLOCK 256
+#define MCX_FAST_SIMPLE_BLOCK 1024
+#define MCX_FAST_BLOCK 768
+
 /* Kernel parameter block, passed by value (lives in the kernarg segment, read
    with scalar loads).  Field meanings follow the reference's MCXParam
    (src/mcx_host.h:106-166), restricted to what this kernel reads. */
@@ -24,7 +34,9 @@ struct MCXKParam {
     int32_t seed, oddphoton, replaydet, srcid;
     uint32_t fieldlen;
     /* deposit privatisation: the box [x0,x0+nx)x[y0,y0+ny)x[z0,z0+nz) of the first ng
-       gates is accumulated in LDS (f64) per workgroup and flushed once; ng=0 disables */
+       gates is accumulated in LDS per workgroup (float64 cells in the
+       bit-reproducible build, float32 in the fast build) and flushed once; ng=0
+       disables */
     uint32_t box_x0, box_y0, box_z0, box_nx, box_ny, box_nz, box_ng, box_cells;
     uint32_t nrep;                /* global accumulator replicas (power of two) */
     uint32_t dynamic;             /* 1: photons are handed out per workgroup at run time

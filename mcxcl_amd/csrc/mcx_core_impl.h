@@ -1220,16 +1220,18 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
     depcache_flush(g, sh, dcache);
 }
 
-template <bool SAVEDET, bool REFLECT, bool SIMPLE>
-#ifndef MCX_MAX_BLOCK
-#define MCX_MAX_BLOCK 256
-#endif
 #ifdef MCX_WAVES_PER_EU
 #define MCX_KATTR __attribute__((amdgpu_waves_per_eu(MCX_WAVES_PER_EU)))
 #else
 #define MCX_KATTR
 #endif
-__global__ void __launch_bounds__(MCX_MAX_BLOCK) MCX_KATTR transport_kernel(const MCXKParam g) {
+#if MCX_PARITY
+#define MCX_KBOUND(SIMPLE) MCX_MAX_BLOCK
+#else
+#define MCX_KBOUND(SIMPLE) ((SIMPLE) ? MCX_FAST_SIMPLE_BLOCK : MCX_FAST_BLOCK)
+#endif
+template <bool SAVEDET, bool REFLECT, bool SIMPLE>
+__global__ void __launch_bounds__(MCX_KBOUND(SIMPLE)) MCX_KATTR transport_kernel(const MCXKParam g) {
     extern __shared__ float4 lds[];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t pplen = g.w0offset + g.srcnum;

@@ -26,16 +26,17 @@
 #include "mcx_core.h"
 #include "mcx_host.h"
 
-/* lanes per workgroup of the transport kernel (its __launch_bounds__) */
-#define MCX_LAUNCH_BLOCK ((size_t)256)
-#ifndef MCX_MAX_BLOCK
-#define MCX_MAX_BLOCK 256
-#endif
-#define MCX_LAUNCH_BLOCK_MAX ((size_t)MCX_MAX_BLOCK)
-/* LDS deposit box: edge in voxels and byte budget per workgroup */
-#define MCX_BOX_EDGE_MAX 24u
+/* lanes per workgroup of the bit-reproducible transport kernel */
+#define MCX_LAUNCH_BLOCK ((size_t)MCX_MAX_BLOCK)
+/* LDS deposit box: largest edge in voxels, the byte budget of the
+   bit-reproducible build and the batched launch, and the per-workgroup LDS cap
+   (gfx950: 160 KB) */
+#define MCX_BOX_EDGE_MAX 48u
 #define MCX_BOX_BYTES (32u * 1024u)
-#define MCX_LDS_LIMIT (64u * 1024u)
+#define MCX_LDS_LIMIT (160u * 1024u)
+/* LDS kept back from the fast build's box budget: static LDS and allocation
+   granularity */
+#define MCX_LDS_SLACK 1024u
 /* global accumulator replicas (power of two) and their total byte cap */
 #define MCX_FIELD_REPLICAS 1u
 #define MCX_REPLICA_BYTES ((size_t)1 << 30)
@@ -245,7 +246,7 @@ void mcx_check_supported(const Config* cfg) {
    geometry, time gates, record layout, the specialised-path flag, the LDS deposit
    box and the accumulator replicas.  Returns the dynamic LDS bytes per workgroup;
    *reflect receives the reference's rule for the reflection kernel variant. */
-size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lblock, int* reflect) {
+size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lblock, int* reflect, size_t boxbytes) {
     const unsigned int maxgate = (unsigned int)((cfg->tend - cfg->tstart) / cfg->tstep + 0.5);
     cfg->maxgate = maxgate;
     const size_t nvox = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
@@ -366,11 +367,12 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
        source voxel (clamped to the grid) is accumulated per workgroup in LDS and
        added to the global grid once at the end.  Cells are float64 in the
        bit-reproducible build and float32 in the fast build (the reference
-       accumulates the whole grid in float32); the budget buys two leading gates of
-       16^3 voxels when it can, else one gate of the largest cube that fits. */
+       accumulates the whole grid in float32); the budget (boxbytes, in bytes)
+       buys two leading gates of 16^3 voxels when it can, else one gate of the
+       largest cube that fits. */
     {
         const size_t elem = (cfg->mathmode == 1) ? sizeof(double) : sizeof(float);
-        size_t budget = MCX_BOX_BYTES;
+        size_t budget = boxbytes;
         const char* env = getenv("MCXHIP_BOX_BYTES");
 
         if (env) {
@@ -487,22 +489,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
 
     /* the reference rounds the thread count down to whole blocks; the photon map
        depends only on the thread count, so the transport kernel itself is launched
-       in workgroups of MCX_LAUNCH_BLOCK lanes with the tail lanes idle */
+       in workgroups of its own size (lblock) with the tail lanes idle */
     nthread = (nthread / nblocksize) * nblocksize;
-    size_t lblock = MCX_LAUNCH_BLOCK;
-    {
-        const char* env = getenv("MCXHIP_BLOCK");
-
-        if (env) {
-            lblock = (size_t)strtoul(env, NULL, 10);
-        }
-
-        if (lblock < 64 || lblock > MCX_LAUNCH_BLOCK_MAX || (lblock & 63)) {
-            free(gpu);
-            MCX_ERROR(-4, "MCXHIP_BLOCK must be a multiple of 64 within the kernel's launch bounds");
-        }
-    }
-    size_t nlaunch = (nthread + lblock - 1) / lblock;
 
     if (nthread == 0) {
         free(gpu);
@@ -512,9 +500,54 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     free(gpu);
     gpu = NULL;
 
+    const char* envblock = getenv("MCXHIP_BLOCK");
+    const char* envdyn = getenv("MCXHIP_DYNAMIC");
+    const int fast = cfg->mathmode == 0 && !(cfg->debuglevel & MCX_DEBUG_RNG);
+    /* fast mode hands photons out at run time (below) unless replaying, where
+       each lane must rerun the photons of its recorded seeds */
+    const int dynamic = fast && cfg->seed != SEED_FROM_FILE && !(envdyn && envdyn[0] == '0');
+    size_t lblock = MCX_LAUNCH_BLOCK;
     MCXKParam g;
     int reflect = 0;
-    size_t shmem = mcx_kparam_setup(cfg, &g, nthread, lblock, &reflect);
+    size_t shmem = mcx_kparam_setup(cfg, &g, nthread, lblock, &reflect, fast ? 0 : MCX_BOX_BYTES);
+    {
+        const size_t bound = fast ? (g.simple ? MCX_FAST_SIMPLE_BLOCK : MCX_FAST_BLOCK) : MCX_MAX_BLOCK;
+
+        if (envblock) {
+            lblock = (size_t)strtoul(envblock, NULL, 10);
+        } else if (dynamic) {
+            lblock = bound;
+        }
+
+        if (lblock < 64 || lblock > bound || (lblock & 63)) {
+            MCX_ERROR(-4, "MCXHIP_BLOCK must be a multiple of 64 within the kernel's launch bounds");
+        }
+    }
+
+    /* fast mode: the deposit box takes the LDS a CU has left for each of its
+       resident workgroups, given the occupancy the kernel's registers allow */
+    if (fast) {
+        int ldscu = 0, ldsblock = 0, nwg = 0;
+        shmem = mcx_kparam_setup(cfg, &g, nthread, lblock, &reflect, 0);
+
+        if (hipDeviceGetAttribute(&ldscu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess &&
+                hipDeviceGetAttribute(&ldsblock, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess) {
+            nwg = mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, (int)g.simple);
+        }
+
+        size_t budget = 0;
+
+        if (nwg > 0 && ldscu > 0 && ldsblock > 0) {
+            size_t per = (size_t)ldscu / (size_t)nwg;
+            per = per < (size_t)ldsblock ? per : (size_t)ldsblock;
+            per = per < MCX_LDS_LIMIT ? per : MCX_LDS_LIMIT;
+            budget = per > shmem + MCX_LDS_SLACK ? per - shmem - MCX_LDS_SLACK : 0;
+        }
+
+        shmem = mcx_kparam_setup(cfg, &g, nthread, lblock, &reflect, budget);
+    }
+
+    size_t nlaunch = (nthread + lblock - 1) / lblock;
     const unsigned int maxgate = g.maxgate;
     const size_t nvox = g.nvox;
     const size_t fieldlen = g.fieldlen;
@@ -530,9 +563,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
        150 photons per lane and a wave waits for its slowest lane).  Bit-parity
        mode keeps the reference's fixed photon-per-thread map. */
     {
-        const char* env = getenv("MCXHIP_DYNAMIC");
-        int want = cfg->mathmode == 0 && !g.debugrng && cfg->seed != SEED_FROM_FILE && !(env && env[0] == '0');
-        int nb = want ? mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, (int)g.simple) : 0;
+        int nb = (dynamic && !g.debugrng) ?
+                 mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, (int)g.simple) : 0;
 
         if (nb > 0 && ncu > 0) {
             nlaunch = (size_t)nb * (size_t)ncu;
