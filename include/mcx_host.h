@@ -57,8 +57,9 @@ int mcx_device_atomic_ceiling(int gpuid, unsigned int ncell, double* rate);
    Config per call): many independent simulations with their own volumes,
    media, sources and photon counts run in one persistent launch ---- */
 typedef struct MCXBatchOpt {
-    unsigned int blocksize;      /* lanes per workgroup: 64, 128, 192 or 256; 0 picks the shape that keeps
-                                    the most lanes resident */
+    unsigned int blocksize;      /* lanes per workgroup: a multiple of 64 up to 1024 (each workgroup's
+                                    LDS deposit box grows with its share of the CU's LDS); 0 picks the
+                                    shape that keeps the most lanes resident, 256 lanes on ties */
     unsigned int noprobe;        /* 1: balance the per-XCD instance lists by photon counts instead of the
                                     costs a probe step measures */
     unsigned int usegraph;       /* 1: each step replays a captured HIP graph (prepare, transport, finalize) */

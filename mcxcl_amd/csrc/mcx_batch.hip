@@ -42,9 +42,11 @@ void mcx_check_supported(const Config* cfg);
 size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lblock, int* reflect, size_t boxbytes);
 
 #define MCX_BATCH_MAXBUF 16
-#define MCX_BATCH_MAXBLOCK 256
-/* LDS deposit box budget per workgroup (bytes) */
-#define MCX_BATCH_BOX_BYTES (32u * 1024u)
+#define MCX_BATCH_MAXBLOCK ((size_t)MCX_BATCH_BLOCK_MAX)
+/* LDS kept back from the deposit-box budget (static LDS, allocation granularity)
+   and the per-workgroup LDS cap (gfx950: 160 KB) */
+#define MCX_BATCH_LDS_SLACK 1024u
+#define MCX_BATCH_LDS_LIMIT (160u * 1024u)
 #define MCX_BATCH_TUNE_PHOTONS 1.0e6
 #define MCX_BATCH_TUNE_MS 150.0
 
@@ -170,9 +172,10 @@ static void batch_free(MCXBatch* b) {
     free(b);
 }
 
-/* kernel parameters of every instance for workgroups of lblock lanes; returns
-   the LDS bytes the launch needs (the largest instance's) */
-static size_t batch_params(MCXBatch* b, size_t lblock) {
+/* kernel parameters of every instance for workgroups of lblock lanes, each
+   with an LDS deposit box of at most boxbytes; returns the LDS bytes the launch
+   needs (the largest instance's) */
+static size_t batch_params(MCXBatch* b, size_t lblock, size_t boxbytes) {
     size_t shmem = 0;
     b->reflect = 0;
 
@@ -180,7 +183,7 @@ static size_t batch_params(MCXBatch* b, size_t lblock) {
         Config* cfg = b->cfg[i];
         MCXKParam& g = b->prm[i];
         int reflect = 0;
-        size_t s = mcx_kparam_setup(cfg, &g, lblock, lblock, &reflect, MCX_BATCH_BOX_BYTES);
+        size_t s = mcx_kparam_setup(cfg, &g, lblock, lblock, &reflect, boxbytes);
         shmem = s > shmem ? s : shmem;
         b->reflect |= reflect;
         g.nrep = 1;
@@ -297,15 +300,31 @@ static void batch_enqueue(MCXBatch* b, int timed, int probe) {
     BHIP(mcx_launch_batch_finalize(b->dnorm, b->denergy, b->n, b->maxlen, b->stream));
 }
 
-/* resident workgroups for lblock lanes; sets b->lblock, b->nblock, b->shmem */
+/* resident workgroups for lblock lanes; sets b->lblock, b->nblock, b->shmem.
+   The deposit box of each instance takes the LDS a CU has left for each of the
+   workgroups its registers let it hold (one 1024-lane workgroup per CU shares
+   one box instead of four 256-lane workgroups splitting it). */
 static int batch_shape(MCXBatch* b, size_t lblock, int ncu) {
-    size_t shmem = batch_params(b, lblock);
+    size_t shmem = batch_params(b, lblock, 0);
+    int ldscu = 0, ldsblock = 0;
 
-    if (shmem > 64 * 1024) {
+    if (hipDeviceGetAttribute(&ldscu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, b->dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ldsblock, hipDeviceAttributeMaxSharedMemoryPerBlock, b->dev) != hipSuccess ||
+            ldscu <= 0 || ldsblock <= 0 || shmem > (size_t)ldsblock) {
         return 0;
     }
 
     int nb = mcx_batch_occupancy((int)lblock, shmem, b->reflect);
+
+    if (nb <= 0) {
+        return 0;
+    }
+
+    size_t per = (size_t)ldscu / (size_t)nb;
+    per = per < (size_t)ldsblock ? per : (size_t)ldsblock;
+    per = per < MCX_BATCH_LDS_LIMIT ? per : MCX_BATCH_LDS_LIMIT;
+    shmem = batch_params(b, lblock, per > shmem + MCX_BATCH_LDS_SLACK ? per - shmem - MCX_BATCH_LDS_SLACK : 0);
+    nb = mcx_batch_occupancy((int)lblock, shmem, b->reflect);
 
     if (nb <= 0 || (size_t)nb * ncu * lblock > b->maxlanes) {
         return 0;
@@ -351,7 +370,7 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
     }
 
     if (opt->blocksize && (opt->blocksize < 64 || opt->blocksize > MCX_BATCH_MAXBLOCK || (opt->blocksize & 63))) {
-        MCX_ERROR(-4, "blocksize must be 64, 128, 192 or 256");
+        MCX_ERROR(-4, "blocksize must be a multiple of 64 from 64 to 1024");
     }
 
     unsigned int activedev = 0;
@@ -397,7 +416,7 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
         Config* cfg = b->cfg[i];
         MCXKParam g;
         int reflect;
-        (void)mcx_kparam_setup(cfg, &g, 64, 64, &reflect, MCX_BATCH_BOX_BYTES);
+        (void)mcx_kparam_setup(cfg, &g, 64, 64, &reflect, 0);
         b->fieldoff[i + 1] = b->fieldoff[i] + g.fieldlen;
         maxlen = g.fieldlen > maxlen ? g.fieldlen : maxlen;
         nvoxall += g.nvox;
@@ -482,7 +501,12 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
        measured in a probe step at a scaled-down photon count (the kernel sums
        each instance's workgroup wall-clock time), unless opt->noprobe is set
        (the lists are then balanced by photon counts). */
-    const size_t blocks[3] = {256, 128, 64};
+    /* equal resident lanes go to the first shape listed.  256 lanes comes first:
+       on 256 instances of 64^3 with one gate (deposits spread over the whole
+       volume) one 1024-lane workgroup per CU with a 4x larger box took 2385 ms
+       per step against 2300 ms for four 256-lane workgroups, whose smaller boxes
+       are flushed faster at each change of instance */
+    const size_t blocks[5] = {256, 512, 1024, 128, 64};
     size_t total = 0;
 
     for (unsigned int i = 0; i < n; i++) {
@@ -496,7 +520,7 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
     if (!bestblock) {
         size_t bestlanes = 0;
 
-        for (int bi = 0; bi < 3; bi++) {
+        for (int bi = 0; bi < 5; bi++) {
             if (batch_shape(b, blocks[bi], ncu) && b->nblock * b->lblock > bestlanes) {
                 bestlanes = b->nblock * b->lblock;
                 bestblock = blocks[bi];

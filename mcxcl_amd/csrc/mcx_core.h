@@ -18,6 +18,9 @@
 #define MCX_MAX_BLOCK 256
 #define MCX_FAST_SIMPLE_BLOCK 1024
 #define MCX_FAST_BLOCK 768
+/* Workgroup size bound of the batched kernel (general path at four waves per
+   SIMD, <=128 VGPRs): up to one 1024-lane workgroup per CU. */
+#define MCX_BATCH_BLOCK_MAX 1024
 
 /* Kernel parameter block, passed by value (lives in the kernarg segment, read
    with scalar loads).  Field meanings follow the reference's MCXParam

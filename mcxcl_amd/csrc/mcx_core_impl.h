@@ -1344,7 +1344,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <bool REFLECT>
 /* four waves per SIMD: at most 128 VGPRs (the general transport path alone
    would take 140 and three waves) */
-__global__ void __launch_bounds__(MCX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) batch_kernel(const MCXKParam* __restrict__ prm,
+__global__ void __launch_bounds__(MCX_BATCH_BLOCK_MAX) __attribute__((amdgpu_waves_per_eu(4))) batch_kernel(const MCXKParam* __restrict__ prm,
         const uint32_t* __restrict__ qlist, const uint32_t nqueue, uint32_t* __restrict__ state,
         uint32_t* __restrict__ nseed, double* __restrict__ energy, unsigned long long* __restrict__ cost) {
     extern __shared__ float4 lds[];

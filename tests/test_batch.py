@@ -85,6 +85,16 @@ def test_batch_fixed_schedule_without_graph():
 
 
 @pytest.mark.gpu
+def test_batch_wide_workgroups():
+    """one 1024-lane workgroup per CU, each with a deposit box sized from the
+    CU's LDS, gives the same instances as the single runs"""
+    cfgs = _small_batch(n=6, seed=17)
+    outs, info = pmcxcl.run_batch(cfgs, blocksize=1024)
+    assert info["blocksize"] == 1024 and info["nblock"] > 0
+    _check_against_single(outs, cfgs)
+
+
+@pytest.mark.gpu
 def test_graph_replays_keep_bookkeeping():
     """replay the captured step several times, with eager work and
     synchronisations in between; every replay must launch every photon and
