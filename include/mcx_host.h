@@ -53,6 +53,14 @@ int mcx_device_rng_stream(int gpuid, const unsigned int seed4[4], unsigned int n
    an ncell-cell grid on device gpuid: the ceiling the fluence deposits run against */
 int mcx_device_atomic_ceiling(int gpuid, unsigned int ncell, double* rate);
 
+/* launch-shape model of the fast transport kernel variant (savedet, reflect,
+   simple) on the current device: its VGPRs per lane and largest workgroup
+   (returns 0 on success), and the workgroups of nthreadblock lanes with shmem
+   bytes of dynamic LDS that one CU holds (gfx950: 4 SIMDs of at most 8 waves
+   and 512 VGPRs per lane each, the device's LDS per CU) */
+int mcx_fast_kernel_attr(int savedet, int reflect, int simple, int* vgpr, int* maxlanes);
+int mcx_fast_occupancy(int nthreadblock, size_t shmem, int savedet, int reflect, int simple);
+
 /* ---- batched launch (no counterpart in the reference, whose host runs one
    Config per call): many independent simulations with their own volumes,
    media, sources and photon counts run in one persistent launch ---- */

@@ -320,6 +320,9 @@ typedef struct MCXConfig {
     float* exportdetweight;  /* optional: exit weight of each detected photon (maxdetphoton floats, caller-owned) */
     float* replayinweight;   /* optional: replay weight of each row of detphotons, overriding the weight computed
                                 from partial path lengths (needed with muamus) */
+    float* exportfield32;    /* optional: the unnormalised field converted to float32 on the device (fieldlen
+                                entries, caller-owned, overwritten); with exportfield NULL no normalised copy
+                                is made, so large replay grids are read back once at half the bytes */
 } Config;
 
 /* ---- configuration API; each line names the reference declaration it replaces ---- */

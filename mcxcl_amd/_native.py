@@ -133,6 +133,7 @@ CONFIG_FIELDS = [
     ("srcdata", ctypes.c_void_p), ("mathmode", ctypes.c_int), ("kernelms", ctypes.c_float),
     ("jobms", ctypes.c_float), ("runtimems", ctypes.c_double), ("exportfield64", ctypes.c_void_p),
     ("muamus", ctypes.c_void_p), ("exportdetweight", ctypes.c_void_p), ("replayinweight", ctypes.c_void_p),
+    ("exportfield32", ctypes.c_void_p),
 ]
 
 lib.mcx_config_sizeof.restype = ctypes.c_size_t

@@ -128,7 +128,9 @@ def signal_jacobians(mua, mus, cfg):
     (srcid -1) give every pair's J_sd and WP_sd, the reference's per-detector
     replay Jacobian (src/mcx_utils.c:1355-1421, src/mcx_host.cpp:1397-1433)
     extended over sources.  The grids hold nsrc x ndet x Nvox float64 values
-    on the device."""
+    on the device; each is converted to float32 there and read back once.  The
+    gradients are views in the grids' memory order (x fastest), so they are not
+    C-contiguous."""
     mua = np.ascontiguousarray(mua, dtype=np.float32)
     mus = np.ascontiguousarray(mus, dtype=np.float32)
     mm = np.stack([mua, mus])
@@ -154,18 +156,21 @@ def signal_jacobians(mua, mus, cfg):
         rep["replaydet"] = 0
     unit = float(run_cfg.get("unitinmm", 1.0))
 
+    rev = shape[::-1]
+
     def slices(outputtype, scale):
-        f = _run(dict(rep, outputtype=outputtype, rawonly=1))["flux64"]   # [Nx, Ny, Nz, gates x det x src]
-        f = f.reshape(shape + (-1, ndet, nsrc), order="F")
-        f = f[..., 0, :, :] if f.shape[3] == 1 else f.sum(3)            # [Nx, Ny, Nz, det, src]
-        out = np.empty((nsrc, ndet) + shape, dtype=np.float32)
-        np.multiply(np.moveaxis(f, (4, 3), (0, 1)), scale, out=out, casting="same_kind")
-        return out
+        """[nsrc, ndet, Nz, Ny, Nx] in memory order, times scale"""
+        f = _run(dict(rep, outputtype=outputtype, rawfield=0, rawfield32=1, rawonly=1))["flux32"]
+        m = f.reshape(-1, order="F").reshape((nsrc, ndet, -1) + rev)   # memory order: src, det, gate, z, y, x
+        m = m[:, :, 0] if m.shape[2] == 1 else m.sum(2)
+        m *= np.float32(scale)
+        return m
 
     dmua = slices("jacobian", -unit)                     # -J
     dmus = slices("wp", 1.0)
     with np.errstate(divide="ignore"):
         inv = np.where(mus > 0, 1.0 / mus, 0.0).astype(np.float32)
-    dmus *= inv                                          # WP / mus
+    dmus *= np.ascontiguousarray(inv.transpose(2, 1, 0))  # WP / mus
     dmus += dmua                                         # - J
-    return sums.reshape(nsrc, ndet), dmua, dmus
+    view = (0, 1, 4, 3, 2)                               # -> [nsrc, ndet, Nx, Ny, Nz]
+    return sums.reshape(nsrc, ndet), dmua.transpose(view), dmus.transpose(view)

@@ -307,16 +307,26 @@ static void batch_enqueue(MCXBatch* b, int timed, int probe) {
 static int batch_shape(MCXBatch* b, size_t lblock, int ncu) {
     size_t shmem = batch_params(b, lblock, 0);
     int ldscu = 0, ldsblock = 0;
+    const int verbose = getenv("MCXHIP_VERBOSE") != NULL;
 
     if (hipDeviceGetAttribute(&ldscu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, b->dev) != hipSuccess ||
             hipDeviceGetAttribute(&ldsblock, hipDeviceAttributeMaxSharedMemoryPerBlock, b->dev) != hipSuccess ||
             ldscu <= 0 || ldsblock <= 0 || shmem > (size_t)ldsblock) {
+        if (verbose) {
+            fprintf(stderr, "mcxhip batch: %zu lanes: LDS per CU %d, per workgroup %d, tables %zu B\n", lblock, ldscu,
+                    ldsblock, shmem);
+        }
+
         return 0;
     }
 
     int nb = mcx_batch_occupancy((int)lblock, shmem, b->reflect);
 
     if (nb <= 0) {
+        if (verbose) {
+            fprintf(stderr, "mcxhip batch: %zu lanes: no resident workgroup with %zu B of LDS\n", lblock, shmem);
+        }
+
         return 0;
     }
 
@@ -327,6 +337,11 @@ static int batch_shape(MCXBatch* b, size_t lblock, int ncu) {
     nb = mcx_batch_occupancy((int)lblock, shmem, b->reflect);
 
     if (nb <= 0 || (size_t)nb * ncu * lblock > b->maxlanes) {
+        if (verbose) {
+            fprintf(stderr, "mcxhip batch: %zu lanes: %d resident workgroups with %zu B of LDS (lane cap %zu)\n",
+                    lblock, nb, shmem, b->maxlanes);
+        }
+
         return 0;
     }
 

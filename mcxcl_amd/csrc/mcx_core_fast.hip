@@ -33,16 +33,36 @@ extern "C" hipError_t mcx_launch_fast(const MCXKParam* prm, int nblock, int nthr
            : launch_variant<false, false>(prm, nblock, nthreadblock, shmem, stream);
 }
 
-template <bool SAVEDET, bool REFLECT, bool SIMPLE>
-static int occupancy_of(int nthreadblock, size_t shmem) {
-    int nb = 0;
+/* Resident workgroups per CU of a kernel from its own register and LDS use and
+   gfx950's per-CU resources: 4 SIMDs, each holding at most 8 waves and 512
+   VGPRs per lane allocated in granules of 8, and the device's LDS per CU in
+   512-byte granules per workgroup.  A workgroup spreads its waves over the 4
+   SIMDs.  (The runtime's occupancy query has returned half of this count on
+   some MI355X hosts for the same code object, ruling out 1024-lane workgroups
+   that launch and run there.) */
+static int gfx950_occupancy(const void* kernel, int nthreadblock, size_t shmem) {
+    hipFuncAttributes a;
+    int dev = 0, ldscu = 0;
 
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxk::transport_kernel<SAVEDET, REFLECT, SIMPLE>,
-            nthreadblock, shmem) != hipSuccess) {
+    if (nthreadblock <= 0 || hipFuncGetAttributes(&a, kernel) != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ldscu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+            nthreadblock > a.maxThreadsPerBlock || ldscu <= 0) {
         return 0;
     }
 
-    return nb;
+    const int waves = (nthreadblock + 63) / 64;
+    const int perSimd = (waves + 3) / 4;
+    const int vgpr = ((a.numRegs > 0 ? a.numRegs : 1) + 7) & ~7;
+    const int simdWaves = (512 / vgpr) < 8 ? (512 / vgpr) : 8;
+    const int byRegs = simdWaves / perSimd;
+    const size_t lds = (shmem + a.sharedSizeBytes + 511) & ~(size_t)511;
+    const int byLds = lds ? (int)((size_t)ldscu / lds) : byRegs;
+    return byRegs < byLds ? byRegs : byLds;
+}
+
+template <bool SAVEDET, bool REFLECT, bool SIMPLE>
+static int occupancy_of(int nthreadblock, size_t shmem) {
+    return gfx950_occupancy((const void*)mcxk::transport_kernel<SAVEDET, REFLECT, SIMPLE>, nthreadblock, shmem);
 }
 
 extern "C" int mcx_fast_occupancy(int nthreadblock, size_t shmem, int savedet, int reflect, int simple) {
@@ -81,11 +101,31 @@ extern "C" hipError_t mcx_launch_batch(const MCXKParam* prm, const uint32_t* qli
 }
 
 extern "C" int mcx_batch_occupancy(int nthreadblock, size_t shmem, int reflect) {
-    int nb = 0;
-    hipError_t e = reflect ?
-                   hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxk::batch_kernel<true>, nthreadblock, shmem) :
-                   hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, mcxk::batch_kernel<false>, nthreadblock, shmem);
-    return e == hipSuccess ? nb : 0;
+    return reflect ? gfx950_occupancy((const void*)mcxk::batch_kernel<true>, nthreadblock, shmem) :
+           gfx950_occupancy((const void*)mcxk::batch_kernel<false>, nthreadblock, shmem);
+}
+
+/* the kernel attributes the occupancy model reads (VGPRs, static LDS, lane
+   bound), for tests */
+extern "C" int mcx_fast_kernel_attr(int savedet, int reflect, int simple, int* vgpr, int* maxlanes) {
+    hipFuncAttributes a;
+    const void* k = savedet ?
+                    (reflect ? (simple ? (const void*)mcxk::transport_kernel<true, true, true> :
+                                (const void*)mcxk::transport_kernel<true, true, false>) :
+                     (simple ? (const void*)mcxk::transport_kernel<true, false, true> :
+                      (const void*)mcxk::transport_kernel<true, false, false>)) :
+                    (reflect ? (simple ? (const void*)mcxk::transport_kernel<false, true, true> :
+                                (const void*)mcxk::transport_kernel<false, true, false>) :
+                     (simple ? (const void*)mcxk::transport_kernel<false, false, true> :
+                      (const void*)mcxk::transport_kernel<false, false, false>));
+
+    if (hipFuncGetAttributes(&a, k) != hipSuccess) {
+        return -1;
+    }
+
+    *vgpr = a.numRegs;
+    *maxlanes = a.maxThreadsPerBlock;
+    return 0;
 }
 
 /* start of a batched step: zero the grids and energy tallies and restore the

@@ -528,11 +528,23 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
        resident workgroups, given the occupancy the kernel's registers allow */
     if (fast) {
         int ldscu = 0, ldsblock = 0, nwg = 0;
+
+        if (hipSetDevice(dev) != hipSuccess) {
+            MCX_ERROR(-99, "cannot select the HIP device");
+        }
+
         shmem = mcx_kparam_setup(cfg, &g, nthread, lblock, &reflect, 0);
 
         if (hipDeviceGetAttribute(&ldscu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess &&
                 hipDeviceGetAttribute(&ldsblock, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess) {
             nwg = mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, (int)g.simple);
+
+            if (nwg <= 0 && dynamic && !envblock && lblock > MCX_MAX_BLOCK) {
+                /* no CU holds the wide workgroup: fall back to 256 lanes */
+                lblock = MCX_MAX_BLOCK;
+                shmem = mcx_kparam_setup(cfg, &g, nthread, lblock, &reflect, 0);
+                nwg = mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, (int)g.simple);
+            }
         }
 
         size_t budget = 0;
@@ -616,6 +628,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     float2* dmuamus = NULL;
     uint32_t* ddetcount = NULL;
     float* dout = NULL;
+    float* draw32 = NULL;
     size_t nrep = 0;
 
     HIP_TRY(hipSetDevice(dev));
@@ -833,7 +846,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         /* a caller that hands in only the float64 grid (exportfield64 set,
            exportfield NULL) gets no float32 copy: large replay grids are then
            read back once */
-        const int wantfloat = cfg->exportfield != NULL || cfg->exportfield64 == NULL || cfg->parentid == mpStandalone;
+        const int wantfloat = cfg->exportfield != NULL || (cfg->exportfield64 == NULL && cfg->exportfield32 == NULL) ||
+                              cfg->parentid == mpStandalone;
 
         if (wantfloat) {
             HIP_TRY(dalloc(bufs, &dout, fieldlen));
@@ -844,7 +858,18 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             HIP_TRY(mcx_launch_finalize(dfield, NULL, fieldlen, g.nrep, 1.f, stream));
         }
 
+        if (cfg->exportfield32) {
+            /* the replicas are folded by now: a plain float64 -> float32 pass */
+            HIP_TRY(dalloc(bufs, &draw32, fieldlen));
+            HIP_TRY(mcx_launch_finalize(dfield, draw32, fieldlen, 1, 1.f, stream));
+        }
+
         HIP_TRY(hipEventRecord(ev2, stream));
+
+        if (cfg->exportfield32) {
+            HIP_TRY(hipMemcpyAsync(cfg->exportfield32, draw32, fieldlen * sizeof(float), hipMemcpyDeviceToHost,
+                                   stream));
+        }
 
         if (cfg->exportfield64) {
             HIP_TRY(hipMemcpyAsync(cfg->exportfield64, dfield, fieldlen * sizeof(double), hipMemcpyDeviceToHost,

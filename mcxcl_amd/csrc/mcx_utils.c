@@ -2224,7 +2224,7 @@ static const struct {
     F(faststep), F(normalizer), F(replay), F(seeddata), F(replaydet), F(seedfile), F(jsonfile), F(maxjumpdebug),
     F(debugdatalen), F(gscatter), F(exportdebugdata), F(savedetflag), F(detectedcount), F(runtime), F(parentid),
     F(optlevel), F(mediabyte), F(bc), F(nphase), F(invcdf), F(nangle), F(angleinvcdf), F(srcid), F(extrasrclen),
-    F(srcdata), F(mathmode), F(kernelms), F(jobms), F(runtimems), F(exportfield64), F(muamus), F(exportdetweight), F(replayinweight)
+    F(srcdata), F(mathmode), F(kernelms), F(jobms), F(runtimems), F(exportfield64), F(muamus), F(exportdetweight), F(replayinweight), F(exportfield32)
 };
 #undef F
 

@@ -258,10 +258,15 @@ class PreparedConfig:
         self.nsrc = (c.extrasrclen + 1) if (c.extrasrclen and c.srcid < 0) else 1
         self.fieldlen = self.nvox * self.ngate * self.ndet * self.nsrc
         self.raw = None
+        self.raw32 = None
         if user.get("rawfield"):
             # unnormalised double-precision accumulator, returned as "flux64"
             self.raw = np.zeros(self.fieldlen, dtype=np.float64)
             c.exportfield64 = self.raw.ctypes.data
+        if user.get("rawfield32"):
+            # the same accumulator converted to float32 on the device, returned as "flux32"
+            self.raw32 = np.empty(self.fieldlen, dtype=np.float32)
+            c.exportfield32 = self.raw32.ctypes.data
         self._set_detweight(user)
 
     def _set_pervoxel(self, user):
@@ -297,14 +302,17 @@ class PreparedConfig:
         out = _collect(self.c, self.user, self.ngate, self.ndet, self.nsrc)
         if self.detw is not None:
             out["detw"] = self.detw[:self.c.detectedcount].copy()
+        dims = [self.c.dim.x, self.c.dim.y, self.c.dim.z, self.ngate * self.ndet * self.nsrc]
         if self.raw is not None:
-            dims = [self.c.dim.x, self.c.dim.y, self.c.dim.z, self.ngate * self.ndet * self.nsrc]
             out["flux64"] = self.raw.reshape(dims, order="F")   # the array the run filled, no copy
+        if self.raw32 is not None:
+            out["flux32"] = self.raw32.reshape(dims, order="F")
         return out
 
     def close(self):
         if self.c is not None:
             self.c.exportfield64 = None
+            self.c.exportfield32 = None
             self.c.muamus = None
             self.c.exportdetweight = None
             self.c.replayinweight = None
@@ -333,11 +341,13 @@ def run(cfg=None, **kwargs):
     [2, Nx, Ny, Nz] per-voxel mua/mus in 1/mm beside a label ``vol``),
     ``savedetweight`` (return each detected photon's exit weight as ``detw``),
     ``replayweight`` (replay weights per row of ``detphotons``, required with
-    per-voxel media) and ``rawonly`` (with ``rawfield``: return only ``flux64``,
-    skipping the float32 copy of large replay grids)."""
+    per-voxel media), ``rawfield32`` (the same accumulator converted to float32
+    on the device, returned as ``flux32`` at half the read-back bytes) and
+    ``rawonly`` (with ``rawfield`` or ``rawfield32``: return only those,
+    skipping the normalised float32 copy of large replay grids)."""
     with prepare(cfg, **kwargs) as pc:
         c = pc.c
-        if c.issave2pt and not (pc.user.get("rawfield") and pc.user.get("rawonly")):
+        if c.issave2pt and not ((pc.user.get("rawfield") or pc.user.get("rawfield32")) and pc.user.get("rawonly")):
             c.exportfield = _native.cmalloc(4 * pc.fieldlen)
         rc = lib.mcx_try_run_simulation(ctypes.byref(c))
         if rc:

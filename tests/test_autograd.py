@@ -20,7 +20,11 @@ NT = 4096
 
 
 def _oracle_run(cfg):
-    return oracle.run(dict(cfg, autopilot=0, nthread=NT, nblocksize=64), nthread=NT)
+    out = oracle.run(dict(cfg, autopilot=0, nthread=NT, nblocksize=64, rawfield=1), nthread=NT)
+    if cfg.get("rawfield32"):
+        # the device converts its float64 accumulator to float32 the same way
+        out["flux32"] = np.asfortranarray(out["flux64"].astype(np.float32))
+    return out
 
 
 @pytest.fixture

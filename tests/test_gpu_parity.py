@@ -219,3 +219,38 @@ def test_pervoxel_media_parity():
     a = np.vstack([gpu["detp"], gpu["detw"][None, :]])
     b = np.vstack([ref["detp"], ref["detw"][None, :]])
     assert np.array_equal(sorted_records(a), sorted_records(b))
+
+
+def test_float32_raw_export_is_the_converted_accumulator():
+    """Config.exportfield32: the float64 accumulator converted on the device
+    (round to nearest) equals the float64 read-back rounded on the host, with
+    and without the normalised float32 copy beside it"""
+    base = workloads.sphere_cube_workload(nphoton=200000, issaveseed=1, savedetflag="DP", tstep=5e-9)
+    fwd = mcxcl_amd.run(dict(base, mathmode=0))
+    rep = dict(base, seed=fwd["seeds"], detphotons=fwd["detp"], outputtype="jacobian", replaydet=-1, mathmode=0)
+    both = mcxcl_amd.run(dict(rep, rawfield=1, rawfield32=1))
+    assert both["flux32"].dtype == np.float32 and both["flux32"].shape == both["flux64"].shape
+    assert np.array_equal(both["flux32"], both["flux64"].astype(np.float32))
+    assert "flux" in both
+    only = mcxcl_amd.run(dict(rep, rawfield32=1, rawonly=1))
+    assert "flux" not in only and "flux64" not in only
+    assert only["flux32"].shape == both["flux32"].shape and only["flux32"].sum() > 0
+
+
+def test_launch_shape_model():
+    """the fast path's launch-shape model: the pencil-beam variant fits in 128
+    VGPRs, so one 1024-lane workgroup (16 waves) fits a CU and four 256-lane
+    ones do; with 150 KB of LDS only one workgroup fits"""
+    import ctypes
+    lib = _native.lib
+    lib.mcx_fast_kernel_attr.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_int)] * 2
+    lib.mcx_fast_occupancy.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    vgpr, lanes = ctypes.c_int(0), ctypes.c_int(0)
+    mcxcl_amd.run(workloads.sphere_cube_workload(nphoton=10000))          # device and code object loaded
+    assert lib.mcx_fast_kernel_attr(1, 1, 1, ctypes.byref(vgpr), ctypes.byref(lanes)) == 0
+    print("pencil-beam variant: %d VGPRs, %d lanes at most" % (vgpr.value, lanes.value), flush=True)
+    assert 32 <= vgpr.value <= 128 and lanes.value >= 1024
+    assert lib.mcx_fast_occupancy(1024, 20000, 1, 1, 1) == 1
+    assert lib.mcx_fast_occupancy(256, 20000, 1, 1, 1) == 4
+    assert lib.mcx_fast_occupancy(1024, 150 * 1024, 1, 1, 1) == 1
+    assert lib.mcx_fast_occupancy(256, 150 * 1024, 1, 1, 1) == 1
