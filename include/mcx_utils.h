@@ -339,6 +339,10 @@ void mcx_loadbenchmark(char* key, Config* cfg);                             /* s
 void mcx_prepdomain(char* filename, Config* cfg);                           /* src/mcx_utils.h:305 */
 void mcx_preprocess(Config* cfg);                                           /* src/mcx_utils.c:1521 (not in the reference header) */
 void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte); /* src/mcx_utils.h:337 */
+void mcx_append_detector_sources(Config* cfg);                              /* the detector-as-source append of
+                                                                               src/mcx_utils.c:1902-1925 and
+                                                                               src/pmcxcl.cpp:1179-1198 (inline there),
+                                                                               for bindings after mcx_validatecfg */
 void mcx_replayinit(Config* cfg, float* detps, int dimdetps[2], int seedbyte); /* src/mcx_utils.h:336 */
 void mcx_maskdet(Config* cfg);                                              /* src/mcx_utils.h:304 */
 void mcx_normalize(float field[], float scale, int fieldlen, int option, int pidx, int srcnum); /* src/mcx_utils.h:300 */

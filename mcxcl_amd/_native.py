@@ -169,6 +169,7 @@ assert ctypes.sizeof(Config) == lib.mcx_config_sizeof()
 
 _P = ctypes.POINTER
 lib.mcx_initcfg.argtypes = [_P(Config)]
+lib.mcx_append_detector_sources.argtypes = [_P(Config)]
 lib.mcx_initcfg.restype = None
 lib.mcx_clearcfg.argtypes = [_P(Config)]
 lib.mcx_clearcfg.restype = None

@@ -375,8 +375,10 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
             MCX_ERROR(-4, "the batched launch runs the fast arithmetic only (mathmode 0)");
         }
 
-        if (cfg->issavedet || cfg->seed == SEED_FROM_FILE || (cfg->debuglevel & MCX_DEBUG_RNG) || cfg->muamus) {
-            MCX_ERROR(-4, "the batched launch takes no detected photons, replay, RNG dump or per-voxel media");
+        if (cfg->issavedet || cfg->seed == SEED_FROM_FILE || (cfg->debuglevel & MCX_DEBUG_RNG) || cfg->muamus ||
+                MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
+            MCX_ERROR(-4, "the batched launch takes no detected photons, replay, RNG dump, per-voxel media or adjoint "
+                      "outputs");
         }
 
         if (cfg->nphoton >= 0xFFFFFFFFull) {

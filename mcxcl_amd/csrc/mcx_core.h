@@ -34,6 +34,8 @@ struct MCXKParam {
     uint32_t issaveseed, issaveref, isspecular, maxgate, outputtype, threadphoton, savedetflag, reclen;
     uint32_t partialdata, w0offset, gscatter, is2d, srcnum, doreflect, save2pt, savedet, nphase, extrasrclen;
     uint32_t nthread, debugrng, mediaformat;
+    uint32_t adjsrc;              /* adjoint outputs: sources 1..adjsrc are the user's, the rest are
+                                     detectors launched from a disk (0: no adjoint output) */
     int32_t seed, oddphoton, replaydet, srcid;
     uint32_t fieldlen;
     /* deposit privatisation: the box [x0,x0+nx)x[y0,y0+ny)x[z0,z0+nz) of the first ng
@@ -87,6 +89,11 @@ hipError_t mcx_launch_fast(const MCXKParam* prm, int nblock, int nthreadblock, s
                            int savedet, int reflect);
 /* RNG stream dump: field[i] for i = idx; i < fieldlen; i += nthread */
 /* resident workgroups per CU of the fast transport kernel variant for this launch shape */
+/* adjoint outputs from the normalised per-source field (kind 0: phi_s phi_d,
+   kind 1: grad phi_s . grad phi_d), scaled, optionally by a per-label factor */
+hipError_t mcx_launch_adjoint(const float* field, float* out, uint32_t nvox, uint32_t nx, uint32_t ny, uint32_t maxgate,
+                              uint32_t ns, uint32_t nd, int kind, float scale, const float* opscale,
+                              const uint32_t* media, hipStream_t stream);
 int mcx_fast_occupancy(int nthreadblock, size_t shmem, int savedet, int reflect, int simple);
 hipError_t mcx_launch_atomic_probe(double* grid, uint32_t n, uint32_t nblock, uint32_t k, hipStream_t s);
 hipError_t mcx_launch_rngdump(const MCXKParam* prm, int nblock, int nthreadblock, hipStream_t stream);

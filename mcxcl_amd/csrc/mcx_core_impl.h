@@ -638,6 +638,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
     s.seed0 = rng.s0;
     s.seed1 = rng.s1;
     float4 lpos = g.srcpos, ldir = g.srcdir, lp1 = g.srcparam1, lp2 = g.srcparam2;
+    int cursid = 1;
 
     if (g.extrasrclen && g.srcid != 1) {
         int sid;
@@ -648,6 +649,8 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
             pp[g.w0offset - 1] = (float)((int)(rng.next() * K_JUST_BELOW_ONE * (float)(g.extrasrclen + 1)) + 1);
             sid = (int)pp[g.w0offset - 1];
         }
+
+        cursid = sid;
 
         if (sid > 1) {
             const float4* es = g.srcdata + (sid - 2) * 4;
@@ -864,6 +867,33 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
                 v.x *= Rn2;
                 v.y *= Rn2;
                 v.z *= Rn2;
+            }
+        }
+
+        /* adjoint outputs: a detector source launches from a disk of the detector's
+           radius (srcparam1.x) across its direction (src/mcx_core.cl:2159-2184) */
+        if (!SIMPLE && g.adjsrc && cursid > (int)g.adjsrc) {
+            float sphi, cphi;
+            ksincos(K_TWO_PI * rng.next(), &sphi, &cphi);
+            const float r = ksqrt(rng.next()) * lp1.x;
+
+            if (v.z > -1.f + K_EPS && v.z < 1.f - K_EPS) {
+                const float t0 = 1.f - v.z * v.z;
+                const float t1 = r * krsqrt(t0);
+                p.x += t1 * (v.x * v.z * cphi - v.y * sphi);
+                p.y += t1 * (v.y * v.z * cphi + v.x * sphi);
+                p.z -= t1 * t0 * cphi;
+            } else {
+                p.x += r * cphi;
+                p.y += r * sphi;
+            }
+
+            s.idx1d = (uint32_t)((int)floorf(p.z) * (int)g.dimxy + (int)floorf(p.y) * (int)g.dimx + (int)floorf(p.x));
+
+            if (p.x < 0.f || p.y < 0.f || p.z < 0.f || p.x >= g.maxidx_x || p.y >= g.maxidx_y || p.z >= g.maxidx_z) {
+                s.mediaid = 0;
+            } else {
+                s.mediaid = g.media[s.idx1d];
             }
         }
 

@@ -80,6 +80,115 @@ extern "C" hipError_t mcx_launch_finalize(double* in, float* out, size_t n, unsi
     return hipGetLastError();
 }
 
+/* ---- adjoint (phi_src . phi_det) outputs (src/mcx_core.cl:3313-3512) ----
+   The normalised float field holds one slot of maxgate gates per source
+   (user sources first, then one per detector) at field[vox + (t + slot *
+   maxgate) * nvox].  Per voxel and per (source s, detector d) pair:
+     kind 0: sum_t phi_s * sum_t phi_d
+     kind 1: grad(sum_t phi_s) . grad(sum_t phi_d), second-order differences
+             in voxel units (one-sided at the faces, first order on 2-voxel axes)
+   times scale, then times opscale[label] when opscale is given.  Built
+   without contraction: the gate sums run in gate order and every product and
+   sum is rounded as written, so the CPU restatement matches bit for bit. */
+__device__ __forceinline__ float cw_sum(const float* __restrict__ f, uint32_t vox, uint32_t slot, uint32_t maxgate,
+                                        uint32_t nvox) {
+    float acc = 0.f;
+
+    for (uint32_t t = 0; t < maxgate; t++) {
+        acc += f[vox + (size_t)(t + slot * maxgate) * nvox];
+    }
+
+    return acc;
+}
+
+__device__ __forceinline__ float fd_grad(const float* __restrict__ f, uint32_t vox, uint32_t slot, uint32_t maxgate,
+        uint32_t nvox, uint32_t i, uint32_t n, uint32_t stride) {
+    if (n <= 1) {
+        return 0.f;
+    }
+
+    const float f0 = cw_sum(f, vox, slot, maxgate, nvox);
+
+    if (i == 0) {
+        const float p1 = cw_sum(f, vox + stride, slot, maxgate, nvox);
+
+        if (n == 2) {
+            return p1 - f0;
+        }
+
+        const float p2 = cw_sum(f, vox + 2 * stride, slot, maxgate, nvox);
+        return (-3.f * f0 + 4.f * p1 - p2) * 0.5f;
+    }
+
+    if (i == n - 1) {
+        const float m1 = cw_sum(f, vox - stride, slot, maxgate, nvox);
+
+        if (n == 2) {
+            return f0 - m1;
+        }
+
+        const float m2 = cw_sum(f, vox - 2 * stride, slot, maxgate, nvox);
+        return (m2 - 4.f * m1 + 3.f * f0) * 0.5f;
+    }
+
+    return (cw_sum(f, vox + stride, slot, maxgate, nvox) - cw_sum(f, vox - stride, slot, maxgate, nvox)) * 0.5f;
+}
+
+__global__ void adjoint_kernel(const float* __restrict__ field, float* __restrict__ out, uint32_t nvox, uint32_t nx,
+                               uint32_t ny, uint32_t maxgate, uint32_t ns, uint32_t nd, int kind, float scale,
+                               const float* __restrict__ opscale, const uint32_t* __restrict__ media) {
+    const uint32_t vox = blockIdx.x * blockDim.x + threadIdx.x;
+
+    if (vox >= nvox) {
+        return;
+    }
+
+    const uint32_t nxy = nx * ny, nz = nvox / nxy;
+    const uint32_t ix = vox % nx, iy = (vox / nx) % ny, iz = vox / nxy;
+    const float op = opscale ? opscale[media[vox] & 0xFFu] : 1.f;
+
+    for (uint32_t s = 0; s < ns; s++) {
+        float sx = 0.f, sy = 0.f, sz = 0.f, sc = 0.f;
+
+        if (kind == 0) {
+            sc = cw_sum(field, vox, s, maxgate, nvox);
+        } else {
+            sx = fd_grad(field, vox, s, maxgate, nvox, ix, nx, 1);
+            sy = fd_grad(field, vox, s, maxgate, nvox, iy, ny, nx);
+            sz = fd_grad(field, vox, s, maxgate, nvox, iz, nz, nxy);
+        }
+
+        for (uint32_t d = 0; d < nd; d++) {
+            float v;
+
+            if (kind == 0) {
+                v = sc * cw_sum(field, vox, ns + d, maxgate, nvox);
+            } else {
+                const float dx = fd_grad(field, vox, ns + d, maxgate, nvox, ix, nx, 1);
+                const float dy = fd_grad(field, vox, ns + d, maxgate, nvox, iy, ny, nx);
+                const float dz = fd_grad(field, vox, ns + d, maxgate, nvox, iz, nz, nxy);
+                v = sx * dx + sy * dy + sz * dz;
+            }
+
+            v = v * scale;
+
+            if (opscale) {
+                v = v * op;
+            }
+
+            out[vox + (size_t)(s * nd + d) * nvox] = v;
+        }
+    }
+}
+
+extern "C" hipError_t mcx_launch_adjoint(const float* field, float* out, uint32_t nvox, uint32_t nx, uint32_t ny,
+        uint32_t maxgate, uint32_t ns, uint32_t nd, int kind, float scale,
+        const float* opscale, const uint32_t* media, hipStream_t stream) {
+    hipLaunchKernelGGL(adjoint_kernel, dim3((nvox + 255) / 256), dim3(256), 0, stream, field, out, nvox, nx, ny, maxgate,
+                       ns, nd, kind, scale, opscale, media);
+    return hipGetLastError();
+}
+
 /* ---- device self-checks: math in parity mode, raw RNG stream ---- */
 __global__ void fpmath_kernel(int which, const float* __restrict__ x, float* __restrict__ y, size_t n) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;

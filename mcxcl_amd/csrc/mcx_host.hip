@@ -294,7 +294,11 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     g.issaveref = (uint32_t)cfg->issaveref;
     g.isspecular = cfg->isspecular > 0;
     g.maxgate = maxgate;
-    g.outputtype = (uint32_t)cfg->outputtype;
+    /* the adjoint outputs deposit fluence per source; their products are formed
+       after the run (src/mcx_core.cl:2843) */
+    g.outputtype = MCX_IS_ADJOINT_TYPE(cfg->outputtype) ? (uint32_t)otFluence : (uint32_t)cfg->outputtype;
+    g.adjsrc = (MCX_IS_ADJOINT_TYPE(cfg->outputtype) && cfg->seed != SEED_FROM_FILE && cfg->detdir && cfg->detnum &&
+                cfg->extrasrclen + 1 > cfg->detnum) ? cfg->extrasrclen + 1 - cfg->detnum : 0;
     g.savedetflag = cfg->savedetflag;
     g.reclen = reclen;
     g.partialdata = partialdata;
@@ -629,6 +633,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     uint32_t* ddetcount = NULL;
     float* dout = NULL;
     float* draw32 = NULL;
+    float* djacob = NULL;
+    float* dopscale = NULL;
     size_t nrep = 0;
 
     HIP_TRY(hipSetDevice(dev));
@@ -811,10 +817,10 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
 
         if (normalize) {
-            if (cfg->outputtype == otFlux || cfg->outputtype == otFluence) {
+            if (cfg->outputtype == otFlux || cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
                 scale = cfg->unitinmm / (float)(cfg->energytot * Vvox * cfg->tstep);
 
-                if (cfg->outputtype == otFluence) {
+                if (cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
                     scale *= cfg->tstep;
                 }
             } else if (cfg->outputtype == otEnergy || cfg->outputtype == otL) {
@@ -856,6 +862,62 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         } else if (g.nrep > 1) {
             /* fold the replicas into replica 0 without a float output */
             HIP_TRY(mcx_launch_finalize(dfield, NULL, fieldlen, g.nrep, 1.f, stream));
+        }
+
+        /* adjoint outputs (src/mcx_host.cpp:1467-1642): products of the normalised
+           per-source fluences of each user source s and detector-as-source d,
+           formed on the device from the float field, scaled by -Vvox (mua) or
+           -unitinmm (D, with 1/(3(1-g)mus^2) or 1/(3(1-g)^2 mus^2) per label for
+           mus and mus'), into cfg->exportjacob [Nvox, Ns*Nd] (a second block for
+           the two-output types) */
+        if (wantfloat && MCX_IS_ADJOINT_TYPE(cfg->outputtype) && g.adjsrc) {
+            const uint32_t ns = g.adjsrc, nd = cfg->detnum;
+            const size_t adjlen = nvox * ns * nd;
+            const int dual = MCX_IS_DUAL_ADJOINT_TYPE(cfg->outputtype);
+            const float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
+            const int musp = cfg->outputtype == otAdjointMusp || cfg->outputtype == otAdjointMuaMusp;
+            const int needop = musp || cfg->outputtype == otAdjointMus;
+            float hop[256];
+
+            for (int m = 0; m < 256; m++) {
+                hop[m] = 0.f;
+
+                if (needop && (unsigned int)m < cfg->medianum) {
+                    const float mus = cfg->prop[m].mus, onemg = 1.f - cfg->prop[m].g;
+
+                    if (mus > 0.f && onemg > 0.f) {
+                        hop[m] = musp ? 1.f / (3.f * onemg * onemg * mus * mus) : 1.f / (3.f * onemg * mus * mus);
+                    }
+                }
+            }
+
+            HIP_TRY(dalloc(bufs, &djacob, adjlen * (dual ? 2 : 1)));
+
+            if (needop) {
+                HIP_TRY(dalloc(bufs, &dopscale, 256));
+                HIP_TRY(hipMemcpyAsync(dopscale, hop, sizeof(hop), hipMemcpyHostToDevice, stream));
+            }
+
+            if (dual || cfg->outputtype == otAdjoint) {
+                HIP_TRY(mcx_launch_adjoint(dout, djacob, (uint32_t)nvox, cfg->dim.x, cfg->dim.y, maxgate, ns, nd, 0, -Vvox,
+                                           NULL, dmedia, stream));
+            }
+
+            if (dual || cfg->outputtype != otAdjoint) {
+                HIP_TRY(mcx_launch_adjoint(dout, djacob + (dual ? adjlen : 0), (uint32_t)nvox, cfg->dim.x, cfg->dim.y,
+                                           maxgate, ns, nd, 1, -cfg->unitinmm, needop ? dopscale : NULL, dmedia,
+                                           stream));
+            }
+
+            free(cfg->exportjacob);
+            cfg->exportjacob = (float*)malloc(adjlen * (dual ? 2 : 1) * sizeof(float));
+
+            if (!cfg->exportjacob) {
+                HIP_TRY(hipErrorOutOfMemory);
+            }
+
+            HIP_TRY(hipMemcpyAsync(cfg->exportjacob, djacob, adjlen * (dual ? 2 : 1) * sizeof(float),
+                                   hipMemcpyDeviceToHost, stream));
         }
 
         if (cfg->exportfield32) {

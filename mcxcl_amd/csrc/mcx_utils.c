@@ -1863,6 +1863,36 @@ void mcx_loadseedjdat(char* filename, Config* cfg) {
 }
 
 /* ---------------- validation (reference src/mcx_utils.c:1822-1964) ---------------- */
+/* append the detectors as extra sources of the adjoint outputs: position and
+   direction of each detector, unit weight, disk radius detpos.w in srcparam1.x;
+   all sources then launch in one run with one field slot each (srcid -1) */
+void mcx_append_detector_sources(Config* cfg) {
+    const unsigned int n0 = cfg->extrasrclen;
+    ExtraSrc* sd = (ExtraSrc*)realloc(cfg->srcdata, (n0 + cfg->detnum) * sizeof(ExtraSrc));
+
+    if (!sd) {
+        MCX_ERROR(-1, "cannot allocate the detector sources");
+    }
+
+    cfg->srcdata = sd;
+    memset(sd + n0, 0, cfg->detnum * sizeof(ExtraSrc));
+
+    for (unsigned int i = 0; i < cfg->detnum; i++) {
+        sd[n0 + i].srcpos.x = cfg->detpos[i].x;
+        sd[n0 + i].srcpos.y = cfg->detpos[i].y;
+        sd[n0 + i].srcpos.z = cfg->detpos[i].z;
+        sd[n0 + i].srcpos.w = 1.f;
+        memcpy(&sd[n0 + i].srcdir, cfg->detdir + i, sizeof(sd[n0 + i].srcdir));
+        sd[n0 + i].srcparam1.x = cfg->detpos[i].w;
+    }
+
+    cfg->extrasrclen += cfg->detnum;
+
+    if (MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
+        cfg->srcid = -1;
+    }
+}
+
 void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
     if (!cfg->issrcfrom0) {
         cfg->srcpos.x--;
@@ -1904,9 +1934,19 @@ void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
         MCX_ERROR(-6, "Jacobian output is only valid in the reply mode. Please define cfg.seed");
     }
 
-    if (cfg->outputtype == otRF || cfg->outputtype == otRFmus || MCX_IS_ADJOINT_TYPE(cfg->outputtype) ||
-            cfg->outputtype == otWLTOF || cfg->outputtype == otWPTOF || cfg->omega > 0.f) {
-        MCX_ERROR(-6, "RF, adjoint and time-weighted replay outputs are not supported by this build");
+    if (cfg->outputtype == otRF || cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
+            cfg->outputtype == otWPTOF || cfg->omega > 0.f || cfg->srcid == -2) {
+        MCX_ERROR(-6, "RF, time-weighted replay outputs and srcid -2 are not supported by this build");
+    }
+
+    /* adjoint outputs: every detector becomes an extra disk source at its
+       position and direction, radius detpos.w, and the field is stacked per
+       source (src/mcx_utils.c:1902-1925).  A binding does this itself after
+       validation (src/pmcxcl.cpp:1179-1198), so only a stand-alone run does it
+       here. */
+    if (cfg->parentid == mpStandalone && MCX_IS_ADJOINT_TYPE(cfg->outputtype) && cfg->seed != SEED_FROM_FILE &&
+            cfg->detnum > 0 && cfg->detdir != NULL) {
+        mcx_append_detector_sources(cfg);
     }
 
     for (unsigned int i = 0; i < cfg->detnum; i++) {

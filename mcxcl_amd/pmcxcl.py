@@ -24,6 +24,10 @@ OUTPUTTYPES = ["flux", "fluence", "energy", "jacobian", "nscat", "wl", "wp", "wm
                "wltof", "wptof", "adjoint", "adjoint_dcoeff", "adjoint_mus", "adjoint_musp", "adjoint_mua_d",
                "adjoint_mua_musp"]
 SAVEFLAGS = "DSPMXVWI"
+# first adjoint output type (enum TOutputType, src/mcx_utils.h:58-62) and the
+# output keys of each adjoint type (src/pmcxcl.cpp:1426-1456)
+OT_ADJOINT = 11
+ADJOINT_KEYS = {11: ("jmua",), 12: ("jd",), 13: ("jmus",), 14: ("jmusp",), 15: ("jmua", "jd"), 16: ("jmua", "jmusp")}
 DEBUGFLAGS = "RMP"
 SEED_FROM_FILE = -999
 
@@ -246,6 +250,10 @@ class PreparedConfig:
                 # the reference binding turns errors of the C layer into RuntimeError
                 # (src/pmcxcl.cpp:1523-1525)
                 raise RuntimeError(_native.last_error())
+            c0 = self.c
+            if (c0.outputtype[0] >= OT_ADJOINT and c0.seed != SEED_FROM_FILE and c0.detnum > 0 and c0.detdir):
+                # adjoint outputs: detectors become disk sources (src/pmcxcl.cpp:1179-1198)
+                lib.mcx_append_detector_sources(ctypes.byref(c0))
             if not self.c.vol or self.c.medianum == 0:
                 raise ValueError("You must define 'vol' and 'prop' field.")
         except Exception:
@@ -460,6 +468,15 @@ def _collect(c, user, ngate, ndet, nsrc):
             out["dref"] = dref.reshape(dims, order="F")
             flux = fl.reshape(dims, order="F")
         out["flux"] = flux
+    ot = c.outputtype[0]
+    if ot >= OT_ADJOINT and c.exportjacob and c.detnum and c.extrasrclen + 1 > c.detnum:
+        # adjoint Jacobians [Nx, Ny, Nz, Ns*Nd], pair s*Nd+d (src/pmcxcl.cpp:1414-1491)
+        npair = (c.extrasrclen + 1 - c.detnum) * c.detnum
+        adjlen = c.dim.x * c.dim.y * c.dim.z * npair
+        keys = ADJOINT_KEYS.get(ot, ())
+        buf = np.ctypeslib.as_array((ctypes.c_float * (adjlen * len(keys))).from_address(c.exportjacob)).copy()
+        for k, name in enumerate(keys):
+            out[name] = buf[k * adjlen:(k + 1) * adjlen].reshape((c.dim.x, c.dim.y, c.dim.z, npair), order="F")
     out["stat"] = {
         "runtime": int(c.runtime),
         "nphoton": int(c.nphoton) * (c.respin if c.respin > 1 else 1),

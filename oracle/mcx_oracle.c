@@ -99,6 +99,7 @@ typedef struct {
     uint32_t maxdetphoton, maxmedia, detnum, voidtime, srctype, maxvoidstep, issaveseed, issaveref, isspecular;
     uint32_t maxgate, outputtype, threadphoton, savedetflag, reclen, partialdata, w0offset, mediaformat, gscatter;
     uint32_t is2d, srcnum, doreflect, save2pt, savedet, nphase, debuglevel, extrasrclen;
+    uint32_t adjsrc;   /* adjoint outputs: sources 1..adjsrc are the user's, the rest are detectors */
     int32_t seed, oddphoton, replaydet, srcid;
     int reflectblock;  /* the reference compiles its reflection block under MCX_DO_REFLECTION */
     unsigned char bc[12];
@@ -498,6 +499,8 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
     s->photonseed[0] = s->t[0];
     s->photonseed[1] = s->t[1];
 
+    int cursid = 1;
+
     if (g->extrasrclen && g->srcid != 1) {
         int sid;
 
@@ -507,6 +510,8 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
             s->pp[g->w0offset - 1].f = (float)((int)(rng_next(s->t, w) * JUST_BELOW_ONE * (float)(g->extrasrclen + 1)) + 1);
             sid = (int)s->pp[g->w0offset - 1].f;
         }
+
+        cursid = sid;
 
         if (sid > 1) {
             const ExtraSrc* es = g->srcdata + (sid - 2);
@@ -735,6 +740,34 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
             }
         }
 
+        /* adjoint outputs: a detector source launches from a disk of the detector's
+           radius (srcparam1.x) across its direction (src/mcx_core.cl:2159-2184) */
+        if (g->adjsrc && cursid > (int)g->adjsrc) {
+            float phi = TWO_PI * rng_next(s->t, w), sphi, cphi;
+            pm_sincos(phi, &sphi, &cphi);
+            float r = pm_sqrt(rng_next(s->t, w)) * lp1.x;
+
+            if (v->z > -1.f + K_EPS && v->z < 1.f - K_EPS) {
+                float t0 = 1.f - v->z * v->z;
+                float t1 = r * pm_rsqrt(t0);
+                p->x += t1 * (v->x * v->z * cphi - v->y * sphi);
+                p->y += t1 * (v->y * v->z * cphi + v->x * sphi);
+                p->z -= t1 * t0 * cphi;
+            } else {
+                p->x += r * cphi;
+                p->y += r * sphi;
+            }
+
+            s->idx1d = (uint32_t)((int)floorf(p->z) * (int)g->dimxy + (int)floorf(p->y) * (int)g->dimx + (int)floorf(p->x));
+
+            if (p->x < 0.f || p->y < 0.f || p->z < 0.f || p->x >= g->maxidx.x || p->y >= g->maxidx.y ||
+                    p->z >= g->maxidx.z) {
+                s->mediaid = 0;
+            } else {
+                s->mediaid = g->media[s->idx1d];
+            }
+        }
+
         if ((s->mediaid & MED_MASK) == 0) {
             int id = skipvoid(g, p, v, f, fd);
 
@@ -929,7 +962,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
 
                 if (g->outputtype == otEnergy) {
                     weight = s->w0 - p->w;
-                } else if (g->outputtype == otFluence || g->outputtype == otFlux) {
+                } else if (g->outputtype == otFluence || g->outputtype == otFlux || MCX_IS_ADJOINT_TYPE(g->outputtype)) {
                     weight = (s->prop.x < K_EPS) ? (s->w0 * pathlen) : ((s->w0 - p->w) / s->prop.x);
                 } else if (g->seed == SEED_FROM_FILE) {
                     if (g->outputtype == otJacobian) {
@@ -1180,6 +1213,8 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
     g.invcdf = cfg->invcdf;
     g.debuglevel = cfg->debuglevel;
     g.extrasrclen = cfg->extrasrclen;
+    g.adjsrc = (MCX_IS_ADJOINT_TYPE(cfg->outputtype) && cfg->seed != SEED_FROM_FILE && cfg->detdir && cfg->detnum &&
+                cfg->extrasrclen + 1 > cfg->detnum) ? cfg->extrasrclen + 1 - cfg->detnum : 0;
     g.srcid = cfg->srcid;
     g.srcdata = cfg->srcdata;
     g.replaydet = cfg->replaydet;
@@ -1384,10 +1419,10 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
             float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
             float scale = 1.f;
 
-            if (cfg->outputtype == otFlux || cfg->outputtype == otFluence) {
+            if (cfg->outputtype == otFlux || cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
                 scale = cfg->unitinmm / (float)(cfg->energytot * Vvox * cfg->tstep);
 
-                if (cfg->outputtype == otFluence) {
+                if (cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
                     scale *= cfg->tstep;
                 }
             } else if (cfg->outputtype == otEnergy || cfg->outputtype == otL) {

@@ -83,9 +83,97 @@ def run(user, nthread, threadoffset=0, threadcount=0, nomp=0):
                 (ctypes.c_float * (2 * res.nthread)).from_address(res.genergy)).copy()
             out["counters"] = {k: getattr(res.cnt, k) for k, _ in OracleCounters._fields_}
             out["seconds"] = res.seconds
+            out.update(adjoint_outputs(pc.c))
         finally:
             L.oracle_free_result(ctypes.byref(res))
     return out
+
+
+def _cw_sum(field, slot, maxgate):
+    """sum over the gates of one source slot, in gate order (float32)"""
+    acc = np.zeros(field.shape[0], np.float32)
+    for t in range(maxgate):
+        acc = acc + field[:, slot * maxgate + t]
+    return acc
+
+
+def _fd_grad(c3, axis):
+    """second-order differences along one axis in voxel units: central inside,
+    one-sided at the faces, first order on a 2-voxel axis, 0 on a 1-voxel axis
+    (src/mcx_core.cl:3344-3382), float32 in the reference's operation order"""
+    n = c3.shape[axis]
+    g = np.zeros_like(c3)
+    if n <= 1:
+        return g
+    a = np.moveaxis(c3, axis, 0)
+    r = np.moveaxis(g, axis, 0)
+    h = np.float32(0.5)
+    if n == 2:
+        r[0] = a[1] - a[0]
+        r[1] = a[1] - a[0]
+        return g
+    r[0] = (np.float32(-3.0) * a[0] + np.float32(4.0) * a[1] - a[2]) * h
+    r[-1] = (a[-3] - np.float32(4.0) * a[-2] + np.float32(3.0) * a[-1]) * h
+    r[1:-1] = (a[2:] - a[:-2]) * h
+    return g
+
+
+def adjoint_outputs(c):
+    """The adjoint outputs from the oracle's normalised per-source field
+    (src/mcx_core.cl:3399-3512 and their scaling, src/mcx_host.cpp:1467-1642),
+    restated in numpy float32: {key: [Nx, Ny, Nz, Ns*Nd]} with pair s*Nd + d."""
+    ot = c.outputtype[0]
+    if ot < pmcxcl.OT_ADJOINT or not c.exportfield or not c.detnum or c.extrasrclen + 1 <= c.detnum:
+        return {}
+    dims = (c.dim.x, c.dim.y, c.dim.z)
+    nvox = int(np.prod(dims))
+    maxgate = int((c.tend - c.tstart) / c.tstep + 0.5)
+    ns, nd = c.extrasrclen + 1 - c.detnum, c.detnum
+    field = np.ctypeslib.as_array((ctypes.c_float * (nvox * maxgate * (ns + nd))).from_address(c.exportfield))
+    field = field.reshape((nvox, -1), order="F")
+    cw = [_cw_sum(field, k, maxgate) for k in range(ns + nd)]
+    grads = [[_fd_grad(v.reshape(dims, order="F"), ax).reshape(-1, order="F") for ax in range(3)] for v in cw]
+    vvox = np.float32(c.steps.x) * np.float32(c.steps.y) * np.float32(c.steps.z)
+    prop = np.ctypeslib.as_array((ctypes.c_float * (4 * c.medianum)).from_address(c.prop)).reshape(-1, 4)
+    labels = np.ctypeslib.as_array((ctypes.c_uint32 * nvox).from_address(c.vol)) & 0xFF
+
+    def opscale(musp):
+        op = np.zeros(256, np.float32)
+        for m in range(c.medianum):
+            mus, onemg = np.float32(prop[m, 1]), np.float32(1.0) - np.float32(prop[m, 2])
+            if mus > 0 and onemg > 0:
+                den = np.float32(3.0) * onemg * onemg * mus * mus if musp else np.float32(3.0) * onemg * mus * mus
+                op[m] = np.float32(1.0) / den
+        return op[labels]
+
+    def block(kind, scale, op=None):
+        out = np.empty((nvox, ns * nd), np.float32)
+        for s in range(ns):
+            for d in range(nd):
+                if kind == 0:
+                    v = cw[s] * cw[ns + d]
+                else:
+                    gs, gd = grads[s], grads[ns + d]
+                    v = gs[0] * gd[0] + gs[1] * gd[1] + gs[2] * gd[2]
+                v = v * np.float32(scale)
+                if op is not None:
+                    v = v * op
+                out[:, s * nd + d] = v
+        return out.reshape(dims + (ns * nd,), order="F")
+
+    mua = lambda: block(0, -vvox)
+    grad = lambda op=None: block(1, -np.float32(c.unitinmm), op)
+    if ot == 11:
+        return {"jmua": mua()}
+    if ot == 12:
+        return {"jd": grad()}
+    if ot == 13:
+        return {"jmus": grad(opscale(False))}
+    if ot == 14:
+        return {"jmusp": grad(opscale(True))}
+    if ot == 15:
+        return {"jmua": mua(), "jd": grad()}
+    return {"jmua": mua(), "jmusp": grad(opscale(True))}
 
 
 def glibc_rand(seed, n):
