@@ -21,6 +21,8 @@ The JSON line also carries:
                 four-layer volume against central finite differences
   inverse       16 sources x 16 detectors on 128^3 per-voxel media: forward,
                 folded backward, all 256 signals' gradients, finite-difference check
+  phiphi        the phi.phi adjoint outputs: 4 sources and 4 detectors on 128^3,
+                run time and the device post-processing's bytes per second
   batch         256 independent 64^3 simulations in one persistent launch,
                 against the same instances run one call each
 
@@ -231,6 +233,49 @@ def inverse_leg(photons_per_source, nfd, seed=0):
         os.environ.pop("MCXHIP_DYNAMIC", None)
 
 
+def phiphi_leg(photons, nrep=3):
+    """The phi.phi adjoint outputs (outputtype adjoint_mua_d): a 128^3
+    homogeneous medium (mua 0.005/mm, mus 1/mm, g 0.8, n 1.37), 4 pencil
+    sources and 4 detectors (r 2 mm, facing into the medium) on the z=0 face,
+    all 8 launched in one run (srcid -1).  Times the run and the device work
+    after the transport (jobms - kernelms: the float field of the 8 sources and
+    the two adjoint kernels), and states that work's algorithmic bytes: the
+    field written once and read by each kernel, the 2 x 16 pair products
+    written."""
+    import mcxcl_amd
+
+    dim = 128
+    pos = [[32.5, 32.5], [95.5, 32.5], [32.5, 95.5], [95.5, 95.5]]
+    cfg = {
+        "nphoton": int(photons), "seed": 1648335518, "vol": np.ones((dim, dim, dim), np.uint8),
+        "prop": [[0, 0, 1, 1], [0.005, 1.0, 0.8, 1.37]], "tstart": 0, "tend": 5e-9, "tstep": 5e-9,
+        "srcpos": [[x, y, 0] for x, y in pos], "srcdir": [[0, 0, 1, 0]] * 4,
+        "detpos": [[x + 16, y + 16, 0, 2] for x, y in pos], "detdir": [[0, 0, 1, 0]] * 4,
+        "outputtype": "adjoint_mua_d", "issavedet": 0,
+    }
+    mcxcl_amd.run(cfg)
+    walls, post, kern = [], [], []
+    for _ in range(nrep):
+        t0 = time.perf_counter()
+        out = mcxcl_amd.run(cfg)
+        walls.append(time.perf_counter() - t0)
+        kern.append(out["stat"]["kernelms"])
+        post.append(out["stat"]["jobms"] - out["stat"]["kernelms"])
+    nvox, ns, nd = dim ** 3, 4, 4
+    nbytes = nvox * (ns + nd) * (8 + 4) + 2 * nvox * (ns + nd) * 4 + 2 * nvox * ns * nd * 4
+    post_ms = float(np.median(post))
+    return {
+        "workload": "128^3 homogeneous, 4 pencil sources + 4 detectors as disk sources in one launch, "
+                    "%d photons, outputtype adjoint_mua_d (jmua and jd for 16 pairs)" % int(photons),
+        "run_wall_s": round(float(np.median(walls)), 4),
+        "transport_ms": round(float(np.median(kern)), 3),
+        "post_ms": round(post_ms, 3),
+        "post_algorithmic_bytes": nbytes,
+        "post_GBps": round(nbytes / (post_ms * 1e-3) / 1e9, 1) if post_ms > 0 else None,
+        "outputs": sorted(k for k in out if k.startswith("j")),
+    }
+
+
 def batch_leg(ninst, steps):
     """Many independent simulations in one persistent launch: ninst 64^3 volumes
     with a random ellipsoid each, per-instance media, log-uniform photon counts
@@ -293,6 +338,8 @@ def main():
     ap.add_argument("--inverse-photons", type=float, default=1e6,
                     help="photons per source of the 16x16 multi-source gradient leg (0 skips it)")
     ap.add_argument("--inverse-fd-voxels", type=int, default=4)
+    ap.add_argument("--phiphi-photons", type=float, default=1e7,
+                    help="photons of the phi.phi adjoint-output leg (0 skips it)")
     ap.add_argument("--batch-instances", type=int, default=256,
                     help="simulations of the batched-launch leg (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -410,6 +457,9 @@ def main():
         res["adjoint"] = adjoint_leg(args.adjoint_photons, args.adjoint_fd_voxels)
     if args.inverse_photons > 0:
         res["inverse"] = inverse_leg(args.inverse_photons, args.inverse_fd_voxels)
+
+    if args.phiphi_photons > 0:
+        res["phiphi"] = phiphi_leg(args.phiphi_photons)
     if args.batch_instances > 0:
         res["batch"] = batch_leg(args.batch_instances, args.batch_steps)
     print(json.dumps(res), flush=True)

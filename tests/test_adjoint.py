@@ -37,6 +37,25 @@ def test_detectors_become_disk_sources():
         assert pc.c.extrasrclen == 0
 
 
+def test_standalone_validation_appends_the_detectors():
+    """a stand-alone run (parentid mpStandalone) gets the same append from
+    mcx_validatecfg itself (src/mcx_utils.c:1902-1925)"""
+    import ctypes
+    from mcxcl_amd import _native
+    c = pmcxcl.Config()
+    _native.lib.mcx_initcfg(ctypes.byref(c))
+    try:
+        pmcxcl._parse_config(_case(), c)
+        c.parentid = 0
+        assert _native.lib.mcx_try_validatecfg(ctypes.byref(c), None, None, 0) == 0
+        assert c.extrasrclen == 2 and c.srcid == -1
+        sd = np.ctypeslib.as_array((ctypes.c_float * 32).from_address(c.srcdata)).reshape(2, 4, 4)
+        assert np.allclose(sd[:, 2, 0], [2, 2]) and np.allclose(sd[:, 1, :3], [[0, 0, 1], [0, 0, 1]])
+        assert np.allclose(sd[:, 0, 3], [1, 1])
+    finally:
+        _native.lib.mcx_clearcfg(ctypes.byref(c))
+
+
 def test_difference_stencil_is_exact_on_quadratics():
     """second-order central and one-sided differences differentiate a quadratic
     exactly; a 2-voxel axis falls back to the first difference"""
@@ -51,14 +70,18 @@ def test_difference_stencil_is_exact_on_quadratics():
     assert np.array_equal(oracle._fd_grad(np.ones((1, 2, 2), np.float32), 0), np.zeros((1, 2, 2), np.float32))
 
 
-@pytest.mark.parametrize("outputtype,keys", [("adjoint_mua_d", ("jmua", "jd")), ("adjoint_mua_musp", ("jmua", "jmusp")),
-                                             ("adjoint_mus", ("jmus",))])
-def test_oracle_outputs_match_closed_forms(outputtype, keys):
-    cfg = _case(outputtype, prop=[[0, 0, 1, 1], [0.005, 1.0, 0.8, 1.37]])
+@pytest.mark.parametrize("outputtype,keys,tstep", [("adjoint_mua_d", ("jmua", "jd"), 5e-9),
+                                                   ("adjoint_mua_musp", ("jmua", "jmusp"), 5e-9),
+                                                   ("adjoint_mus", ("jmus",), 5e-9),
+                                                   ("adjoint_mua_d", ("jmua", "jd"), 1e-9)])
+def test_oracle_outputs_match_closed_forms(outputtype, keys, tstep):
+    """with one gate and with five (the products use the gate sums)"""
+    cfg = _case(outputtype, prop=[[0, 0, 1, 1], [0.005, 1.0, 0.8, 1.37]], tstep=tstep)
     out = oracle.run(cfg, nthread=NT)
     assert set(keys) <= set(out) and out[keys[0]].shape == (60, 60, 60, 2)
-    phi = out["flux"].reshape(60, 60, 60, -1, order="F").astype(np.float64)   # [x, y, z, gate x source]
-    assert phi.shape[3] == 3 and (phi[..., 1:] > 0).any()
+    ng = int(round(5e-9 / tstep))
+    phi = out["flux"].reshape(60, 60, 60, ng, 3, order="F").astype(np.float64).sum(3)   # [x, y, z, source]
+    assert (phi[..., 1:] > 0).any()
     grads = [np.gradient(phi[..., k], edge_order=2) for k in range(3)]
     musp_factor = 1.0 / (3.0 * (1 - 0.8) ** 2 * 1.0 ** 2)
     mus_factor = 1.0 / (3.0 * (1 - 0.8) * 1.0 ** 2)
@@ -95,11 +118,12 @@ ADJOINT_TYPES = ["adjoint", "adjoint_dcoeff", "adjoint_mus", "adjoint_musp", "ad
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("outputtype", ADJOINT_TYPES)
-def test_adjoint_outputs_device_parity(outputtype):
+@pytest.mark.parametrize("outputtype,tstep", [(t, 5e-9) for t in ADJOINT_TYPES] + [("adjoint_mua_musp", 1e-9)])
+def test_adjoint_outputs_device_parity(outputtype, tstep):
     """device against the oracle in bit-reproducible mode: the per-source field
-    as for any run, the adjoint products within float32 rounding"""
-    cfg = dict(_case(outputtype, nphoton=20000, prop=[[0, 0, 1, 1], [0.005, 1.0, 0.8, 1.37]]),
+    as for any run, the adjoint products within float32 rounding (one gate, and
+    five for one type)"""
+    cfg = dict(_case(outputtype, nphoton=20000, prop=[[0, 0, 1, 1], [0.005, 1.0, 0.8, 1.37]], tstep=tstep),
                autopilot=0, nthread=2048, nblocksize=64, mathmode=1, rawfield=1)
     gpu, ref = mcxcl_amd.run(cfg), oracle.run(cfg, nthread=2048)
     assert gpu["stat"]["energytot"] == ref["stat"]["energytot"]
