@@ -635,6 +635,9 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     float* draw32 = NULL;
     float* djacob = NULL;
     float* dopscale = NULL;
+    /* float output wanted; adjoint outputs wanted, their length and opscale use */
+    int wantfloat = 0, wantadj = 0, adjdual = 0, adjop = 0, adjmusp = 0;
+    size_t adjlen = 0;
     size_t nrep = 0;
 
     HIP_TRY(hipSetDevice(dev));
@@ -741,6 +744,36 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     g.detcount = ddetcount;
     g.detweight = ddetw;
     g.muamus = dmuamus;
+
+    /* the post-processing buffers are allocated before the timed region: a
+       caller that hands in only the float64 or float32 raw grid (exportfield
+       NULL) gets no normalised float32 copy, so large replay grids are read
+       back once */
+    if (cfg->issave2pt) {
+        wantfloat = cfg->exportfield != NULL || (cfg->exportfield64 == NULL && cfg->exportfield32 == NULL) ||
+                    cfg->parentid == mpStandalone;
+        wantadj = wantfloat && MCX_IS_ADJOINT_TYPE(cfg->outputtype) && g.adjsrc;
+
+        if (wantfloat) {
+            HIP_TRY(dalloc(bufs, &dout, fieldlen));
+        }
+
+        if (wantadj) {
+            adjlen = nvox * g.adjsrc * cfg->detnum;
+            adjdual = MCX_IS_DUAL_ADJOINT_TYPE(cfg->outputtype);
+            adjmusp = cfg->outputtype == otAdjointMusp || cfg->outputtype == otAdjointMuaMusp;
+            adjop = adjmusp || cfg->outputtype == otAdjointMus;
+            HIP_TRY(dalloc(bufs, &djacob, adjlen * (adjdual ? 2 : 1)));
+
+            if (adjop) {
+                HIP_TRY(dalloc(bufs, &dopscale, 256));
+            }
+        }
+
+        if (cfg->exportfield32) {
+            HIP_TRY(dalloc(bufs, &draw32, fieldlen));
+        }
+    }
 
     HIP_TRY(hipStreamSynchronize(stream));
     t0 = wall_ms();
@@ -849,14 +882,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             }
         }
 
-        /* a caller that hands in only the float64 grid (exportfield64 set,
-           exportfield NULL) gets no float32 copy: large replay grids are then
-           read back once */
-        const int wantfloat = cfg->exportfield != NULL || (cfg->exportfield64 == NULL && cfg->exportfield32 == NULL) ||
-                              cfg->parentid == mpStandalone;
-
         if (wantfloat) {
-            HIP_TRY(dalloc(bufs, &dout, fieldlen));
             HIP_TRY(mcx_launch_finalize(dfield, dout, fieldlen, g.nrep, (normalize && scale != 0.f) ? scale : 1.f,
                                         stream));
         } else if (g.nrep > 1) {
@@ -870,13 +896,10 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
            -unitinmm (D, with 1/(3(1-g)mus^2) or 1/(3(1-g)^2 mus^2) per label for
            mus and mus'), into cfg->exportjacob [Nvox, Ns*Nd] (a second block for
            the two-output types) */
-        if (wantfloat && MCX_IS_ADJOINT_TYPE(cfg->outputtype) && g.adjsrc) {
+        if (wantadj) {
             const uint32_t ns = g.adjsrc, nd = cfg->detnum;
-            const size_t adjlen = nvox * ns * nd;
-            const int dual = MCX_IS_DUAL_ADJOINT_TYPE(cfg->outputtype);
+            const int dual = adjdual, musp = adjmusp, needop = adjop;
             const float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
-            const int musp = cfg->outputtype == otAdjointMusp || cfg->outputtype == otAdjointMuaMusp;
-            const int needop = musp || cfg->outputtype == otAdjointMus;
             float hop[256];
 
             for (int m = 0; m < 256; m++) {
@@ -891,10 +914,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
                 }
             }
 
-            HIP_TRY(dalloc(bufs, &djacob, adjlen * (dual ? 2 : 1)));
-
             if (needop) {
-                HIP_TRY(dalloc(bufs, &dopscale, 256));
                 HIP_TRY(hipMemcpyAsync(dopscale, hop, sizeof(hop), hipMemcpyHostToDevice, stream));
             }
 
@@ -908,25 +928,26 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
                                            maxgate, ns, nd, 1, -cfg->unitinmm, needop ? dopscale : NULL, dmedia,
                                            stream));
             }
+        }
 
+        if (cfg->exportfield32) {
+            /* the replicas are folded by now: a plain float64 -> float32 pass */
+            HIP_TRY(mcx_launch_finalize(dfield, draw32, fieldlen, 1, 1.f, stream));
+        }
+
+        HIP_TRY(hipEventRecord(ev2, stream));
+
+        if (wantadj) {
             free(cfg->exportjacob);
-            cfg->exportjacob = (float*)malloc(adjlen * (dual ? 2 : 1) * sizeof(float));
+            cfg->exportjacob = (float*)malloc(adjlen * (adjdual ? 2 : 1) * sizeof(float));
 
             if (!cfg->exportjacob) {
                 HIP_TRY(hipErrorOutOfMemory);
             }
 
-            HIP_TRY(hipMemcpyAsync(cfg->exportjacob, djacob, adjlen * (dual ? 2 : 1) * sizeof(float),
+            HIP_TRY(hipMemcpyAsync(cfg->exportjacob, djacob, adjlen * (adjdual ? 2 : 1) * sizeof(float),
                                    hipMemcpyDeviceToHost, stream));
         }
-
-        if (cfg->exportfield32) {
-            /* the replicas are folded by now: a plain float64 -> float32 pass */
-            HIP_TRY(dalloc(bufs, &draw32, fieldlen));
-            HIP_TRY(mcx_launch_finalize(dfield, draw32, fieldlen, 1, 1.f, stream));
-        }
-
-        HIP_TRY(hipEventRecord(ev2, stream));
 
         if (cfg->exportfield32) {
             HIP_TRY(hipMemcpyAsync(cfg->exportfield32, draw32, fieldlen * sizeof(float), hipMemcpyDeviceToHost,
