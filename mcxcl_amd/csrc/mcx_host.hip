@@ -362,8 +362,8 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     size_t shmem = cfg->medianum * sizeof(float4) + ((cfg->nphase + 3) & ~3u) * sizeof(float) +
                    lblock * pplen * sizeof(float);
 
-    if (shmem > 64 * 1024) {
-        MCX_ERROR(-4, "too many media/phase samples for the LDS tables");
+    if (shmem + MCX_LDS_SLACK > MCX_LDS_LIMIT) {
+        MCX_ERROR(-4, "too many media/phase samples or record columns for the LDS tables");
     }
 
     /* deposit privatisation: most fluence deposits, and every heavily contended
@@ -520,7 +520,12 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         if (envblock) {
             lblock = (size_t)strtoul(envblock, NULL, 10);
         } else if (dynamic) {
-            lblock = bound;
+            /* the wide workgroup holds the same lanes per CU as four 256-lane ones,
+               so its lane scratch costs no more LDS per CU; it only has to fit in
+               one workgroup's LDS (many media with full records may not) */
+            const size_t lane = ((size_t)g.w0offset + g.srcnum) * sizeof(float);
+            lblock = (shmem + (bound - MCX_LAUNCH_BLOCK) * lane + MCX_LDS_SLACK <= MCX_LDS_LIMIT) ? bound
+                     : MCX_LAUNCH_BLOCK;
         }
 
         if (lblock < 64 || lblock > bound || (lblock & 63)) {
