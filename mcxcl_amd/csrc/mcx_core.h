@@ -10,14 +10,19 @@
 #include <hip/hip_runtime.h>
 
 /* Workgroup size bounds of the transport kernel.  The bit-reproducible build
-   launches 256-lane workgroups.  The fast build runs one workgroup per CU,
-   as wide as the kernel's registers allow: 1024 lanes (16 waves, <=128 VGPRs)
-   for the pencil-beam specialisation and 768 (12 waves, <=168 VGPRs) for the
-   general path, so that a single LDS deposit box can use most of the CU's
-   LDS instead of being split between several workgroups. */
+   launches 256-lane workgroups.  The fast build runs one workgroup per CU, so
+   that a single LDS deposit box can use most of the CU's LDS instead of being
+   split between several workgroups: 1024 lanes (16 waves, <=128 VGPRs) for the
+   pencil-beam specialisation (109 VGPRs) and for the general path without
+   detected photons or without reflection (7-9 VGPRs spilled to scratch, still
+   6-9% faster than 768 lanes), 768 lanes (12 waves, <=168 VGPRs, no spills)
+   for the general path with both, which would spill 21. */
 #define MCX_MAX_BLOCK 256
 #define MCX_FAST_SIMPLE_BLOCK 1024
-#define MCX_FAST_BLOCK 768
+#define MCX_FAST_BLOCK 1024
+#define MCX_FAST_DETREF_BLOCK 768
+#define MCX_FAST_BOUND(SAVEDET, REFLECT, SIMPLE) \
+    ((SIMPLE) ? MCX_FAST_SIMPLE_BLOCK : (((SAVEDET) && (REFLECT)) ? MCX_FAST_DETREF_BLOCK : MCX_FAST_BLOCK))
 /* Workgroup size bound of the batched kernel (general path at four waves per
    SIMD, <=128 VGPRs): up to one 1024-lane workgroup per CU. */
 #define MCX_BATCH_BLOCK_MAX 1024

@@ -1256,12 +1256,12 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
 #define MCX_KATTR
 #endif
 #if MCX_PARITY
-#define MCX_KBOUND(SIMPLE) MCX_MAX_BLOCK
+#define MCX_KBOUND(SAVEDET, REFLECT, SIMPLE) MCX_MAX_BLOCK
 #else
-#define MCX_KBOUND(SIMPLE) ((SIMPLE) ? MCX_FAST_SIMPLE_BLOCK : MCX_FAST_BLOCK)
+#define MCX_KBOUND(SAVEDET, REFLECT, SIMPLE) MCX_FAST_BOUND(SAVEDET, REFLECT, SIMPLE)
 #endif
 template <bool SAVEDET, bool REFLECT, bool SIMPLE>
-__global__ void __launch_bounds__(MCX_KBOUND(SIMPLE)) MCX_KATTR transport_kernel(const MCXKParam g) {
+__global__ void __launch_bounds__(MCX_KBOUND(SAVEDET, REFLECT, SIMPLE)) MCX_KATTR transport_kernel(const MCXKParam g) {
     extern __shared__ float4 lds[];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t pplen = g.w0offset + g.srcnum;

@@ -515,7 +515,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     int reflect = 0;
     size_t shmem = mcx_kparam_setup(cfg, &g, nthread, lblock, &reflect, fast ? 0 : MCX_BOX_BYTES);
     {
-        const size_t bound = fast ? (g.simple ? MCX_FAST_SIMPLE_BLOCK : MCX_FAST_BLOCK) : MCX_MAX_BLOCK;
+        const size_t bound = fast ? (size_t)MCX_FAST_BOUND(cfg->issavedet != 0, reflect != 0, g.simple != 0) : MCX_MAX_BLOCK;
 
         if (envblock) {
             lblock = (size_t)strtoul(envblock, NULL, 10);
