@@ -350,14 +350,34 @@ __device__ __forceinline__ void depcache_flush(const MCXKParam& g, const Shared&
 }
 
 /* voxel-exit deposit: LDS box when the voxel and gate fall inside it, else the
-   lane's write-combining buffer in front of the global grid */
+   lane's write-combining buffer in front of the global grid.
+
+   A float32 box cell (fast build) keeps its magnitude below K_BOX_MOVE: the
+   lane whose add finds the cell above it moves K_BOX_MOVE (exactly
+   representable) from the cell to the float64 grid, as the reference moves
+   MAX_ACCUM from its float32 grid to the shadow half (src/mcx_core.cl:2882-2886).
+   Deposits therefore keep the rounding of a float32 sum below 2^11 however
+   many photons a workgroup runs, and the moved amount is exact, so nothing is
+   lost or counted twice when several lanes move at once. */
+#define K_BOX_MOVE 1024.f
 __device__ __forceinline__ void deposit_voxel(const MCXKParam& g, const Shared& sh, DepCache& c, int ox, int oy,
         int oz, uint32_t idx1d, int tshift, float val) {
     uint32_t dx = (uint32_t)(ox - (int)g.box_x0), dy = (uint32_t)(oy - (int)g.box_y0);
     uint32_t dz = (uint32_t)(oz - (int)g.box_z0);
 
     if (dx < g.box_nx && dy < g.box_ny && dz < g.box_nz && (uint32_t)tshift < g.box_ng) {
-        atomicAdd(sh.box + (((uint32_t)tshift * g.box_nz + dz) * g.box_ny + dy) * g.box_nx + dx, (BoxCell)val);
+        BoxCell* cell = sh.box + (((uint32_t)tshift * g.box_nz + dz) * g.box_ny + dy) * g.box_nx + dx;
+#if MCX_PARITY
+        atomicAdd(cell, (BoxCell)val);
+#else
+        const float old = atomicAdd(cell, val);
+
+        if (fabsf(old) > K_BOX_MOVE) {
+            const float mv = old > 0.f ? K_BOX_MOVE : -K_BOX_MOVE;
+            atomicAdd(cell, -mv);
+            atomicAdd(g.field + sh.repoff + idx1d + (uint32_t)tshift * g.nvox, (double)mv);
+        }
+#endif
     } else {
         depcache_add(g, sh, c, idx1d + (uint32_t)tshift * g.nvox, val);
     }
@@ -1287,11 +1307,12 @@ __global__ void __launch_bounds__(MCX_KBOUND(SAVEDET, REFLECT, SIMPLE)) MCX_KATT
     }
 
     if (g.dynamic && threadIdx.x == 0) {
-        /* this workgroup's share: threadphoton photons, +1 for the first oddphoton groups */
-        uint32_t b = blockIdx.x;
-        uint32_t start = b * g.threadphoton + (b < (uint32_t)g.oddphoton ? b : (uint32_t)g.oddphoton);
-        pool[0] = start;
-        pool[1] = start + g.threadphoton + (b < (uint32_t)g.oddphoton ? 1u : 0u);
+        /* this workgroup's share: threadphoton photons, +1 for the first oddphoton
+           groups.  The counter counts from 0 (run-time hand-out uses no photon
+           ids), so the host only has to keep threadphoton + 1 + the lanes' one
+           overshooting draw each below 2^32, whatever the total photon count. */
+        pool[0] = 0u;
+        pool[1] = g.threadphoton + (blockIdx.x < (uint32_t)g.oddphoton ? 1u : 0u);
     }
 
     /* lane-interleaved scratch: element k of lane t at [k*blockDim + t] would avoid

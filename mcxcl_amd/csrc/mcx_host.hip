@@ -50,41 +50,60 @@ extern "C" hipError_t mcx_launch_rngstream(const uint32_t* seed4, unsigned int n
 /* glibc random() TYPE_3 additive-feedback generator, written from its published
    description: r[0]=seed, r[i]=16807*r[i-1] mod (2^31-1) for i<31,
    r[i]=r[i-31] for 31<=i<34, r[i]=r[i-31]+r[i-3] (mod 2^32) afterwards; the k-th
-   output of rand() is r[k+344]>>1.  srand(0) behaves as srand(1). */
-extern "C" void mcx_host_seed_words(unsigned int seed, size_t n, unsigned int* out) {
-    uint32_t r[34 + 344];
-    int32_t s = (int32_t)(seed == 0 ? 1 : seed);
-    r[0] = (uint32_t)s;
+   output of rand() is r[k+344]>>1.  srand(0) behaves as srand(1).
+   The state keeps the last 34 values, so a caller can draw the stream in
+   pieces (one piece per repetition of a respin run). */
+namespace {
 
-    for (int i = 1; i < 31; i++) {
-        int32_t hi = s / 127773, lo = s % 127773;
-        int32_t word = 16807 * lo - 2836 * hi;
+struct GlibcRand {
+    uint32_t ring[34];
+    size_t pos;   /* index of the next value */
 
-        if (word < 0) {
-            word += 2147483647;
+    explicit GlibcRand(unsigned int seed) {
+        int32_t s = (int32_t)(seed == 0 ? 1 : seed);
+        ring[0] = (uint32_t)s;
+
+        for (int i = 1; i < 31; i++) {
+            int32_t hi = s / 127773, lo = s % 127773;
+            int32_t word = 16807 * lo - 2836 * hi;
+
+            if (word < 0) {
+                word += 2147483647;
+            }
+
+            s = word;
+            ring[i] = (uint32_t)s;
         }
 
-        s = word;
-        r[i] = (uint32_t)s;
+        for (int i = 31; i < 34; i++) {
+            ring[i] = ring[i - 31];
+        }
+
+        pos = 34;
+
+        while (pos < 344) {
+            step();
+        }
     }
-
-    for (int i = 31; i < 34; i++) {
-        r[i] = r[i - 31];
-    }
-
-    /* ring of the last 34 values */
-    uint32_t ring[34];
-    memcpy(ring, r, sizeof(ring));
-    size_t pos = 34;   /* index of the next value */
-
-    for (; pos < 344 + n; pos++) {
+    uint32_t step() {
         uint32_t v = ring[(pos - 31) % 34] + ring[(pos - 3) % 34];
         ring[pos % 34] = v;
-
-        if (pos >= 344) {
-            out[pos - 344] = v >> 1;
+        pos++;
+        return v;
+    }
+    /* the next n outputs of rand() */
+    void fill(uint32_t* out, size_t n) {
+        for (size_t i = 0; i < n; i++) {
+            out[i] = step() >> 1;
         }
     }
+};
+
+}  // namespace
+
+extern "C" void mcx_host_seed_words(unsigned int seed, size_t n, unsigned int* out) {
+    GlibcRand r(seed);
+    r.fill(out, n);
 }
 
 extern "C" int mcx_device_count(void) {
@@ -225,6 +244,10 @@ void mcx_check_supported(const Config* cfg) {
     if (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->srctype == MCX_SRC_FOURIERX ||
             cfg->srctype == MCX_SRC_FOURIERX2D || cfg->srctype == MCX_SRC_HYPERBOLOID_GAUSSIAN) {
         MCX_ERROR(-4, "this source type is not supported by the MI355X kernel");
+    }
+
+    if (cfg->seed == SEED_FROM_FILE && cfg->respin != 1) {
+        MCX_ERROR(-4, "respin is disabled in the replay mode");
     }
 
     if (cfg->minenergy >= cfg->srcpos.w) {
@@ -466,6 +489,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     float* hfield = NULL;
     uint32_t hdetcount = 0;
     char msg[512];
+    GlibcRand seedgen(cfg->seed > 0 ? (unsigned int)cfg->seed : (unsigned int)time(0));
 
     mcx_check_supported(cfg);
 
@@ -573,10 +597,21 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     const size_t nvox = g.nvox;
     const size_t fieldlen = g.fieldlen;
     const unsigned int reclen = g.reclen;
+    /* respin: |respin| repetitions of nphoton/|respin| photons each, every one on
+       fresh seeds drawn further down the same rand() stream
+       (src/mcx_host.cpp:1089-1104, :1319-1332) */
+    const unsigned int respin = (unsigned int)(cfg->respin < 0 ? -cfg->respin : (cfg->respin ? cfg->respin : 1));
+    const size_t passphoton = cfg->nphoton / respin;
 
-    /* photon split across the persistent threads (src/mcx_host.cpp:1101-1102) */
-    g.threadphoton = (uint32_t)(cfg->nphoton / nthread);
-    g.oddphoton = (int32_t)(cfg->nphoton - (size_t)g.threadphoton * nthread);
+    /* photon split across the persistent threads (src/mcx_host.cpp:1101-1102).
+       A lane counts its photons in a float (as the reference's f.w), exact
+       below 2^24. */
+    if (passphoton / nthread >= ((size_t)1 << 24)) {
+        MCX_ERROR(-4, "too many photons per thread (more than 2^24): use more threads or respin");
+    }
+
+    g.threadphoton = (uint32_t)(passphoton / nthread);
+    g.oddphoton = (int32_t)(passphoton - (size_t)g.threadphoton * nthread);
 
     /* fast mode: one resident wave of workgroups, each drawing its share of photons
        one at a time, so lanes stay busy until their workgroup's share is done
@@ -589,11 +624,25 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
 
         if (nb > 0 && ncu > 0) {
             nlaunch = (size_t)nb * (size_t)ncu;
+            /* MCXHIP_MAXWG caps the workgroup count (tests use it to give each
+               workgroup, and so each deposit box, many photons) */
+            const char* envmaxwg = getenv("MCXHIP_MAXWG");
+
+            if (envmaxwg && strtoul(envmaxwg, NULL, 10) > 0 && strtoul(envmaxwg, NULL, 10) < nlaunch) {
+                nlaunch = (size_t)strtoul(envmaxwg, NULL, 10);
+            }
+
+            /* a workgroup's counter runs from 0 to its share plus one overshooting
+               draw per lane, in 32 bits */
+            if (passphoton / nlaunch + 1 + lblock >= (size_t)0xFFFFFFFFu) {
+                MCX_ERROR(-4, "too many photons per workgroup (2^32): use respin to split the run");
+            }
+
             nthread = nlaunch * lblock;
             g.nthread = (uint32_t)nthread;
             g.dynamic = 1;
-            g.threadphoton = (uint32_t)(cfg->nphoton / nlaunch);
-            g.oddphoton = (int32_t)(cfg->nphoton - (size_t)g.threadphoton * nlaunch);
+            g.threadphoton = (uint32_t)(passphoton / nlaunch);
+            g.oddphoton = (int32_t)(passphoton - (size_t)g.threadphoton * nlaunch);
         }
     }
 
@@ -686,7 +735,12 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
 
     /* seeds: glibc srand(seed) then one rand() per 32-bit word (src/mcx_host.cpp:696-700, :759-768) */
     hseed = (uint32_t*)malloc(nthread * 4 * sizeof(uint32_t));
-    mcx_host_seed_words(cfg->seed > 0 ? (unsigned int)cfg->seed : (unsigned int)time(0), nthread * 4, hseed);
+
+    if (!hseed) {
+        HIP_TRY(hipErrorOutOfMemory);
+    }
+
+    seedgen.fill(hseed, nthread * 4);
     HIP_TRY(dalloc(bufs, &dnseed, nthread * 4));
     HIP_TRY(hipMemcpyAsync(dnseed, hseed, nthread * 4 * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
 
@@ -782,22 +836,38 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
 
     HIP_TRY(hipStreamSynchronize(stream));
     t0 = wall_ms();
-    HIP_TRY(hipEventRecord(ev0, stream));
 
-    if (g.debugrng) {
-        HIP_TRY(mcx_launch_rngdump(&g, (int)nlaunch, (int)lblock, stream));
-    } else if (cfg->mathmode == 1) {
-        HIP_TRY(mcx_launch_parity(&g, (int)nlaunch, (int)lblock, shmem, stream, cfg->issavedet != 0,
-                                  reflect));
-    } else {
-        HIP_TRY(mcx_launch_fast(&g, (int)nlaunch, (int)lblock, shmem, stream, cfg->issavedet != 0,
-                                reflect));
+    /* one launch per repetition: the grid, the energy tallies and the detected
+       records accumulate on the device; each repetition after the first gets the
+       next nthread*4 rand() words as seeds (src/mcx_host.cpp:1319-1332) */
+    for (unsigned int pass = 0; pass < respin; pass++) {
+        float passms = 0.f;
+
+        if (pass > 0) {
+            seedgen.fill(hseed, nthread * 4);
+            HIP_TRY(hipMemcpyAsync(dnseed, hseed, nthread * 4 * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+        }
+
+        HIP_TRY(hipEventRecord(ev0, stream));
+
+        if (g.debugrng) {
+            HIP_TRY(mcx_launch_rngdump(&g, (int)nlaunch, (int)lblock, stream));
+        } else if (cfg->mathmode == 1) {
+            HIP_TRY(mcx_launch_parity(&g, (int)nlaunch, (int)lblock, shmem, stream, cfg->issavedet != 0,
+                                      reflect));
+        } else {
+            HIP_TRY(mcx_launch_fast(&g, (int)nlaunch, (int)lblock, shmem, stream, cfg->issavedet != 0,
+                                    reflect));
+        }
+
+        HIP_TRY(hipEventRecord(ev2, stream));
+        HIP_TRY(hipEventSynchronize(ev2));
+        HIP_TRY(hipEventElapsedTime(&passms, ev0, ev2));
+        kernelms += passms;
     }
 
-    HIP_TRY(hipEventRecord(ev1, stream));
-    HIP_TRY(hipEventSynchronize(ev1));
     t1 = wall_ms();
-    HIP_TRY(hipEventElapsedTime(&kernelms, ev0, ev1));
     cfg->kernelms = kernelms;
     cfg->jobms = kernelms;
     cfg->runtimems = t1 - t0;
@@ -851,6 +921,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
        (src/mcx_host.cpp:1382-1465, src/mcx_utils.c:1208-1218) */
     if (cfg->issave2pt) {
         float scale = 1.f;
+        HIP_TRY(hipEventRecord(ev1, stream));
         int normalize = cfg->isnormalized && !g.debugrng;
         float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
 
@@ -976,25 +1047,32 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
 
         HIP_TRY(hipStreamSynchronize(stream));
         {
-            float jobms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&jobms, ev0, ev2));
-            cfg->jobms = jobms;
+            float postms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&postms, ev1, ev2));
+            cfg->jobms = kernelms + postms;
         }
 
         if (wantfloat && normalize && scale == 0.f && cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
-            for (int d = 1; d <= (int)cfg->detnum; d++) {
-                float sd = 0.f;
+            /* one slice per detector (src/mcx_host.cpp:1398-1422), and with srcid < 0 one
+               per (source, detector) pair, each scaled by its own photons' weights */
+            const int nsrc = (cfg->extrasrclen && cfg->srcid < 0) ? (int)cfg->extrasrclen + 1 : 1;
 
-                for (size_t i = 0; i < cfg->nphoton; i++)
-                    if ((cfg->replay.detid[i] & 0xFFFF) == d) {
-                        sd += cfg->replay.weight[i];
+            for (int s = 1; s <= nsrc; s++) {
+                for (int d = 1; d <= (int)cfg->detnum; d++) {
+                    float sd = 0.f;
+
+                    for (size_t i = 0; i < cfg->nphoton; i++)
+                        if ((cfg->replay.detid[i] & 0xFFFF) == d && (nsrc == 1 || (cfg->replay.detid[i] >> 16) == s)) {
+                            sd += cfg->replay.weight[i];
+                        }
+
+                    if (sd > 0.f) {
+                        sd = cfg->unitinmm / sd;
                     }
 
-                if (sd > 0.f) {
-                    sd = cfg->unitinmm / sd;
+                    mcx_normalize(hfield + ((size_t)(s - 1) * cfg->detnum + (d - 1)) * nvox * maxgate, sd,
+                                  (int)(nvox * maxgate), 1, 0, 1);
                 }
-
-                mcx_normalize(hfield + (size_t)(d - 1) * nvox * maxgate, sd, (int)(nvox * maxgate), 1, 0, 1);
             }
         }
 

@@ -1265,7 +1265,10 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
 
     g.muamus = muamus;
 
-    size_t nphoton = cfg->nphoton;
+    /* respin: |respin| repetitions of nphoton/|respin| photons, each on the next
+       nthread*4 rand() words (src/mcx_host.cpp:1089-1104, :1319-1332) */
+    const unsigned int respin = (unsigned int)(cfg->respin < 0 ? -cfg->respin : (cfg->respin ? cfg->respin : 1));
+    size_t nphoton = cfg->nphoton / respin;
     g.threadphoton = (uint32_t)(nphoton / nthread);
     g.oddphoton = (int32_t)(nphoton - (size_t)g.threadphoton * nthread);
 
@@ -1301,35 +1304,43 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
 
     double tic = now_sec();
 
-    if (cfg->debuglevel & MCX_DEBUG_RNG) {
-        /* RNG dump: thread idx writes its stream to field[idx], field[idx+nthread], ...
-           (src/mcx_core.cl:2408-2414) */
-        for (unsigned int idx = t0; idx < t1; idx++) {
-            uint64_t t[2];
-            t[0] = (uint64_t)nseed[idx * 4] << 32 | nseed[idx * 4 + 1];
-            t[1] = (uint64_t)nseed[idx * 4 + 2] << 32 | nseed[idx * 4 + 3];
-
-            for (size_t i = idx; i < fieldlen; i += nthread) {
-                works[0].field[i] = rng_next(t, &works[0]);
+    for (unsigned int pass = 0; pass < respin; pass++) {
+        if (pass > 0) {
+            for (size_t j = 0; j < (size_t)nthread * 4; j++) {
+                nseed[j] = (uint32_t)rand();
             }
         }
-    } else {
-        #pragma omp parallel num_threads(nworkers)
-        {
-            int wid = 0;
-#ifdef _OPENMP
-            wid = omp_get_thread_num();
-#endif
-            set_ieee_fpu();
-            OWork* w = works + wid;
-            #pragma omp for schedule(dynamic, 64)
 
-            for (long idx = (long)t0; idx < (long)t1; idx++) {
-                if ((uint32_t)idx >= g.threadphoton * nthread + (uint32_t)g.oddphoton) {
-                    continue;
+        if (cfg->debuglevel & MCX_DEBUG_RNG) {
+            /* RNG dump: thread idx writes its stream to field[idx], field[idx+nthread], ...
+               (src/mcx_core.cl:2408-2414) */
+            for (unsigned int idx = t0; idx < t1; idx++) {
+                uint64_t t[2];
+                t[0] = (uint64_t)nseed[idx * 4] << 32 | nseed[idx * 4 + 1];
+                t[1] = (uint64_t)nseed[idx * 4 + 2] << 32 | nseed[idx * 4 + 3];
+
+                for (size_t i = idx; i < fieldlen; i += nthread) {
+                    works[0].field[i] = rng_next(t, &works[0]);
                 }
+            }
+        } else {
+            #pragma omp parallel num_threads(nworkers)
+            {
+                int wid = 0;
+#ifdef _OPENMP
+                wid = omp_get_thread_num();
+#endif
+                set_ieee_fpu();
+                OWork* w = works + wid;
+                #pragma omp for schedule(dynamic, 64)
 
-                run_thread(&g, w, (int)idx, nseed, genergy);
+                for (long idx = (long)t0; idx < (long)t1; idx++) {
+                    if ((uint32_t)idx >= g.threadphoton * nthread + (uint32_t)g.oddphoton) {
+                        continue;
+                    }
+
+                    run_thread(&g, w, (int)idx, nseed, genergy);
+                }
             }
         }
     }
@@ -1429,19 +1440,27 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
                 scale = 1.f / (float)cfg->energytot;
             } else if (cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS) {
                 if (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
-                    for (int d = 1; d <= (int)cfg->detnum; d++) {
-                        float sd = 0.f;
+                    /* per detector (src/mcx_host.cpp:1398-1422); with srcid < 0 per
+                       (source, detector) slice, each by its own photons' weights (the
+                       reference scales only the first source's slices) */
+                    const int nsrc = (cfg->extrasrclen && cfg->srcid < 0) ? (int)cfg->extrasrclen + 1 : 1;
 
-                        for (size_t i = 0; i < cfg->nphoton; i++)
-                            if ((cfg->replay.detid[i] & 0xFFFF) == d) {
-                                sd += cfg->replay.weight[i];
+                    for (int sr = 1; sr <= nsrc; sr++) {
+                        for (int d = 1; d <= (int)cfg->detnum; d++) {
+                            float sd = 0.f;
+
+                            for (size_t i = 0; i < cfg->nphoton; i++)
+                                if ((cfg->replay.detid[i] & 0xFFFF) == d && (nsrc == 1 || (cfg->replay.detid[i] >> 16) == sr)) {
+                                    sd += cfg->replay.weight[i];
+                                }
+
+                            if (sd > 0.f) {
+                                sd = cfg->unitinmm / sd;
                             }
 
-                        if (sd > 0.f) {
-                            sd = cfg->unitinmm / sd;
+                            mcx_normalize(cfg->exportfield + ((size_t)(sr - 1) * cfg->detnum + (d - 1)) * g.nvox * maxgate, sd,
+                                          (int)(g.nvox * maxgate), 1, 0, 1);
                         }
-
-                        mcx_normalize(cfg->exportfield + (size_t)(d - 1) * g.nvox * maxgate, sd, (int)(g.nvox * maxgate), 1, 0, 1);
                     }
 
                     scale = 0.f;

@@ -26,8 +26,8 @@ NT = 2048
 
 
 def _cfg(srcid, nphoton=20000, **over):
-    cfg = workloads.homogeneous_cube_workload(nphoton=nphoton, srcpos=SRC, srcdir=DIRS, srcid=srcid,
-                                              tstep=1e-9, **over)
+    over.setdefault("tstep", 1e-9)
+    cfg = workloads.homogeneous_cube_workload(nphoton=nphoton, srcpos=SRC, srcdir=DIRS, srcid=srcid, **over)
     return cfg
 
 
@@ -96,3 +96,67 @@ def test_multisource_parity(srcid):
     np.testing.assert_allclose(gpu["flux"], ref["flux"], rtol=1e-6, atol=0)
     assert gpu["detp"].shape == ref["detp"].shape and gpu["detp"].shape[1] > 50
     assert np.array_equal(sorted_records(gpu["detp"]), sorted_records(ref["detp"]))
+
+
+def _replay_srcdet(outputtype, replaydet, fwd, base):
+    return _oracle(dict(base, seed=fwd["seeds"], detphotons=fwd["detp"], outputtype=outputtype,
+                        replaydet=replaydet))
+
+
+@pytest.fixture(scope="module")
+def multisource_forward():
+    det = [[29, 19, 0, 2], [12, 39, 0, 2], [45, 30, 0, 2]]
+    base = _cfg(-1, nphoton=60000, detpos=det, issavedet=1, savedetflag="DP", issaveseed=1, tstep=5e-9)
+    return base, _oracle(base)
+
+
+def test_replay_slices_normalised_per_source_and_detector(multisource_forward):
+    """srcid -1 with replaydet -1: one Jacobian slice per (source, detector),
+    each scaled by unitinmm over the replay weights of its own photons.  (The
+    reference scales only the first source's slices, src/mcx_host.cpp:1398-1422.)"""
+    base, fwd = multisource_forward
+    word = fwd["detp"][0].astype(np.int64)
+    src, det = word >> 16, word & 0xFFFF
+    L = fwd["detp"][1].astype(np.float64)
+    w = np.exp(-0.005 * L)
+    r = _replay_srcdet("jacobian", -1, fwd, base)
+    f64 = r["flux64"].reshape((60, 60, 60, 1, 3, 3), order="F")
+    flux = r["flux"].reshape(f64.shape, order="F")
+    for s in range(1, 4):
+        for d in range(1, 4):
+            sel = (src == s) & (det == d)
+            sl64 = f64[..., d - 1, s - 1]
+            if not sel.any():
+                assert sl64.sum() == 0
+                continue
+            np.testing.assert_allclose(sl64.sum(), (w[sel] * L[sel]).sum(), rtol=1e-6)
+            np.testing.assert_allclose(flux[..., d - 1, s - 1], (sl64 / w[sel].sum()).astype(np.float32),
+                                       rtol=2e-6, atol=1e-12)
+
+
+def test_replay_wp_per_detector_slices(multisource_forward):
+    """WP deposits with replaydet -1: slice d equals the replaydet = d run, and the
+    slices add up to the replaydet = 0 field.  The gate is clamped before the
+    detector and source offsets are added; the reference clamps the sum
+    (src/mcx_core.cl:2591), which with one gate sends every detector's deposits
+    into the first detector's slice."""
+    base, fwd = multisource_forward
+    allr = _replay_srcdet("wp", -1, fwd, base)["flux64"].reshape((60, 60, 60, 1, 3, 3), order="F")
+    pooled = _replay_srcdet("wp", 0, fwd, base)["flux64"].reshape((60, 60, 60, 1, 1, 3), order="F")
+    np.testing.assert_allclose(allr.sum(axis=4), pooled[..., 0, :], rtol=1e-12, atol=1e-300)
+    for d in (1, 3):
+        one = _replay_srcdet("wp", d, fwd, base)["flux64"].reshape((60, 60, 60, 1, 1, 3), order="F")
+        np.testing.assert_allclose(allr[..., d - 1, :], one[..., 0, :], rtol=1e-12, atol=1e-300)
+        assert one.sum() > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("outputtype", ["jacobian", "wp"])
+def test_multisource_replay_parity(multisource_forward, outputtype):
+    base, fwd = multisource_forward
+    cfg = dict(base, seed=fwd["seeds"], detphotons=fwd["detp"], outputtype=outputtype, replaydet=-1,
+               autopilot=0, nthread=NT, nblocksize=64, mathmode=1, rawfield=1)
+    gpu = mcxcl_amd.run(cfg)
+    ref = oracle.run(cfg, nthread=NT)
+    np.testing.assert_allclose(gpu["flux64"], ref["flux64"], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(gpu["flux"], ref["flux"], rtol=1e-6, atol=1e-12)
