@@ -798,6 +798,50 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
     return 0;
 }
 
+/* azimuth and polar angle of one scattering event (src/mcx_core.cl:2446-2500):
+   the azimuth is uniform (skipped in 2D, where rotatevector2d draws a sign);
+   the polar angle comes from the sampled phase function (nphase > 2, linear
+   interpolation of the inverse CDF, :2459-2475) or from Henyey-Greenstein
+   inversion, isotropic for |g| <= EPS and after gscatter events (the
+   similarity rule, :2484-2497) */
+static void scatter_angles(const OParam* g, OWork* w, OPhoton* s, float* stheta, float* ctheta, float* sphi,
+                           float* cphi) {
+    float theta, tmp0;
+    *cphi = 1.f;
+    *sphi = 0.f;
+
+    if (!g->is2d) {
+        tmp0 = TWO_PI * rng_next(s->t, w);
+        pm_sincos(tmp0, sphi, cphi);
+    }
+
+    if (g->nphase > 2) {
+        tmp0 = rng_next(s->t, w) * (float)(g->nphase - 1);
+        theta = tmp0 - (float)((int)tmp0);
+        uint32_t i0 = (uint32_t)tmp0 >= g->nphase ? g->nphase - 1 : (uint32_t)(int)tmp0;
+        uint32_t i1 = (uint32_t)tmp0 + 1 >= g->nphase ? g->nphase - 1 : (uint32_t)((int)tmp0 + 1);
+        tmp0 = (1.f - theta) * g->invcdf[i0] + theta * g->invcdf[i1];
+        theta = pm_acos(tmp0);
+        *stheta = pm_sin(theta);
+        *ctheta = tmp0;
+    } else {
+        tmp0 = (s->v.w > (float)g->gscatter) ? 0.f : s->prop.z;
+
+        if (pm_fabs(tmp0) > K_EPS) {
+            tmp0 = (1.f - s->prop.z * s->prop.z) / (1.f - s->prop.z + 2.f * s->prop.z * rng_next(s->t, w));
+            tmp0 *= tmp0;
+            tmp0 = (1.f + s->prop.z * s->prop.z - tmp0) / (2.f * s->prop.z);
+            tmp0 = fmaxf(-1.f, fminf(1.f, tmp0));
+            theta = pm_acos(tmp0);
+            *stheta = pm_sin(theta);
+            *ctheta = tmp0;
+        } else {
+            theta = pm_acos(2.f * rng_next(s->t, w) - 1.f);
+            pm_sincos(theta, stheta, ctheta);
+        }
+    }
+}
+
 static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed, float* genergy) {
     OPhoton S;
     OPhoton* s = &S;
@@ -837,38 +881,8 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
             f->x = rng_scatlen(s->t, w);
 
             if (v->w != K_EPS) {
-                float cphi = 1.f, sphi = 0.f, theta, stheta, ctheta, tmp0 = 0.f;
-
-                if (!g->is2d) {
-                    tmp0 = TWO_PI * rng_next(s->t, w);
-                    pm_sincos(tmp0, &sphi, &cphi);
-                }
-
-                if (g->nphase > 2) {
-                    tmp0 = rng_next(s->t, w) * (float)(g->nphase - 1);
-                    theta = tmp0 - (float)((int)tmp0);
-                    uint32_t i0 = (uint32_t)tmp0 >= g->nphase ? g->nphase - 1 : (uint32_t)(int)tmp0;
-                    uint32_t i1 = (uint32_t)tmp0 + 1 >= g->nphase ? g->nphase - 1 : (uint32_t)((int)tmp0 + 1);
-                    tmp0 = (1.f - theta) * g->invcdf[i0] + theta * g->invcdf[i1];
-                    theta = pm_acos(tmp0);
-                    stheta = pm_sin(theta);
-                    ctheta = tmp0;
-                } else {
-                    tmp0 = (v->w > (float)g->gscatter) ? 0.f : s->prop.z;
-
-                    if (pm_fabs(tmp0) > K_EPS) {
-                        tmp0 = (1.f - s->prop.z * s->prop.z) / (1.f - s->prop.z + 2.f * s->prop.z * rng_next(s->t, w));
-                        tmp0 *= tmp0;
-                        tmp0 = (1.f + s->prop.z * s->prop.z - tmp0) / (2.f * s->prop.z);
-                        tmp0 = fmaxf(-1.f, fminf(1.f, tmp0));
-                        theta = pm_acos(tmp0);
-                        stheta = pm_sin(theta);
-                        ctheta = tmp0;
-                    } else {
-                        theta = pm_acos(2.f * rng_next(s->t, w) - 1.f);
-                        pm_sincos(theta, &stheta, &ctheta);
-                    }
-                }
+                float cphi, sphi, stheta, ctheta, tmp0;
+                scatter_angles(g, w, s, &stheta, &ctheta, &sphi, &cphi);
 
                 if (g->savedet && (s->mediaid & MED_MASK) > 0) {
                     if (SAVE_NSCAT(g->savedetflag)) {
@@ -1154,75 +1168,69 @@ static double now_sec(void) {
  * cfg->exportdetected/seeddata/detectedcount, cfg->energytot/energyesc/
  * energyabs and cfg->normalizer.  Returns 0 on success.
  */
-int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, unsigned int threadcount, int nomp,
-               OracleResult* res) {
-    memset(res, 0, sizeof(*res));
-
-    if (nthread == 0) {
-        return -1;
-    }
-
-    OParam g;
-    memset(&g, 0, sizeof(g));
-    memcpy(&g.srcpos, &cfg->srcpos, 16);
-    memcpy(&g.srcdir, &cfg->srcdir, 16);
-    memcpy(&g.srcparam1, &cfg->srcparam1, 16);
-    memcpy(&g.srcparam2, &cfg->srcparam2, 16);
-    g.maxidx.x = (float)cfg->dim.x;
-    g.maxidx.y = (float)cfg->dim.y;
-    g.maxidx.z = (float)cfg->dim.z;
-    g.dimx = cfg->dim.x;
-    g.dimxy = cfg->dim.x * cfg->dim.y;
-    g.nvox = g.dimxy * cfg->dim.z;
+/* kernel parameters from a validated Config (the MCXParam fill of
+   src/mcx_host.cpp:438-700); allocates *props and *muamus (free both) */
+static void oracle_setup(Config* cfg, OParam* gp, F4** props_out, float** muamus_out) {
+    memset(gp, 0, sizeof(*gp));
+    memcpy(&gp->srcpos, &cfg->srcpos, 16);
+    memcpy(&gp->srcdir, &cfg->srcdir, 16);
+    memcpy(&gp->srcparam1, &cfg->srcparam1, 16);
+    memcpy(&gp->srcparam2, &cfg->srcparam2, 16);
+    gp->maxidx.x = (float)cfg->dim.x;
+    gp->maxidx.y = (float)cfg->dim.y;
+    gp->maxidx.z = (float)cfg->dim.z;
+    gp->dimx = cfg->dim.x;
+    gp->dimxy = cfg->dim.x * cfg->dim.y;
+    gp->nvox = gp->dimxy * cfg->dim.z;
     unsigned int maxgate = (unsigned int)((cfg->tend - cfg->tstart) / cfg->tstep + 0.5);
     cfg->maxgate = maxgate;
-    g.maxgate = maxgate;
+    gp->maxgate = maxgate;
     unsigned int nsrcslot = (cfg->srcid < 0) ? cfg->extrasrclen + 1 : 1;
     unsigned int ndetslot = (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) ? cfg->detnum : 1;
-    size_t fieldlen = (size_t)g.nvox * maxgate * nsrcslot * ndetslot;
-    g.fieldlen = (uint32_t)fieldlen;
+    size_t fieldlen = (size_t)gp->nvox * maxgate * nsrcslot * ndetslot;
+    gp->fieldlen = (uint32_t)fieldlen;
     float minstep = fminf(fminf(cfg->steps.x, cfg->steps.y), cfg->steps.z);
-    g.twin0 = cfg->tstart;
-    g.twin1 = cfg->tend;
-    g.oneoverc0 = R_C0 * cfg->unitinmm;
-    g.Rtstep = 1.f / cfg->tstep;
-    g.minenergy = cfg->minenergy;
-    g.minaccumtime = minstep * R_C0 * cfg->unitinmm;
-    g.maxdetphoton = cfg->maxdetphoton;
-    g.maxmedia = cfg->medianum - 1;
-    g.detnum = cfg->detnum;
-    g.voidtime = (uint32_t)cfg->voidtime;
-    g.srctype = (uint32_t)cfg->srctype;
-    g.maxvoidstep = (uint32_t)cfg->maxvoidstep;
-    g.issaveseed = cfg->issaveseed > 0;
-    g.issaveref = (uint32_t)cfg->issaveref;
-    g.isspecular = cfg->isspecular > 0;
-    g.seed = cfg->seed;
-    g.outputtype = (uint32_t)cfg->outputtype;
-    g.savedetflag = cfg->savedetflag;
-    g.partialdata = mcx_partialdata(cfg);
-    g.reclen = mcx_detreclen(cfg);
-    g.w0offset = g.partialdata + 4;
-    g.mediaformat = cfg->mediabyte;
-    g.gscatter = cfg->gscatter;
-    g.srcnum = cfg->srcnum;
-    g.doreflect = (uint32_t)cfg->isreflect;
-    g.save2pt = (uint32_t)cfg->issave2pt;
-    g.savedet = (uint32_t)cfg->issavedet;
-    g.nphase = cfg->nphase;
-    g.invcdf = cfg->invcdf;
-    g.debuglevel = cfg->debuglevel;
-    g.extrasrclen = cfg->extrasrclen;
-    g.adjsrc = (MCX_IS_ADJOINT_TYPE(cfg->outputtype) && cfg->seed != SEED_FROM_FILE && cfg->detdir && cfg->detnum &&
+    gp->twin0 = cfg->tstart;
+    gp->twin1 = cfg->tend;
+    gp->oneoverc0 = R_C0 * cfg->unitinmm;
+    gp->Rtstep = 1.f / cfg->tstep;
+    gp->minenergy = cfg->minenergy;
+    gp->minaccumtime = minstep * R_C0 * cfg->unitinmm;
+    gp->maxdetphoton = cfg->maxdetphoton;
+    gp->maxmedia = cfg->medianum - 1;
+    gp->detnum = cfg->detnum;
+    gp->voidtime = (uint32_t)cfg->voidtime;
+    gp->srctype = (uint32_t)cfg->srctype;
+    gp->maxvoidstep = (uint32_t)cfg->maxvoidstep;
+    gp->issaveseed = cfg->issaveseed > 0;
+    gp->issaveref = (uint32_t)cfg->issaveref;
+    gp->isspecular = cfg->isspecular > 0;
+    gp->seed = cfg->seed;
+    gp->outputtype = (uint32_t)cfg->outputtype;
+    gp->savedetflag = cfg->savedetflag;
+    gp->partialdata = mcx_partialdata(cfg);
+    gp->reclen = mcx_detreclen(cfg);
+    gp->w0offset = gp->partialdata + 4;
+    gp->mediaformat = cfg->mediabyte;
+    gp->gscatter = cfg->gscatter;
+    gp->srcnum = cfg->srcnum;
+    gp->doreflect = (uint32_t)cfg->isreflect;
+    gp->save2pt = (uint32_t)cfg->issave2pt;
+    gp->savedet = (uint32_t)cfg->issavedet;
+    gp->nphase = cfg->nphase;
+    gp->invcdf = cfg->invcdf;
+    gp->debuglevel = cfg->debuglevel;
+    gp->extrasrclen = cfg->extrasrclen;
+    gp->adjsrc = (MCX_IS_ADJOINT_TYPE(cfg->outputtype) && cfg->seed != SEED_FROM_FILE && cfg->detdir && cfg->detnum &&
                 cfg->extrasrclen + 1 > cfg->detnum) ? cfg->extrasrclen + 1 - cfg->detnum : 0;
-    g.srcid = cfg->srcid;
-    g.srcdata = cfg->srcdata;
-    g.replaydet = cfg->replaydet;
-    g.replayweight = cfg->replay.weight;
-    g.replaytof = cfg->replay.tof;
-    g.replaydetid = cfg->replay.detid;
-    g.replayseed = (const uint64_t*)cfg->replay.seed;
-    memcpy(g.bc, cfg->bc, 12);
+    gp->srcid = cfg->srcid;
+    gp->srcdata = cfg->srcdata;
+    gp->replaydet = cfg->replaydet;
+    gp->replayweight = cfg->replay.weight;
+    gp->replaytof = cfg->replay.tof;
+    gp->replaydetid = cfg->replay.detid;
+    gp->replayseed = (const uint64_t*)cfg->replay.seed;
+    memcpy(gp->bc, cfg->bc, 12);
 
     /* the reflection block exists when reflection is on, when the boundary flags
        differ from "all absorbing" (as C strings), or when a face reflects or
@@ -1230,11 +1238,11 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
     {
         char bcs[13] = {0}, allabsorb[7] = {bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, bcAbsorb, 0};
         memcpy(bcs, cfg->bc, 12);
-        g.reflectblock = cfg->isreflect || (strcmp(bcs, allabsorb) && bcs[0] != 0);
+        gp->reflectblock = cfg->isreflect || (strcmp(bcs, allabsorb) && bcs[0] != 0);
 
         for (int i = 0; i < 6; i++) {
             if (cfg->bc[i] == bcReflect || cfg->bc[i] == bcMirror) {
-                g.reflectblock = 1;
+                gp->reflectblock = 1;
             }
         }
     }
@@ -1245,13 +1253,13 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
         is2d = is2d * (((cfg->dim.x > 1) + (cfg->dim.y > 1) + (cfg->dim.z > 1)) == 2);
     }
 
-    g.is2d = is2d;
+    gp->is2d = is2d;
 
     F4* props = (F4*)malloc(sizeof(F4) * cfg->medianum);
     memcpy(props, cfg->prop, sizeof(F4) * cfg->medianum);
-    g.prop = props;
-    g.detpos = (const F4*)cfg->detpos;
-    g.media = cfg->vol;
+    gp->prop = props;
+    gp->detpos = (const F4*)cfg->detpos;
+    gp->media = cfg->vol;
     float* muamus = NULL;
 
     if (cfg->muamus) {
@@ -1263,7 +1271,26 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
         }
     }
 
-    g.muamus = muamus;
+    gp->muamus = muamus;
+
+    *props_out = props;
+    *muamus_out = muamus;
+}
+
+int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, unsigned int threadcount, int nomp,
+               OracleResult* res) {
+    memset(res, 0, sizeof(*res));
+
+    if (nthread == 0) {
+        return -1;
+    }
+
+    OParam g;
+    F4* props;
+    float* muamus;
+    oracle_setup(cfg, &g, &props, &muamus);
+    const unsigned int maxgate = g.maxgate;
+    const size_t fieldlen = g.fieldlen;
 
     /* respin: |respin| repetitions of nphoton/|respin| photons, each on the next
        nthread*4 rand() words (src/mcx_host.cpp:1089-1104, :1319-1332) */
@@ -1592,4 +1619,99 @@ void oracle_fpmath(int which, const float* x, float* y, size_t n) {
 
 size_t oracle_result_sizeof(void) {
     return sizeof(OracleResult);
+}
+
+/* Launch sampler check: n successive launches of thread 0 (its seed words are
+   the first four rand() words after srand(cfg->seed)), each recorded right
+   after launch_photon as 8 floats {p.x, p.y, p.z, p.w, v.x, v.y, v.z, label}.
+   Returns the number of launches recorded (a launch that fails ends the run). */
+int oracle_launch_samples(Config* cfg, unsigned int n, float* out) {
+    OParam g;
+    F4* props;
+    float* muamus;
+    oracle_setup(cfg, &g, &props, &muamus);
+    g.threadphoton = n;
+    g.oddphoton = 0;
+    g.savedet = 0;
+    g.save2pt = 0;
+    set_ieee_fpu();
+
+    OWork w;
+    memset(&w, 0, sizeof(w));
+    OPhoton* s = (OPhoton*)calloc(1, sizeof(OPhoton));
+    srand((unsigned int)(cfg->seed > 0 ? cfg->seed : 1));
+    uint32_t words[4];
+
+    for (int i = 0; i < 4; i++) {
+        words[i] = (uint32_t)rand();
+    }
+
+    s->t[0] = (uint64_t)words[0] << 32 | words[1];
+    s->t[1] = (uint64_t)words[2] << 32 | words[3];
+    s->p.w = NAN;
+    s->mediaid = pm_asuint(g.srcparam2.w);
+    unsigned int k = 0;
+
+    for (; k < n; k++) {
+        if (launch_photon(&g, &w, s, 0, 0)) {
+            break;
+        }
+
+        float* o = out + (size_t)k * 8;
+        o[0] = s->p.x;
+        o[1] = s->p.y;
+        o[2] = s->p.z;
+        o[3] = s->p.w;
+        o[4] = s->v.x;
+        o[5] = s->v.y;
+        o[6] = s->v.z;
+        o[7] = (float)(s->mediaid & MED_MASK);
+        s->p.w = -1.f;   /* the next call tallies a zero-weight exit: no deposit, no record */
+        s->mediaid = 1;
+    }
+
+    free(s);
+    free(props);
+    free(muamus);
+    return (int)k;
+}
+
+/* Scattering sampler check: n scattering events of a photon in medium `label`
+   that has already scattered `nscat` times (gscatter compares against it),
+   travelling along (0.6, 0, 0.8) before each event; 3 floats per event
+   {ctheta, stheta, cos of the angle between the old and new directions}. */
+void oracle_scatter_samples(Config* cfg, unsigned int label, unsigned int nscat, unsigned int n, float* out) {
+    OParam g;
+    F4* props;
+    float* muamus;
+    oracle_setup(cfg, &g, &props, &muamus);
+    set_ieee_fpu();
+    OWork w;
+    memset(&w, 0, sizeof(w));
+    OPhoton* s = (OPhoton*)calloc(1, sizeof(OPhoton));
+    uint32_t seed4[4] = {0x2545F491u, 0x4F6CDD1Du, (uint32_t)cfg->seed, 0x9E3779B9u};
+    s->t[0] = (uint64_t)seed4[0] << 32 | seed4[1];
+    s->t[1] = (uint64_t)seed4[2] << 32 | seed4[3];
+    s->prop = g.prop[label];
+
+    for (unsigned int k = 0; k < n; k++) {
+        float stheta, ctheta, sphi, cphi;
+        F4 v0 = {0.6f, 0.f, 0.8f, (float)nscat};
+        s->v = v0;
+        scatter_angles(&g, &w, s, &stheta, &ctheta, &sphi, &cphi);
+
+        if (g.is2d) {
+            rotatevector2d(&s->v, (rng_next(s->t, &w) > 0.5f ? stheta : -stheta), ctheta, (int)g.is2d);
+        } else {
+            rotatevector(&s->v, stheta, ctheta, sphi, cphi);
+        }
+
+        out[(size_t)k * 3] = ctheta;
+        out[(size_t)k * 3 + 1] = stheta;
+        out[(size_t)k * 3 + 2] = s->v.x * v0.x + s->v.y * v0.y + s->v.z * v0.z;
+    }
+
+    free(s);
+    free(props);
+    free(muamus);
 }

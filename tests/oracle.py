@@ -49,6 +49,10 @@ def load():
     lib.oracle_rotatevector.argtypes = [ctypes.c_void_p] + [ctypes.c_float] * 4
     lib.oracle_fpmath.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.oracle_result_sizeof.restype = ctypes.c_size_t
+    lib.oracle_launch_samples.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p]
+    lib.oracle_launch_samples.restype = ctypes.c_int
+    lib.oracle_scatter_samples.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                           ctypes.c_void_p]
     assert lib.oracle_result_sizeof() == ctypes.sizeof(OracleResult)
     return lib
 
@@ -215,3 +219,22 @@ def rotatevector(v, stheta, ctheta, sphi, cphi):
     v = np.array(v, dtype=np.float32)
     lib().oracle_rotatevector(v.ctypes.data, stheta, ctheta, sphi, cphi)
     return v
+
+
+def launch_samples(user, n):
+    """n successive photon launches of thread 0 on a pmcxcl-style config:
+    {"p": [n, 3], "w": [n], "v": [n, 3], "label": [n]} right after launch"""
+    out = np.zeros((n, 8), np.float32)
+    with pmcxcl.prepare(user) as pc:
+        k = lib().oracle_launch_samples(ctypes.addressof(pc.c), n, out.ctypes.data)
+    out = out[:k]
+    return {"p": out[:, :3], "w": out[:, 3], "v": out[:, 4:7], "label": out[:, 7].astype(np.int64)}
+
+
+def scatter_samples(user, label, nscat, n):
+    """n scattering events in medium `label` after `nscat` earlier events:
+    {"ctheta", "stheta", "cosdev"} (cosdev: cosine between old and new direction)"""
+    out = np.zeros((n, 3), np.float32)
+    with pmcxcl.prepare(user) as pc:
+        lib().oracle_scatter_samples(ctypes.addressof(pc.c), label, nscat, n, out.ctypes.data)
+    return {"ctheta": out[:, 0], "stheta": out[:, 1], "cosdev": out[:, 2]}
