@@ -224,3 +224,59 @@ def test_multisource_jacobians_on_device_match_finite_differences():
                 sig.append(float(autograd.detector_signals(torch.tensor(m), torch.tensor(mus0), cfg)[s_, d_]))
             fd = (sig[0] - sig[1]) / (2 * h)
             assert abs(fd - g[v]) <= 0.02 * abs(g[v]), (s_, d_, v, fd, g[v])
+
+
+def _mus_scaling_check(base, label, h, seeds, photons):
+    """d S / d a for mus -> mus (1 + a) over the voxels of one label: the score
+    gradient sum_v mus_v dS/dmus_v against central differences S(+h), S(-h) on
+    the same seeds (common random numbers up to the first changed flight).
+    Returns per-seed (gradient, difference, sum of the WP term)."""
+    vol = base["vol"]
+    prop = np.asarray(base["prop"], np.float32)
+    mua0 = prop[vol, 0].astype(np.float32)
+    mus0 = prop[vol, 1].astype(np.float32)
+    sel = vol == label
+    rows = []
+    for seed in seeds:
+        cfg = dict(base, seed=int(seed), nphoton=int(photons))
+        mus = torch.tensor(mus0, requires_grad=True)
+        mua = torch.tensor(mua0, requires_grad=True)
+        autograd.detector_signals(mua, mus, cfg).sum().backward()
+        grad = float((mus0[sel].astype(np.float64) * mus.grad.numpy()[sel]).sum())
+        wp_term = float((mus0[sel].astype(np.float64) * (mus.grad.numpy()[sel] - mua.grad.numpy()[sel])).sum())
+        sig = []
+        for s in (1, -1):
+            m = mus0.copy()
+            m[sel] *= np.float32(1 + s * h)
+            sig.append(float(autograd.detector_signals(torch.tensor(mua0), torch.tensor(m), cfg).sum()))
+        rows.append((grad, (sig[0] - sig[1]) / (2 * h), wp_term))
+    return np.array(rows)
+
+
+def _assert_mus_scaling(rows):
+    g, fd, wp = rows[:, 0], rows[:, 1], rows[:, 2]
+    d = fd - g
+    se = d.std(ddof=1) / np.sqrt(len(d))
+    assert abs(d.mean()) < 4.5 * se, (g.mean(), fd.mean(), se)
+    # the check has the power to see either score term dropped
+    assert 10 * se < abs(wp.mean()), (se, wp.mean())
+
+
+def test_mus_gradient_matches_layerwise_finite_differences(oracle_engine):
+    """dS/dmus = WP/mus - J: scaling mus of the whole background label by
+    (1 +- 10%) on 6 seeds; the mean difference between the score gradient and
+    the central difference is zero within 4.5 standard errors, and that error
+    is below a tenth of the WP term (dropping WP/mus or -J fails)"""
+    base = workloads.sphere_cube_workload(nphoton=100000, tstep=5e-9)
+    _assert_mus_scaling(_mus_scaling_check(base, 1, 0.1, 1000 + 17 * np.arange(6), 100000))
+
+
+@pytest.mark.gpu
+def test_mus_gradient_on_device_matches_layerwise_finite_differences(monkeypatch):
+    """the same check on the device at 2e6 photons per seed, 8 seeds, for the
+    background and the sphere label (fast arithmetic, fixed photon-per-lane map
+    so each seed's three runs share their random numbers)"""
+    monkeypatch.setenv("MCXHIP_DYNAMIC", "0")
+    base = workloads.sphere_cube_workload(nphoton=2000000, tstep=5e-9, mathmode=0)
+    for label in (1, 2):
+        _assert_mus_scaling(_mus_scaling_check(base, label, 0.1, 500 + 31 * np.arange(8), 2000000))
