@@ -15,8 +15,10 @@ The JSON line also carries:
                 committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present;
                 "atomic" = the kernel's float64 atomic rate against the ceiling
                 measured live by mcx_device_atomic_ceiling
-  cpu_baseline  the CPU oracle (C restatement of the reference kernel, OpenMP)
-                timed on this host on a bounded sample of the same workload
+  cpu_baseline  the C restatement of the reference kernel in its CPU-baseline
+                build (-O3, libm float32 math, float32 tallies, OpenMP) timed
+                on this host on a bounded sample of the same workload, with
+                the host's thread count and CPU model
   adjoint       replay-adjoint gradient of detector signals on the 200^3
                 four-layer volume against central finite differences
   inverse       16 sources x 16 detectors on 128^3 per-voxel media: forward,
@@ -101,6 +103,18 @@ def host_threads():
     if omp and omp.isdigit():
         n = min(n, int(omp))
     return max(n, 1)
+
+
+def cpu_model():
+    """the host CPU's model name (lscpu's 'Model name', from /proc/cpuinfo)"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def copy_probe(nbytes):
@@ -330,6 +344,8 @@ def main():
     ap.add_argument("--photons", type=float, default=1e8)
     ap.add_argument("--mathmode", type=int, default=0, help="0: hardware transcendentals, 1: bit-reproducible")
     ap.add_argument("--cpu-photons", type=float, default=3e6, help="photons in the CPU-baseline sample")
+    ap.add_argument("--count-photons", type=float, default=1e6,
+                    help="photons of the oracle run that counts events for the operation model")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="run one plain device copy of this many bytes and exit")
     ap.add_argument("--adjoint-photons", type=float, default=1e7,
@@ -411,10 +427,16 @@ def main():
         import oracle
 
         ncpu = host_threads()
+        # CPU baseline: the -O3 float32 build of the C restatement, OpenMP on the
+        # host threads, timed on a bounded sample of the same workload
         cpu_n = int(args.cpu_photons)
         ccfg = workloads.sphere_cube_workload(nphoton=cpu_n, autopilot=0, nthread=65536, nblocksize=64)
-        ref = oracle.run(ccfg, nthread=65536, nomp=ncpu)
-        cpu_rate = cpu_n / (ref["seconds"] * 1e3)
+        base = oracle.run(ccfg, nthread=65536, nomp=ncpu, fast=True)
+        cpu_rate = cpu_n / (base["seconds"] * 1e3)
+        # event counts per photon for the operation model: the checker build counts them
+        ncount = int(args.count_photons)
+        ref = oracle.run(workloads.sphere_cube_workload(nphoton=ncount, autopilot=0, nthread=65536, nblocksize=64),
+                         nthread=65536, nomp=ncpu)
         opp, depp = ops_per_photon(ref["counters"])
         transport_ms = float(np.mean(kernel_ms))
         achieved = nphoton * opp / (transport_ms * 1e-3) / 1e12
@@ -449,9 +471,12 @@ def main():
             "unit": "photons/ms",
             "cores": ncpu,
             "kind": "port",
-            "sample": "%d photons of the same workload (65536-thread map), C oracle with OpenMP, %.1f s" %
-                      (cpu_n, ref["seconds"]),
-            "absorbed_fraction": round(ref["stat"]["energyabs"] / ref["stat"]["energytot"], 6),
+            "sample": "%d photons of the same workload (65536-thread map): the C restatement of mcx_core.cl "
+                      "semantics built -O3 -march=x86-64-v3 with libm float32 math and float32 tallies, OpenMP "
+                      "on %d threads, %.1f s" % (cpu_n, ncpu, base["seconds"]),
+            "absorbed_fraction": round(base["stat"]["energyabs"] / base["stat"]["energytot"], 6),
+            "nproc": os.cpu_count(),
+            "cpu_model": cpu_model(),
         }
     if args.adjoint_photons > 0:
         res["adjoint"] = adjoint_leg(args.adjoint_photons, args.adjoint_fd_voxels)

@@ -28,7 +28,7 @@
  *          the half-angle identity acos(x) = 2 asin(sqrt((1-x)/2)) for x > 1/2
  *          and acos(x) = pi - 2 asin(sqrt((1+x)/2)) for x < -1/2.
  *
- * Accuracy against double-precision libm is checked in tests/test_fpmath.py.
+ * Accuracy against double-precision libm is checked in tests/test_oracle_pins.py.
  */
 #ifndef MCX_FPMATH_H
 #define MCX_FPMATH_H

@@ -24,7 +24,7 @@
  *
  * Arithmetic: every transcendental goes through the deterministic FP32
  * functions of mcx_fpmath.h (shared with the kernel's parity mode and pinned
- * against libm in tests/test_fpmath.py); division and square root are IEEE.
+ * against libm in tests/test_oracle_pins.py); division and square root are IEEE.
  * The file is compiled with -ffp-contract=off and without fast-math, so
  * each float operation rounds exactly once, in source order.
  *
@@ -67,6 +67,25 @@
 
 #include "mcx_utils.h"
 #include "mcx_fpmath.h"
+
+#ifdef ORACLE_FAST
+/* CPU-baseline build of the same source (oracle/Makefile, libmcxoracle_fast.so):
+   libm single-precision functions instead of the deterministic ones, no event
+   counters and float32 private tallies, compiled with -O3 and contraction.  It
+   is timed beside the GPU and never used as the checker. */
+#define pm_exp(x) expf(x)
+#define pm_log(x) logf(x)
+#define pm_sin(x) sinf(x)
+#define pm_cos(x) cosf(x)
+#define pm_acos(x) acosf(x)
+#define pm_sincos(x, s, c) (*(s) = sinf(x), *(c) = cosf(x))
+#define pm_rsqrt(x) (1.f / sqrtf(x))
+#define OCOUNT(w, k) ((void)0)
+typedef float OTally;
+#else
+#define OCOUNT(w, k) ((w)->cnt.k += 1)
+typedef double OTally;
+#endif
 
 #define K_EPS          FLT_EPSILON
 #define JUST_BELOW_ONE 0.9998f
@@ -116,7 +135,7 @@ typedef struct {
 } OParam;
 
 typedef struct {
-    double* field;        /* private double field, fieldlen entries */
+    OTally* field;        /* private field, fieldlen entries (double in the checker) */
     float* det;           /* private detected-photon records */
     uint64_t* seeds;      /* private detected-photon seeds (2 words each) */
     float* detw;          /* private exit weights of the detected photons */
@@ -132,7 +151,7 @@ static inline float rng_next(uint64_t t[2], OWork* w) {
     s1 ^= s1 << 23;
     t[1] = s1 ^ s0 ^ (s1 >> 18) ^ (s0 >> 5);
     uint32_t lo = (uint32_t)(t[1] + s0);
-    w->cnt.nrand += 1;
+    OCOUNT(w, nrand);
     return pm_asfloat(0x3F800000u | (lo >> 9)) - 1.0f;
 }
 
@@ -433,8 +452,8 @@ static void save_detected(const OParam* g, OWork* w, OPhoton* s, uint32_t isdet)
 
 static inline void deposit(const OParam* g, OWork* w, uint32_t addr, float val) {
     (void)g;
-    w->field[addr] += (double)val;
-    w->cnt.ndep += 1;
+    w->field[addr] += (OTally)val;
+    OCOUNT(w, ndep);
 }
 
 /* time gate of a time-of-flight, clamped to the last gate */
@@ -794,7 +813,7 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
     pp[2] = (g->srcnum > 1) ? pp[2] : p->w;
     v->w = K_EPS;
     s->Lmove = 0.f;
-    w->cnt.nlaunch += 1;
+    OCOUNT(w, nlaunch);
     return 0;
 }
 
@@ -875,7 +894,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
     s->mediaid &= MED_MASK;
 
     while (f->w <= (float)nlimit) {
-        w->cnt.nstep += 1;
+        OCOUNT(w, nstep);
 
         if (f->x <= 0.f) {
             f->x = rng_scatlen(s->t, w);
@@ -902,7 +921,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                 }
 
                 v->w += 1.f;
-                w->cnt.nscat += 1;
+                OCOUNT(w, nscat);
 
                 if (g->outputtype == otWP || g->outputtype == otDCS) {
                     uint32_t pid = photon_index(g, idx, f->w) - 1;
@@ -1071,7 +1090,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                     (((isdet & 0xF) == bcMirror) || n1 != s->prop.w)) {
                 float Rtotal = 1.f;
                 float cphi, sphi, stheta, ctheta;
-                w->cnt.nbnd += 1;
+                OCOUNT(w, nbnd);
                 s->prop = getvoxprop(g, s->mediaid, s->idx1d);
                 float tmp0 = n1 * n1;
                 float tmp1 = s->prop.w * s->prop.w;
@@ -1326,7 +1345,7 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
     OWork* works = (OWork*)calloc(nworkers, sizeof(OWork));
 
     for (int i = 0; i < nworkers; i++) {
-        works[i].field = (double*)calloc(fieldlen, sizeof(double));
+        works[i].field = (OTally*)calloc(fieldlen, sizeof(OTally));
     }
 
     double tic = now_sec();

@@ -15,6 +15,9 @@ from mcxcl_amd import pmcxcl
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "build", "libmcxoracle.so")
+# the CPU-baseline build of the same source (libm math, no counters, float32
+# tallies, -O3): timed beside the GPU, never used as the checker
+ORACLE_FAST_SO = os.path.join(ROOT, "oracle", "build", "libmcxoracle_fast.so")
 
 
 class OracleCounters(ctypes.Structure):
@@ -32,10 +35,10 @@ class OracleResult(ctypes.Structure):
     ]
 
 
-def load():
-    if not os.path.exists(ORACLE_SO):
+def load(path=ORACLE_SO):
+    if not os.path.exists(path):
         raise ImportError("oracle not built: run `make -C oracle`")
-    lib = ctypes.CDLL(ORACLE_SO)
+    lib = ctypes.CDLL(path)
     lib.oracle_run.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_int,
                                ctypes.POINTER(OracleResult)]
     lib.oracle_run.restype = ctypes.c_int
@@ -58,6 +61,7 @@ def load():
 
 
 _LIB = None
+_FAST = None
 
 
 def lib():
@@ -67,12 +71,20 @@ def lib():
     return _LIB
 
 
-def run(user, nthread, threadoffset=0, threadcount=0, nomp=0):
+def fastlib():
+    global _FAST
+    if _FAST is None:
+        _FAST = load(ORACLE_FAST_SO)
+    return _FAST
+
+
+def run(user, nthread, threadoffset=0, threadcount=0, nomp=0, fast=False):
     """Run the oracle on a pmcxcl-style config with the given persistent-thread count.
 
     Returns the pmcxcl output dict plus ``flux64`` (unnormalised double field),
-    ``genergy`` (per-thread float tallies), ``counters`` and ``seconds``."""
-    L = lib()
+    ``genergy`` (per-thread float tallies), ``counters`` and ``seconds``.
+    ``fast`` runs the CPU-baseline build instead of the checker (no counters)."""
+    L = fastlib() if fast else lib()
     with pmcxcl.prepare(user) as pc:
         res = OracleResult()
         rc = L.oracle_run(ctypes.addressof(pc.c), nthread, threadoffset, threadcount, nomp, ctypes.byref(res))
