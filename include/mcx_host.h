@@ -32,6 +32,28 @@ int mcx_list_gpu(Config* cfg, unsigned int* activedev, int* activedevlist, GPUIn
    success, the error id otherwise (message in mcx_last_error()) */
 int mcx_try_run_simulation(Config* cfg);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): a forward fluence run of a validated Config (as
+   mcx_run_simulation takes it, without detected photons, replay, adjoint
+   outputs, RNG dump or per-voxel media) on the CURRENT HIP device, with the
+   volume and the outputs in caller-owned device memory:
+     media    uint32[nmedia]    cfg->vol as validated (labels, bit 31 detector mask)
+     field64  double[fieldlen]  unnormalised accumulator (cleared here)
+     fluence  float[fieldlen]   normalised float output, or NULL
+   fieldlen = voxels x gates (x sources when srcid < 0); stream is a
+   hipStream_t (NULL: the null stream).  The energies and the normaliser are
+   returned in cfg; the call returns once the outputs are complete.  Nothing
+   is allocated: per-lane seeds and tallies and the staged tables live in a
+   6.1 MB per-device static scratch, so a run may use at most 262144 lanes
+   (the fast mode's resident wave on 256 CUs; set nthread in bit-reproducible
+   mode), 4096 media, 4096 phase samples and 256 extra sources; larger runs
+   are rejected with an error.  Calls on one device share that scratch (and
+   a host staging chunk): overlapping calls race.  Returns 0, or the error id
+   with the message in mcx_last_error(). */
+int mcx_run_simulation_noalloc(Config* cfg, const unsigned int* media, size_t nmedia, double* field64, float* fluence,
+                               size_t fieldlen, void* stream);
+
 /* number of HIP devices visible to this process (0 when none) */
 int mcx_device_count(void);
 

@@ -475,59 +475,52 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     return shmem;
 }
 
-extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalenergy) {
-    (void)fluence;
-    (void)totalenergy;
-    hipError_t err = hipSuccess;
-    int errline = 0;
-    DevBuf bufs;
-    bufs.n = 0;
-    hipStream_t stream = NULL;
-    hipEvent_t ev0 = NULL, ev1 = NULL, ev2 = NULL;
-    uint32_t* hseed = NULL;
-    float* hgenergy = NULL;
-    float* hfield = NULL;
-    uint32_t hdetcount = 0;
-    char msg[512];
-    GlibcRand seedgen(cfg->seed > 0 ? (unsigned int)cfg->seed : (unsigned int)time(0));
+/* normalisation factor of the float output (src/mcx_host.cpp:1382-1465, after
+   the energy tallies are known): flux = unitinmm / (E_launched Vvox tstep),
+   fluence = flux x tstep, energy and path length = 1 / E_launched, replay
+   outputs = unitinmm / sum of the replay weights; x (number of sources) when
+   every source is simulated into its own slice.  0 when replay output is
+   scaled per detector afterwards. */
+static float mcx_output_scale(const Config* cfg) {
+    const float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
+    float scale = 1.f;
 
-    mcx_check_supported(cfg);
+    if (cfg->outputtype == otFlux || cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
+        scale = cfg->unitinmm / (float)(cfg->energytot * Vvox * cfg->tstep);
 
-    unsigned int activedev = 0;
-    int devlist[MAX_DEVICE];
-    GPUInfo* gpu = NULL;
-    int ndev = mcx_list_gpu(cfg, &activedev, devlist, &gpu);
+        if (cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
+            scale *= cfg->tstep;
+        }
+    } else if (cfg->outputtype == otEnergy || cfg->outputtype == otL) {
+        scale = 1.f / (float)cfg->energytot;
+    } else if (cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS) {
+        if (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
+            return 0.f;   /* per-detector scaling after the read-back */
+        }
 
-    if (ndev == 0 || activedev == 0) {
-        free(gpu);
-        MCX_ERROR(-99, "No HIP device found (or the specified GPU does not exist)");
+        float sw = 0.f;
+
+        for (size_t i = 0; i < cfg->nphoton; i++) {
+            sw += cfg->replay.weight[i];
+        }
+
+        scale = sw > 0.f ? cfg->unitinmm / sw : 0.f;
     }
 
-    const int dev = devlist[0];
-
-    /* thread map: reference autopilot for AMD devices, or the user's nthread/nblocksize */
-    size_t nblocksize = cfg->autopilot ? gpu[dev].autoblock : cfg->nblocksize;
-    size_t nthread = cfg->autopilot ? gpu[dev].autothread : cfg->nthread;
-    const int ncu = gpu[dev].sm;
-
-    if (nblocksize == 0) {
-        free(gpu);
-        MCX_ERROR(-4, "block size must be positive");
+    if (cfg->extrasrclen && cfg->srcid < 0) {
+        scale *= (float)(cfg->extrasrclen + 1);
     }
 
-    /* the reference rounds the thread count down to whole blocks; the photon map
-       depends only on the thread count, so the transport kernel itself is launched
-       in workgroups of its own size (lblock) with the tail lanes idle */
-    nthread = (nthread / nblocksize) * nblocksize;
+    return scale;
+}
 
-    if (nthread == 0) {
-        free(gpu);
-        MCX_ERROR(-4, "thread number must be at least one block");
-    }
-
-    free(gpu);
-    gpu = NULL;
-
+/* Launch plan of one run on device dev (ncu compute units) with *nthread_io
+   lanes of the reference's thread map: the kernel parameters, the workgroup
+   size and count, the dynamic LDS bytes and the reflection variant.  Fast mode
+   (outside replay) turns the thread map into one resident wave of workgroups
+   that hand photons out at run time and updates *nthread_io to its lanes. */
+static void mcx_plan_run(Config* cfg, int dev, int ncu, size_t* nthread_io, MCXKParam* gp, size_t* lblock_out,
+                         size_t* nlaunch_out, size_t* shmem_out, int* reflect_out) {
     const char* envblock = getenv("MCXHIP_BLOCK");
     const char* envdyn = getenv("MCXHIP_DYNAMIC");
     const int fast = cfg->mathmode == 0 && !(cfg->debuglevel & MCX_DEBUG_RNG);
@@ -535,8 +528,9 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
        each lane must rerun the photons of its recorded seeds */
     const int dynamic = fast && cfg->seed != SEED_FROM_FILE && !(envdyn && envdyn[0] == '0');
     size_t lblock = MCX_LAUNCH_BLOCK;
-    MCXKParam g;
+    MCXKParam& g = *gp;
     int reflect = 0;
+    size_t nthread = *nthread_io;
     size_t shmem = mcx_kparam_setup(cfg, &g, nthread, lblock, &reflect, fast ? 0 : MCX_BOX_BYTES);
     {
         const size_t bound = fast ? (size_t)MCX_FAST_BOUND(cfg->issavedet != 0, reflect != 0, g.simple != 0) : MCX_MAX_BLOCK;
@@ -593,10 +587,6 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     }
 
     size_t nlaunch = (nthread + lblock - 1) / lblock;
-    const unsigned int maxgate = g.maxgate;
-    const size_t nvox = g.nvox;
-    const size_t fieldlen = g.fieldlen;
-    const unsigned int reclen = g.reclen;
     /* respin: |respin| repetitions of nphoton/|respin| photons each, every one on
        fresh seeds drawn further down the same rand() stream
        (src/mcx_host.cpp:1089-1104, :1319-1332) */
@@ -652,6 +642,76 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
                 g.dynamic ? "dynamic photons" : "fixed photons per lane", shmem, g.box_nx, g.box_ny, g.box_nz,
                 g.box_ng, g.box_x0, g.box_y0, g.box_z0, g.nrep, g.simple);
     }
+
+    *nthread_io = nthread;
+    *lblock_out = lblock;
+    *nlaunch_out = nlaunch;
+    *shmem_out = shmem;
+    *reflect_out = reflect;
+}
+
+extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalenergy) {
+    (void)fluence;
+    (void)totalenergy;
+    hipError_t err = hipSuccess;
+    int errline = 0;
+    DevBuf bufs;
+    bufs.n = 0;
+    hipStream_t stream = NULL;
+    hipEvent_t ev0 = NULL, ev1 = NULL, ev2 = NULL;
+    uint32_t* hseed = NULL;
+    float* hgenergy = NULL;
+    float* hfield = NULL;
+    uint32_t hdetcount = 0;
+    char msg[512];
+    GlibcRand seedgen(cfg->seed > 0 ? (unsigned int)cfg->seed : (unsigned int)time(0));
+
+    mcx_check_supported(cfg);
+
+    unsigned int activedev = 0;
+    int devlist[MAX_DEVICE];
+    GPUInfo* gpu = NULL;
+    int ndev = mcx_list_gpu(cfg, &activedev, devlist, &gpu);
+
+    if (ndev == 0 || activedev == 0) {
+        free(gpu);
+        MCX_ERROR(-99, "No HIP device found (or the specified GPU does not exist)");
+    }
+
+    const int dev = devlist[0];
+
+    /* thread map: reference autopilot for AMD devices, or the user's nthread/nblocksize */
+    size_t nblocksize = cfg->autopilot ? gpu[dev].autoblock : cfg->nblocksize;
+    size_t nthread = cfg->autopilot ? gpu[dev].autothread : cfg->nthread;
+    const int ncu = gpu[dev].sm;
+
+    if (nblocksize == 0) {
+        free(gpu);
+        MCX_ERROR(-4, "block size must be positive");
+    }
+
+    /* the reference rounds the thread count down to whole blocks; the photon map
+       depends only on the thread count, so the transport kernel itself is launched
+       in workgroups of its own size (lblock) with the tail lanes idle */
+    nthread = (nthread / nblocksize) * nblocksize;
+
+    if (nthread == 0) {
+        free(gpu);
+        MCX_ERROR(-4, "thread number must be at least one block");
+    }
+
+    free(gpu);
+    gpu = NULL;
+
+    size_t lblock = 0, nlaunch = 0, shmem = 0;
+    int reflect = 0;
+    MCXKParam g;
+    mcx_plan_run(cfg, dev, ncu, &nthread, &g, &lblock, &nlaunch, &shmem, &reflect);
+    const unsigned int maxgate = g.maxgate;
+    const size_t nvox = g.nvox;
+    const size_t fieldlen = g.fieldlen;
+    const unsigned int reclen = g.reclen;
+    const unsigned int respin = (unsigned int)(cfg->respin < 0 ? -cfg->respin : (cfg->respin ? cfg->respin : 1));
 
     if (cfg->seed == SEED_FROM_FILE &&
             (cfg->nphoton == 0 || !cfg->replay.seed || !cfg->replay.weight || !cfg->replay.tof || !cfg->replay.detid)) {
@@ -923,34 +983,9 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         float scale = 1.f;
         HIP_TRY(hipEventRecord(ev1, stream));
         int normalize = cfg->isnormalized && !g.debugrng;
-        float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
 
         if (normalize) {
-            if (cfg->outputtype == otFlux || cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
-                scale = cfg->unitinmm / (float)(cfg->energytot * Vvox * cfg->tstep);
-
-                if (cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
-                    scale *= cfg->tstep;
-                }
-            } else if (cfg->outputtype == otEnergy || cfg->outputtype == otL) {
-                scale = 1.f / (float)cfg->energytot;
-            } else if (cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS) {
-                if (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
-                    scale = 0.f;   /* per-detector scaling below */
-                } else {
-                    float sw = 0.f;
-
-                    for (size_t i = 0; i < cfg->nphoton; i++) {
-                        sw += cfg->replay.weight[i];
-                    }
-
-                    scale = sw > 0.f ? cfg->unitinmm / sw : 0.f;
-                }
-            }
-
-            if (cfg->extrasrclen && cfg->srcid < 0) {
-                scale *= (float)(cfg->extrasrclen + 1);
-            }
+            scale = mcx_output_scale(cfg);
 
             if (scale != 0.f) {
                 cfg->normalizer = scale;
@@ -1296,4 +1331,228 @@ extern "C" int mcx_device_atomic_ceiling(int gpuid, unsigned int ncell, double* 
     }
 
     return (int)e;
+}
+
+/* ---------------- no-allocation entry point ----------------
+   mcx_run_simulation_noalloc keeps its per-run device state in `scratch`, a
+   file-scope array that each device holds for the life of the process: the
+   per-lane seed words and energy tallies and the staged medium, phase-function
+   and source tables.  Host code never uses the array's name as an address: on
+   a device's first call it asks hipGetSymbolAddress for that device's copy and
+   keeps the pointer (mcx_noalloc_base).  The caller passes the volume and the
+   outputs in device memory; nothing is allocated per call, on the device or
+   on the host (seed words and energy tallies pass through fixed host chunks). */
+#define MCX_NOALLOC_LANES ((size_t)262144) /* one resident wave: 256 CUs x 1024 lanes */
+#define MCX_NOALLOC_MEDIA ((size_t)4096)
+#define MCX_NOALLOC_PHASE ((size_t)4096)
+#define MCX_NOALLOC_SRC ((size_t)256)
+#define MCX_NOALLOC_OFF_ENERGY (MCX_NOALLOC_LANES * 4 * sizeof(uint32_t))
+#define MCX_NOALLOC_OFF_PROP (MCX_NOALLOC_OFF_ENERGY + MCX_NOALLOC_LANES * 2 * sizeof(float))
+#define MCX_NOALLOC_OFF_PHASE (MCX_NOALLOC_OFF_PROP + MCX_NOALLOC_MEDIA * sizeof(float4))
+#define MCX_NOALLOC_OFF_SRC (MCX_NOALLOC_OFF_PHASE + MCX_NOALLOC_PHASE * sizeof(float))
+#define MCX_NOALLOC_BYTES (MCX_NOALLOC_OFF_SRC + MCX_NOALLOC_SRC * sizeof(ExtraSrc))
+#define MCX_NOALLOC_CHUNK ((size_t)65536)
+
+static __device__ __align__(256) unsigned char scratch[MCX_NOALLOC_BYTES];
+
+static void* mcx_noalloc_base[MAX_DEVICE];
+static uint32_t mcx_noalloc_words[MCX_NOALLOC_CHUNK];
+static float mcx_noalloc_energy[MCX_NOALLOC_CHUNK];
+
+/* zero the energy tallies and the caller's float64 field (a fill kernel rather
+   than hipMemsetAsync, so the call can sit in a captured stream) */
+static __global__ void noalloc_clear_kernel(float* __restrict__ energy, size_t nenergy, double* __restrict__ field,
+                                     size_t nfield) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nenergy; i += stride) {
+        energy[i] = 0.f;
+    }
+
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nfield; i += stride) {
+        field[i] = 0.0;
+    }
+}
+
+static void noalloc_run(Config* cfg, const uint32_t* media, size_t nmedia, double* field64, float* fluence,
+                        size_t fieldlen, hipStream_t stream) {
+    hipError_t err = hipSuccess;
+    int errline = 0;
+    char msg[512];
+    int dev = 0, ncu = 0;
+    GlibcRand seedgen(cfg->seed > 0 ? (unsigned int)cfg->seed : (unsigned int)time(0));
+
+    mcx_check_supported(cfg);
+
+    if (cfg->issavedet || cfg->seed == SEED_FROM_FILE || MCX_IS_ADJOINT_TYPE(cfg->outputtype) ||
+            (cfg->debuglevel & MCX_DEBUG_RNG) || cfg->muamus || !cfg->issave2pt) {
+        MCX_ERROR(-4, "mcx_run_simulation_noalloc runs forward fluence simulations only: no detected photons, "
+                  "replay, adjoint outputs, RNG dump or per-voxel media, and the field must be saved");
+    }
+
+    if (!media || !field64) {
+        MCX_ERROR(-4, "mcx_run_simulation_noalloc needs the volume and the float64 field in device memory");
+    }
+
+    if (cfg->medianum > MCX_NOALLOC_MEDIA || cfg->nphase > MCX_NOALLOC_PHASE || cfg->extrasrclen > MCX_NOALLOC_SRC) {
+        MCX_ERROR(-4, "mcx_run_simulation_noalloc holds at most 4096 media, 4096 phase-function samples and 256 "
+                  "extra sources");
+    }
+
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICE ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) {
+        MCX_ERROR(-99, "mcx_run_simulation_noalloc: no usable current HIP device");
+    }
+
+    if (!mcx_noalloc_base[dev]) {
+        void* p = NULL;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+
+        if (e != hipSuccess || !p) {
+            snprintf(msg, sizeof(msg), "mcx_run_simulation_noalloc: scratch address of device %d not found (%s)", dev,
+                     hipGetErrorString(e));
+            MCX_ERROR(-99, msg);
+        }
+
+        mcx_noalloc_base[dev] = p;
+    }
+
+    unsigned char* base = (unsigned char*)mcx_noalloc_base[dev];
+    size_t nblocksize = cfg->autopilot ? 64 : cfg->nblocksize;
+    size_t nthread = cfg->autopilot ? (size_t)2560 * (size_t)ncu : cfg->nthread;
+
+    if (nblocksize == 0 || nthread < nblocksize) {
+        MCX_ERROR(-4, "thread number must be at least one block");
+    }
+
+    nthread = (nthread / nblocksize) * nblocksize;
+    size_t lblock = 0, nlaunch = 0, shmem = 0;
+    int reflect = 0;
+    MCXKParam g;
+    mcx_plan_run(cfg, dev, ncu, &nthread, &g, &lblock, &nlaunch, &shmem, &reflect);
+
+    if (nthread > MCX_NOALLOC_LANES) {
+        snprintf(msg, sizeof(msg), "mcx_run_simulation_noalloc: %zu lanes exceed its scratch (%zu lanes); set "
+                 "nthread (autopilot 0) or use fast arithmetic", nthread, MCX_NOALLOC_LANES);
+        MCX_ERROR(-4, msg);
+    }
+
+    if (nmedia != g.nvox || fieldlen != g.fieldlen) {
+        snprintf(msg, sizeof(msg), "mcx_run_simulation_noalloc: buffer sizes %zu/%zu do not match the volume (%u "
+                 "voxels) and the field (%u values)", nmedia, fieldlen, g.nvox, g.fieldlen);
+        MCX_ERROR(-4, msg);
+    }
+
+    uint32_t* dseed = (uint32_t*)base;
+    float* denergy = (float*)(base + MCX_NOALLOC_OFF_ENERGY);
+    float4* dprop = (float4*)(base + MCX_NOALLOC_OFF_PROP);
+    float* dphase = (float*)(base + MCX_NOALLOC_OFF_PHASE);
+    float4* dsrc = (float4*)(base + MCX_NOALLOC_OFF_SRC);
+    const unsigned int respin = (unsigned int)(cfg->respin < 0 ? -cfg->respin : (cfg->respin ? cfg->respin : 1));
+    double t0 = wall_ms();
+
+    g.nrep = 1;
+    g.media = media;
+    g.prop = dprop;
+    g.detpos = NULL;
+    g.invcdf = cfg->nphase ? dphase : NULL;
+    g.srcdata = cfg->extrasrclen ? dsrc : NULL;
+    g.nseed = dseed;
+    g.field = field64;
+    g.genergy = denergy;
+    g.detcount = NULL;
+
+    cfg->detectedcount = 0;
+    cfg->his.detected = 0;
+    cfg->his.respin = cfg->respin;
+    cfg->his.colcount = g.reclen;
+    cfg->energytot = 0;
+    cfg->energyesc = 0;
+
+    HIP_TRY(hipMemcpyAsync(dprop, cfg->prop, cfg->medianum * sizeof(float4), hipMemcpyHostToDevice, stream));
+
+    if (cfg->nphase) {
+        HIP_TRY(hipMemcpyAsync(dphase, cfg->invcdf, cfg->nphase * sizeof(float), hipMemcpyHostToDevice, stream));
+    }
+
+    if (cfg->extrasrclen) {
+        HIP_TRY(hipMemcpyAsync(dsrc, cfg->srcdata, cfg->extrasrclen * sizeof(ExtraSrc), hipMemcpyHostToDevice, stream));
+    }
+
+    {
+        int nb = (int)((fieldlen > 2 * nthread ? fieldlen : 2 * nthread) / 256 + 1);
+        nb = nb > 4096 ? 4096 : nb;
+        hipLaunchKernelGGL(noalloc_clear_kernel, dim3(nb), dim3(256), 0, stream, denergy, 2 * nthread, field64,
+                           fieldlen);
+        HIP_TRY(hipGetLastError());
+    }
+
+    for (unsigned int pass = 0; pass < respin; pass++) {
+        /* seeds: the next nthread*4 words of the srand(seed) stream, as
+           mcx_run_simulation draws them, through the fixed host chunk */
+        for (size_t off = 0; off < nthread * 4; off += MCX_NOALLOC_CHUNK) {
+            const size_t n = nthread * 4 - off < MCX_NOALLOC_CHUNK ? nthread * 4 - off : MCX_NOALLOC_CHUNK;
+            seedgen.fill(mcx_noalloc_words, n);
+            HIP_TRY(hipMemcpyAsync(dseed + off, mcx_noalloc_words, n * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                   stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+        }
+
+        if (cfg->mathmode == 1) {
+            HIP_TRY(mcx_launch_parity(&g, (int)nlaunch, (int)lblock, shmem, stream, 0, reflect));
+        } else {
+            HIP_TRY(mcx_launch_fast(&g, (int)nlaunch, (int)lblock, shmem, stream, 0, reflect));
+        }
+    }
+
+    /* energy tallies summed in thread order in double, as mcx_run_simulation does */
+    for (size_t off = 0; off < nthread * 2; off += MCX_NOALLOC_CHUNK) {
+        const size_t n = nthread * 2 - off < MCX_NOALLOC_CHUNK ? nthread * 2 - off : MCX_NOALLOC_CHUNK;
+        HIP_TRY(hipMemcpyAsync(mcx_noalloc_energy, denergy + off, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+
+        for (size_t i = 0; i < n; i += 2) {
+            cfg->energyesc += mcx_noalloc_energy[i];
+            cfg->energytot += mcx_noalloc_energy[i + 1];
+        }
+    }
+
+    cfg->energyabs = cfg->energytot - cfg->energyesc;
+
+    if (fluence) {
+        float scale = 1.f;
+
+        if (cfg->isnormalized) {
+            scale = mcx_output_scale(cfg);
+            cfg->normalizer = scale;
+            cfg->his.normalizer = scale;
+        }
+
+        HIP_TRY(mcx_launch_finalize(field64, fluence, fieldlen, 1, scale, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+
+    cfg->runtimems = wall_ms() - t0;
+    cfg->runtime = (unsigned int)(cfg->runtimems + 0.5);
+    cfg->kernelms = 0.f;
+    cfg->jobms = 0.f;
+    return;
+
+fail:
+    snprintf(msg, sizeof(msg), "HIP error %d (%s) at %s:%d", (int)err, hipGetErrorString(err), __FILE__, errline);
+    mcx_error(-(int)err, msg, __FILE__, errline);
+}
+
+extern "C" int mcx_run_simulation_noalloc(Config* cfg, const unsigned int* media, size_t nmedia, double* field64,
+        float* fluence, size_t fieldlen, void* stream) {
+    jmp_buf jb;
+    void* old = mcx_push_errjmp(&jb);
+    int rc = setjmp(jb);
+
+    if (rc == 0) {
+        noalloc_run(cfg, media, nmedia, field64, fluence, fieldlen, (hipStream_t)stream);
+    }
+
+    mcx_pop_errjmp(old);
+    return rc;
 }
