@@ -76,7 +76,9 @@ int mcx_device_rng_stream(int gpuid, const unsigned int seed4[4], unsigned int n
 int mcx_device_atomic_ceiling(int gpuid, unsigned int ncell, double* rate);
 
 /* launch-shape model of the fast transport kernel variant (savedet, reflect,
-   simple) on the current device: its VGPRs per lane and largest workgroup
+   simple: 1 the pencil-beam path, 0 the general path, 2 the full general path
+   with pattern/fourierx/hyperboloid/angle-CDF launches and RF replay outputs) on
+   the current device: its VGPRs per lane and largest workgroup
    (returns 0 on success), and the workgroups of nthreadblock lanes with shmem
    bytes of dynamic LDS that one CU holds (gfx950: 4 SIMDs of at most 8 waves
    and 512 VGPRs per lane each, the device's LDS per CU) */

@@ -381,6 +381,10 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
                       "outputs");
         }
 
+        if (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->nangle) {
+            MCX_ERROR(-4, "the batched launch takes no pattern sources or angle-CDF launches");
+        }
+
         if (cfg->nphoton >= 0xFFFFFFFFull) {
             MCX_ERROR(-4, "more than 2^32-1 photons in one instance");
         }

@@ -23,6 +23,12 @@
 #define MCX_FAST_DETREF_BLOCK 768
 #define MCX_FAST_BOUND(SAVEDET, REFLECT, SIMPLE) \
     ((SIMPLE) ? MCX_FAST_SIMPLE_BLOCK : (((SAVEDET) && (REFLECT)) ? MCX_FAST_DETREF_BLOCK : MCX_FAST_BLOCK))
+/* The full general path (pattern, fourierx, hyperboloid and angle-CDF launches,
+   shared patterns, RF and time-weighted replay outputs) is a separate instance so
+   the common general path keeps its registers: 768 lanes (<=168 VGPRs). */
+#define MCX_FAST_FULL_BLOCK 768
+#define MCX_FAST_BOUND_FULL(SAVEDET, REFLECT, SIMPLE, FULL) \
+    ((FULL) ? MCX_FAST_FULL_BLOCK : MCX_FAST_BOUND(SAVEDET, REFLECT, SIMPLE))
 /* Workgroup size bound of the batched kernel (general path at four waves per
    SIMD, <=128 VGPRs): up to one 1024-lane workgroup per CU. */
 #define MCX_BATCH_BLOCK_MAX 1024
@@ -49,16 +55,27 @@ struct MCXKParam {
        disables */
     uint32_t box_x0, box_y0, box_z0, box_nx, box_ny, box_nz, box_ng, box_cells;
     uint32_t nrep;                /* global accumulator replicas (power of two) */
+    uint32_t npattern;            /* patterns sharing each photon (a pattern source with srcnum > 1), else 1;
+                                     their slots interleave after the voxel: (vox + gate*nvox)*npattern + k */
+    uint32_t nangle;              /* entries of the launch polar-angle table (0: none) */
+    uint32_t pplen;               /* floats of per-lane LDS scratch (records, tallies, w0, source id,
+                                     pattern weights, the RF replay phase) */
+    uint32_t imoff;               /* complex RF replay outputs: offset of the imaginary half (0: real output) */
+    float omega;                  /* RF replay: angular modulation frequency, rad/s */
     uint32_t dynamic;             /* 1: photons are handed out per workgroup at run time
                                      (threadphoton/oddphoton then split photons over workgroups) */
     uint32_t simple;              /* 1: pencil beam from a non-void voxel without focus, extra sources,
                                      replay, 2D, phase table, reflectance tally or replay output types */
+    uint32_t full;                /* 1: the run uses a pattern, fourierx or hyperboloid launch, an
+                                     angle-CDF launch, shared patterns or RF/time-weighted replay outputs */
     unsigned char bc[12];
 
     const uint32_t* media;        /* label volume, x fastest; bit 31 marks detector-adjacent voxels */
     const float4* prop;           /* {mua, mus, g, n} per label */
     const float4* detpos;         /* {x, y, z, r} per detector */
     const float* invcdf;          /* sampled phase-function inverse CDF (nphase) */
+    const float* srcpattern;      /* pattern/pattern3d pixel weights, npattern per pixel */
+    const float* angleinvcdf;     /* launch polar angle / pi at nangle equal CDF steps */
     const float4* srcdata;        /* extra sources, 4 float4 each */
     const float* replayweight;
     const float* replaytof;

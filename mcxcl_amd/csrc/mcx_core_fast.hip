@@ -12,11 +12,14 @@
 template <bool SAVEDET, bool REFLECT>
 static hipError_t launch_variant(const MCXKParam* prm, int nblock, int nthreadblock, size_t shmem, hipStream_t stream) {
     if (prm->simple) {
-        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, true>), dim3(nblock), dim3(nthreadblock), shmem,
-                           stream, *prm);
+        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, true, false>), dim3(nblock), dim3(nthreadblock),
+                           shmem, stream, *prm);
+    } else if (prm->full) {
+        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, false, true>), dim3(nblock), dim3(nthreadblock),
+                           shmem, stream, *prm);
     } else {
-        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, false>), dim3(nblock), dim3(nthreadblock), shmem,
-                           stream, *prm);
+        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, false, false>), dim3(nblock), dim3(nthreadblock),
+                           shmem, stream, *prm);
     }
 
     return hipGetLastError();
@@ -60,29 +63,22 @@ static int gfx950_occupancy(const void* kernel, int nthreadblock, size_t shmem) 
     return byRegs < byLds ? byRegs : byLds;
 }
 
-template <bool SAVEDET, bool REFLECT, bool SIMPLE>
-static int occupancy_of(int nthreadblock, size_t shmem) {
-    return gfx950_occupancy((const void*)mcxk::transport_kernel<SAVEDET, REFLECT, SIMPLE>, nthreadblock, shmem);
+/* the transport kernel instance of a variant: path 1 the pencil-beam
+   specialisation, 0 the general path, 2 the full general path */
+template <bool SAVEDET, bool REFLECT>
+static const void* kernel_of(int path) {
+    return path == 1 ? (const void*)mcxk::transport_kernel<SAVEDET, REFLECT, true, false> :
+           (path == 2 ? (const void*)mcxk::transport_kernel<SAVEDET, REFLECT, false, true> :
+            (const void*)mcxk::transport_kernel<SAVEDET, REFLECT, false, false>);
+}
+
+static const void* transport_kernel_of(int savedet, int reflect, int path) {
+    return savedet ? (reflect ? kernel_of<true, true>(path) : kernel_of<true, false>(path)) :
+           (reflect ? kernel_of<false, true>(path) : kernel_of<false, false>(path));
 }
 
 extern "C" int mcx_fast_occupancy(int nthreadblock, size_t shmem, int savedet, int reflect, int simple) {
-    if (savedet) {
-        if (reflect) {
-            return simple ? occupancy_of<true, true, true>(nthreadblock, shmem)
-                   : occupancy_of<true, true, false>(nthreadblock, shmem);
-        }
-
-        return simple ? occupancy_of<true, false, true>(nthreadblock, shmem)
-               : occupancy_of<true, false, false>(nthreadblock, shmem);
-    }
-
-    if (reflect) {
-        return simple ? occupancy_of<false, true, true>(nthreadblock, shmem)
-               : occupancy_of<false, true, false>(nthreadblock, shmem);
-    }
-
-    return simple ? occupancy_of<false, false, true>(nthreadblock, shmem)
-           : occupancy_of<false, false, false>(nthreadblock, shmem);
+    return gfx950_occupancy(transport_kernel_of(savedet, reflect, simple), nthreadblock, shmem);
 }
 
 /* ---------------- batched launch ---------------- */
@@ -109,15 +105,7 @@ extern "C" int mcx_batch_occupancy(int nthreadblock, size_t shmem, int reflect) 
    bound), for tests */
 extern "C" int mcx_fast_kernel_attr(int savedet, int reflect, int simple, int* vgpr, int* maxlanes) {
     hipFuncAttributes a;
-    const void* k = savedet ?
-                    (reflect ? (simple ? (const void*)mcxk::transport_kernel<true, true, true> :
-                                (const void*)mcxk::transport_kernel<true, true, false>) :
-                     (simple ? (const void*)mcxk::transport_kernel<true, false, true> :
-                      (const void*)mcxk::transport_kernel<true, false, false>)) :
-                    (reflect ? (simple ? (const void*)mcxk::transport_kernel<false, true, true> :
-                                (const void*)mcxk::transport_kernel<false, true, false>) :
-                     (simple ? (const void*)mcxk::transport_kernel<false, false, true> :
-                      (const void*)mcxk::transport_kernel<false, false, false>));
+    const void* k = transport_kernel_of(savedet, reflect, simple);
 
     if (hipFuncGetAttributes(&a, k) != hipSuccess) {
         return -1;

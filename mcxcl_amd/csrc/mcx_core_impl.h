@@ -45,9 +45,12 @@
 #define K_SEED_FROM_FILE (-999)
 
 enum {kReflect = 1, kAbsorb = 2, kMirror = 3, kCyclic = 4};
-enum {kFlux = 0, kFluence = 1, kEnergy = 2, kJacobian = 3, kWP = 4, kDCS = 5, kL = 7};
-enum {kPencil = 0, kIsotropic = 1, kCone = 2, kGaussian = 3, kPlanar = 4, kFourier = 6, kArcsine = 7, kDisk = 8,
-      kZGaussian = 11, kLine = 12, kSlit = 13, kPencilArray = 14, kRing = 17
+enum {kFlux = 0, kFluence = 1, kEnergy = 2, kJacobian = 3, kWP = 4, kDCS = 5, kRF = 6, kL = 7, kRFmus = 8,
+      kWLTOF = 9, kWPTOF = 10
+     };
+enum {kPencil = 0, kIsotropic = 1, kCone = 2, kGaussian = 3, kPlanar = 4, kPattern = 5, kFourier = 6, kArcsine = 7,
+      kDisk = 8, kFourierX = 9, kFourierX2D = 10, kZGaussian = 11, kLine = 12, kSlit = 13, kPencilArray = 14,
+      kPattern3D = 15, kHyperboloid = 16, kRing = 17
      };
 
 /* Each build (parity, fast) gets its own namespace: the kernels are templates
@@ -573,10 +576,29 @@ __device__ void save_detected(const MCXKParam& g, const Shared& sh, const Photon
     }
 }
 
+/* launch weight of a pattern pixel (src/mcx_core.cl:1678-1727): one pattern
+   scales the photon's weight; several share a unit-weight photon, each keeping
+   its pixel value in the lane scratch (ppl = the scratch past the record
+   columns), the pixel index in the w0 slot as bits */
+__device__ __forceinline__ void pattern_weights(const MCXKParam& g, float* ppl, float4& p, int pix, float srcw) {
+    if (g.npattern <= 1) {
+        p.w = srcw * g.srcpattern[pix];
+        ppl[4] = p.w;
+    } else {
+        ppl[2] = __uint_as_float((uint32_t)pix);
+
+        for (uint32_t i = 0; i < g.npattern; i++) {
+            ppl[4 + i] = g.srcpattern[(size_t)pix * g.npattern + i];
+        }
+
+        p.w = 1.f;
+    }
+}
+
 /* terminate the current photon and launch the next one (src/mcx_core.cl:1466-2275).
    Returns 0 on a launch, 1 when the lane's photon budget is used up, -1 when
    no launch succeeded within maxvoidstep attempts. */
-template <bool SAVEDET, bool SIMPLE>
+template <bool SAVEDET, bool SIMPLE, bool FULL>
 __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rng& rng, int idx, uint32_t isdet) {
     float* pp = sh.pp;
     const uint32_t pd = g.partialdata;
@@ -701,14 +723,30 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
             const int st = (int)g.srctype;
             bool positioned = false;
 
-            if (st == kPlanar || st == kFourier || st == kPencilArray) {
+            if (st == kPlanar || (FULL && (st == kPattern || st == kPattern3D)) || st == kFourier || st == kPencilArray) {
                 float rx = rng.next();
                 float ry = rng.next();
-                p.x = p.x + rx * lp1.x + ry * lp2.x;
-                p.y = p.y + rx * lp1.y + ry * lp2.y;
-                p.z = p.z + rx * lp1.z + ry * lp2.z;
 
-                if (st == kFourier) {
+                if (FULL && st == kPattern3D) {
+                    /* a param1.x x param1.y x param1.z box of voxels (src/mcx_core.cl:1661-1667) */
+                    const float rz = rng.next();
+                    p.x = p.x + rx * lp1.x;
+                    p.y = p.y + ry * lp1.y;
+                    p.z = p.z + rz * lp1.z;
+                    pattern_weights(g, ppl, p, (int)(rz * K_JUST_BELOW_ONE * lp1.z) * (int)(lp1.y) * (int)(lp1.x) +
+                                    (int)(ry * K_JUST_BELOW_ONE * lp1.y) * (int)(lp1.x) +
+                                    (int)(rx * K_JUST_BELOW_ONE * lp1.x), lpos.w);
+                } else {
+                    p.x = p.x + rx * lp1.x + ry * lp2.x;
+                    p.y = p.y + rx * lp1.y + ry * lp2.y;
+                    p.z = p.z + rx * lp1.z + ry * lp2.z;
+                }
+
+                if (FULL && st == kPattern) {
+                    /* param1.w x param2.w pixels over the parallelogram (src/mcx_core.cl:1678-1693) */
+                    pattern_weights(g, ppl, p, (int)(ry * K_JUST_BELOW_ONE * lp2.w) * (int)(lp1.w) +
+                                    (int)(rx * K_JUST_BELOW_ONE * lp1.w), lpos.w);
+                } else if (st == kFourier) {
                     p.w = lpos.w * (kcos((floorf(lp1.w) * rx + floorf(lp2.w) * ry + lp1.w - floorf(lp1.w)) * K_TWO_PI) *
                                     (1.f - lp2.w + floorf(lp2.w)) + 1.f) * 0.5f;
                 } else if (st == kPencilArray) {
@@ -722,6 +760,72 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
                 s.prop.y = s.prop.y + (lp1.y + lp2.y) * 0.5f;
                 s.prop.z = s.prop.z + (lp1.z + lp2.z) * 0.5f;
                 s.prop.w = 0.f;
+            } else if (FULL && (st == kFourierX || st == kFourierX2D)) {
+                /* edges param1 and param1.w dir x param1/|param1|; param2 the pattern
+                   (src/mcx_core.cl:1776-1812) */
+                const float rx = rng.next();
+                const float ry = rng.next();
+                float4 v2 = lp1;
+                v2.w *= krsqrt(lp1.x * lp1.x + lp1.y * lp1.y + lp1.z * lp1.z);
+                v2.x = v2.w * (ldir.y * lp1.z - ldir.z * lp1.y);
+                v2.y = v2.w * (ldir.z * lp1.x - ldir.x * lp1.z);
+                v2.z = v2.w * (ldir.x * lp1.y - ldir.y * lp1.x);
+                p.x = p.x + rx * lp1.x + ry * v2.x;
+                p.y = p.y + rx * lp1.y + ry * v2.y;
+                p.z = p.z + rx * lp1.z + ry * v2.z;
+
+                if (st == kFourierX2D) {
+                    p.w = lpos.w * (ksin((lp2.x * rx + lp2.z) * K_TWO_PI) * ksin((lp2.y * ry + lp2.w) * K_TWO_PI) + 1.f) *
+                          0.5f;
+                } else {
+                    p.w = lpos.w * (kcos((lp2.x * rx + lp2.y * ry + lp2.z) * K_TWO_PI) * (1.f - lp2.w) + 1.f) * 0.5f;
+                }
+
+                positioned = true;
+                s.prop.x = s.prop.x + (lp1.x + v2.x) * 0.5f;
+                s.prop.y = s.prop.y + (lp1.y + v2.y) * 0.5f;
+                s.prop.z = s.prop.z + (lp1.z + v2.z) * 0.5f;
+                s.prop.w = 0.f;
+            } else if (FULL && st == kHyperboloid) {
+                /* rays on the one-sheet hyperboloid of a Gaussian beam: waist param1.x at
+                   param1.y along the axis, param1.z its length scale; the launch voxel is
+                   left as set (src/mcx_core.cl:1820-1852) */
+                float sphi, cphi;
+                float r = K_TWO_PI * rng.next();
+                ksincos(r, &sphi, &cphi);
+                r = ksqrt(0.5f * rng.scatlen()) * lp1.x;
+                const float hx = kdiv(-lp1.y, lp1.z);
+                const float hy = krsqrt(r * r + lp1.z * lp1.z);
+                float qx = r * (cphi - hx * sphi), qy = r * (sphi + hx * cphi), qz = 0.f;
+                const float dx = -r * sphi * hy, dy = r * cphi * hy, dz = lp1.z * hy;
+
+                if (v.z > -1.f + K_EPS && v.z < 1.f - K_EPS) {
+                    float t = 1.f - v.z * v.z;
+                    const float stheta = ksqrt(t);
+                    t = krsqrt(t);
+                    cphi = v.x * t;
+                    sphi = v.y * t;
+                    const float nx = qx * cphi * v.z - qy * sphi, ny = qx * sphi * v.z + qy * cphi, nz = -qx * stheta;
+                    const float vx = dx * cphi * v.z - dy * sphi + dz * cphi * stheta;
+                    const float vy = dx * sphi * v.z + dy * cphi + dz * sphi * stheta;
+                    const float vz = -dx * stheta + dz * v.z;
+                    qx = nx;
+                    qy = ny;
+                    qz = nz;
+                    v.x = vx;
+                    v.y = vy;
+                    v.z = vz;
+                } else {
+                    v.z = (v.z > 0.f) ? dz : -dz;
+                    v.x = dx;
+                    v.y = dy;
+                }
+
+                p.x = qx + lpos.x;
+                p.y = qy + lpos.y;
+                p.z = qz + lpos.z;
+                s.prop = make_float4(lpos.x, lpos.y, lpos.z, 0.f);
+                s.Lmove = 0.f;
             } else if (st == kDisk || st == kGaussian || st == kRing) {
                 float sphi, cphi, phi, r;
 
@@ -860,7 +964,34 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
             continue;   /* re-tests the loop condition, as the reference does */
         }
 
-        if (s.Lmove < 0.f) {
+        if (FULL && g.nangle) {
+            /* sampled launch polar angle (src/mcx_core.cl:2098-2125): theta/pi at nangle
+               equal CDF steps, taken as steps (srcdir.w > 0) or interpolated; uniform
+               azimuth about srcdir (used as is when 0 <= srcdir.w < 1.5) */
+            float ang, stheta, ctheta, sphi, cphi;
+
+            if (ldir.w > 0.f) {
+                ang = fminf(rng.next() * (float)g.nangle, (float)g.nangle - K_EPS);
+                cphi = g.angleinvcdf[(int)ang];
+            } else {
+                ang = fminf(rng.next() * (float)(g.nangle - 1), (float)(g.nangle - 1) - K_EPS);
+                sphi = ang - (float)((int)ang);
+                const uint32_t i0 = ((uint32_t)ang >= g.nangle - 1) ? g.nangle - 1 : (uint32_t)(int)ang;
+                const uint32_t i1 = ((uint32_t)ang + 1 >= g.nangle - 1) ? g.nangle - 1 : (uint32_t)((int)ang + 1);
+                cphi = (1.f - sphi) * g.angleinvcdf[i0] + sphi * g.angleinvcdf[i1];
+            }
+
+            cphi *= K_ONE_PI;
+            ksincos(cphi, &stheta, &ctheta);
+            ang = K_TWO_PI * rng.next();
+            ksincos(ang, &sphi, &cphi);
+
+            if (ldir.w < 1.5f && ldir.w >= 0.f) {
+                v = ldir;
+            }
+
+            rotatevector(v, stheta, ctheta, sphi, cphi);
+        } else if (s.Lmove < 0.f) {
             if (ldir.w != ldir.w) {
                 float ang, stheta, ctheta, sphi, cphi;
                 ang = K_TWO_PI * rng.next();
@@ -943,6 +1074,16 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
     ppl[2] = (g.srcnum > 1) ? ppl[2] : p.w;
     v.w = K_EPS;
     s.Lmove = 0.f;
+
+    if (FULL && g.imoff) {
+        /* RF replay: the phase omega * tof of this photon (src/mcx_core.cl:2257-2263,
+           which reads the next photon's time of flight; here the photon's own) */
+        float rs, rc;
+        ksincos(g.omega * g.replaytof[photon_index(g, idx, f.w) - 1], &rs, &rc);
+        ppl[4 + g.srcnum] = rc;
+        ppl[5 + g.srcnum] = rs;
+    }
+
     return 0;
 }
 
@@ -950,7 +1091,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
    workgroup's pool (run-time hand-out) is exhausted.  The photon state starts
    fresh; the RNG, the LDS tables and the lane's scratch come in, and the
    lane's pending grid deposits are flushed before returning. */
-template <bool SAVEDET, bool REFLECT, bool SIMPLE>
+template <bool SAVEDET, bool REFLECT, bool SIMPLE, bool FULL>
 __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared& sh, Rng& rng, const int idx) {
     Photon s;
     s.p = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x7FC00000u));
@@ -979,7 +1120,7 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
 
     for (;;) {
         if (relaunch) {
-            if (launch_photon<SAVEDET, SIMPLE>(g, sh, s, rng, idx, launchflag)) {
+            if (launch_photon<SAVEDET, SIMPLE, FULL>(g, sh, s, rng, idx, launchflag)) {
                 break;
             }
 
@@ -1053,10 +1194,26 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
 
                 v.w += 1.f;
 
-                if (!SIMPLE && (g.outputtype == kWP || g.outputtype == kDCS)) {
+                if (!SIMPLE && (g.outputtype == kWP || g.outputtype == kDCS ||
+                                (FULL && (g.outputtype == kWPTOF ||
+                                          (g.seed == K_SEED_FROM_FILE && g.outputtype == kRFmus))))) {
+                    /* replay deposits at each scattering site (src/mcx_core.cl:2568-2623):
+                       w (WP), (1 - cos theta) w (DCS), tof w (WPTOF), w e^{i omega tof} (RFmus) */
                     uint32_t pid = photon_index(g, idx, f.w) - 1;
+                    float im = 0.f;
                     tmp0 = (g.outputtype == kDCS) ? (1.f - ctheta) : 1.f;
-                    tmp0 *= g.replayweight[pid];
+
+                    if (FULL && g.outputtype == kRFmus) {
+                        const float rw = g.replayweight[pid];
+                        tmp0 = rw * sh.pp[g.w0offset + g.srcnum];
+                        im = rw * sh.pp[g.w0offset + g.srcnum + 1];
+                    } else if (FULL && g.outputtype == kWPTOF) {
+                        tmp0 = g.replaytof[pid];
+                        tmp0 *= g.replayweight[pid];
+                    } else {
+                        tmp0 *= g.replayweight[pid];
+                    }
+
                     int tshift = gate_of(g, g.replaytof[pid]) +
                                  ((g.replaydet == -1) ? (((g.replaydetid[pid] & 0xFFFF) - 1) * (int)g.maxgate) : 0);
 
@@ -1066,6 +1223,10 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
                     }
 
                     deposit(g, sh.repoff, s.idx1d + (uint32_t)tshift * g.nvox, tmp0);
+
+                    if (FULL && g.outputtype == kRFmus) {
+                        deposit(g, sh.repoff, s.idx1d + (uint32_t)tshift * g.nvox + g.imoff, im);
+                    }
                 }
             }
 
@@ -1130,7 +1291,7 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
 
         if (s.idx1d != idx1dold && idx1dold < g.nvox && mediaidold) {
             if (g.save2pt && f.y >= g.twin0 && f.y < g.twin1) {
-                float weight = 0.f;
+                float weight = 0.f, weight_im = 0.f;
                 int tshift = gate_of(g, f.y);
 
                 if (g.outputtype == kEnergy) {
@@ -1138,11 +1299,20 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
                 } else if (g.outputtype == kFluence || g.outputtype == kFlux) {
                     weight = (s.prop.x < K_EPS) ? (s.w0 * pathlen) : kdiv(s.w0 - p.w, s.prop.x);
                 } else if (!SIMPLE && g.seed == K_SEED_FROM_FILE) {
-                    if (g.outputtype == kJacobian) {
+                    if (g.outputtype == kJacobian || (FULL && (g.outputtype == kRF || g.outputtype == kWLTOF))) {
+                        /* replay path deposits (src/mcx_core.cl:2845-2858): w L (J),
+                           -w L e^{i omega tof} (RF), tof w L (WLTOF) */
                         uint32_t pid = photon_index(g, idx, f.w) - 1;
                         weight = g.replayweight[pid] * pathlen;
                         tshift = gate_of(g, g.replaytof[pid]) +
                                  ((g.replaydet == -1) ? (((g.replaydetid[pid] & 0xFFFF) - 1) * (int)g.maxgate) : 0);
+
+                        if (FULL && g.outputtype == kRF) {
+                            weight_im = -weight * sh.pp[g.w0offset + g.srcnum + 1];
+                            weight = -weight * sh.pp[g.w0offset + g.srcnum];
+                        } else if (FULL && g.outputtype == kWLTOF) {
+                            weight = weight * g.replaytof[pid];
+                        }
                     }
                 } else if (g.outputtype == kL) {
                     weight = s.w0 * pathlen;
@@ -1153,8 +1323,24 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
                               (int)g.maxgate;
                 }
 
-                if (fabsf(weight) > 0.f) {
+                if (FULL && g.npattern > 1) {
+                    /* several patterns share the photon: one interleaved slot per pattern
+                       (src/mcx_core.cl:2902-2914), straight to the grid */
+                    const uint32_t base = (idx1dold + (uint32_t)tshift * g.nvox) * g.npattern;
+
+                    for (uint32_t i = 0; i < g.npattern; i++) {
+                        const float pw = sh.pp[g.w0offset + i];
+
+                        if (fabsf(pw) > 0.f && fabsf(weight) > 0.f) {
+                            deposit(g, sh.repoff, base + i, weight * pw);
+                        }
+                    }
+                } else if (fabsf(weight) > 0.f) {
                     deposit_voxel(g, sh, dcache, ox, oy, oz, idx1dold, tshift, weight);
+                }
+
+                if (FULL && g.outputtype == kRF && g.omega > 0.f && fabsf(weight_im) > 0.f) {
+                    deposit(g, sh.repoff, idx1dold + (uint32_t)tshift * g.nvox + g.imoff, weight_im);
                 }
             }
 
@@ -1276,15 +1462,15 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
 #define MCX_KATTR
 #endif
 #if MCX_PARITY
-#define MCX_KBOUND(SAVEDET, REFLECT, SIMPLE) MCX_MAX_BLOCK
+#define MCX_KBOUND(SAVEDET, REFLECT, SIMPLE, FULL) MCX_MAX_BLOCK
 #else
-#define MCX_KBOUND(SAVEDET, REFLECT, SIMPLE) MCX_FAST_BOUND(SAVEDET, REFLECT, SIMPLE)
+#define MCX_KBOUND(SAVEDET, REFLECT, SIMPLE, FULL) MCX_FAST_BOUND_FULL(SAVEDET, REFLECT, SIMPLE, FULL)
 #endif
-template <bool SAVEDET, bool REFLECT, bool SIMPLE>
-__global__ void __launch_bounds__(MCX_KBOUND(SAVEDET, REFLECT, SIMPLE)) MCX_KATTR transport_kernel(const MCXKParam g) {
+template <bool SAVEDET, bool REFLECT, bool SIMPLE, bool FULL>
+__global__ void __launch_bounds__(MCX_KBOUND(SAVEDET, REFLECT, SIMPLE, FULL)) MCX_KATTR transport_kernel(const MCXKParam g) {
     extern __shared__ float4 lds[];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t pplen = g.w0offset + g.srcnum;
+    const uint32_t pplen = g.pplen;
     __shared__ uint32_t pool[2];
     Shared sh;
     sh.pool = pool;
@@ -1340,7 +1526,7 @@ __global__ void __launch_bounds__(MCX_KBOUND(SAVEDET, REFLECT, SIMPLE)) MCX_KATT
     rng.s1 = (uint64_t)g.nseed[sidx * 4 + 2] << 32 | g.nseed[sidx * 4 + 3];
 
     if (active) {
-        transport_lane<SAVEDET, REFLECT, SIMPLE>(g, sh, rng, idx);
+        transport_lane<SAVEDET, REFLECT, SIMPLE, FULL>(g, sh, rng, idx);
         g.genergy[idx * 2] = sh.pp[pd];
         g.genergy[idx * 2 + 1] = sh.pp[pd + 1];
     }
@@ -1447,7 +1633,7 @@ __global__ void __launch_bounds__(MCX_BATCH_BLOCK_MAX) __attribute__((amdgpu_wav
 
         const MCXKParam& g = prm[inst];
         const long long tstart = cost ? wall_clock64() : 0;
-        const uint32_t pplen = g.w0offset + g.srcnum;
+        const uint32_t pplen = g.pplen;
         Shared sh;
         sh.pool = pools + 2 * inst;
         sh.prop = lds;
@@ -1476,7 +1662,7 @@ __global__ void __launch_bounds__(MCX_BATCH_BLOCK_MAX) __attribute__((amdgpu_wav
 
         /* the general path only: with the pencil-beam special path inlined beside
            it the kernel needs more registers and runs fewer waves per SIMD */
-        transport_lane<false, REFLECT, false>(g, sh, rng, lane);
+        transport_lane<false, REFLECT, false, false>(g, sh, rng, lane);
 
         const uint32_t pd = g.partialdata;
         double esc = wave_sum((double)sh.pp[pd]);

@@ -9,11 +9,14 @@
 template <bool SAVEDET, bool REFLECT>
 static hipError_t launch_variant(const MCXKParam* prm, int nblock, int nthreadblock, size_t shmem, hipStream_t stream) {
     if (prm->simple) {
-        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, true>), dim3(nblock), dim3(nthreadblock), shmem,
-                           stream, *prm);
+        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, true, false>), dim3(nblock), dim3(nthreadblock),
+                           shmem, stream, *prm);
+    } else if (prm->full) {
+        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, false, true>), dim3(nblock), dim3(nthreadblock),
+                           shmem, stream, *prm);
     } else {
-        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, false>), dim3(nblock), dim3(nthreadblock), shmem,
-                           stream, *prm);
+        hipLaunchKernelGGL((mcxk::transport_kernel<SAVEDET, REFLECT, false, false>), dim3(nblock), dim3(nthreadblock),
+                           shmem, stream, *prm);
     }
 
     return hipGetLastError();

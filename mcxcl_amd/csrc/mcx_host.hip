@@ -241,9 +241,8 @@ void mcx_check_supported(const Config* cfg) {
         MCX_ERROR(-4, "unsupported media format for the GPU kernel");
     }
 
-    if (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->srctype == MCX_SRC_FOURIERX ||
-            cfg->srctype == MCX_SRC_FOURIERX2D || cfg->srctype == MCX_SRC_HYPERBOLOID_GAUSSIAN) {
-        MCX_ERROR(-4, "this source type is not supported by the MI355X kernel");
+    if ((cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D) && !cfg->srcpattern) {
+        MCX_ERROR(-4, "pattern sources need the srcpattern array");
     }
 
     if (cfg->seed == SEED_FROM_FILE && cfg->respin != 1) {
@@ -258,10 +257,14 @@ void mcx_check_supported(const Config* cfg) {
         MCX_ERROR(-4, "cone source needs a positive half angle in srcparam1.x");
     }
 
-    if (cfg->nangle || cfg->polmedianum || cfg->srcnum > 1 || cfg->issaveref > 1 || cfg->issavedet == FILL_MAXDETPHOTON ||
+    if (cfg->srcnum > 1 && cfg->srctype != MCX_SRC_PATTERN && cfg->srctype != MCX_SRC_PATTERN3D) {
+        MCX_ERROR(-4, "photon sharing (srcnum > 1) needs a pattern or pattern3d source");
+    }
+
+    if (cfg->polmedianum || cfg->issaveref > 1 || cfg->issavedet == FILL_MAXDETPHOTON ||
             (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY))) {
-        MCX_ERROR(-4, "angle-CDF launch, polarization, photon sharing, issaveref>1, fill-max mode and trajectory "
-                  "saving are not supported by the MI355X kernel");
+        MCX_ERROR(-4, "polarization, issaveref>1, fill-max mode and trajectory saving are not supported by the "
+                  "MI355X kernel");
     }
 }
 
@@ -273,9 +276,15 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     const unsigned int maxgate = (unsigned int)((cfg->tend - cfg->tstart) / cfg->tstep + 0.5);
     cfg->maxgate = maxgate;
     const size_t nvox = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
-    const unsigned int nsrcslot = (cfg->srcid < 0) ? cfg->extrasrclen + 1 : 1;
+    /* several patterns sharing each photon keep one slot each, interleaved after the voxel */
+    const unsigned int npattern = ((cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D) &&
+                                   cfg->srcnum > 1) ? cfg->srcnum : 1;
+    const unsigned int nsrcslot = (cfg->srcid < 0) ? cfg->extrasrclen + 1 : npattern;
     const unsigned int ndetslot = (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) ? cfg->detnum : 1;
-    const size_t fieldlen = nvox * maxgate * nsrcslot * ndetslot;
+    /* complex RF replay outputs: the imaginary half follows the real one */
+    const int isrf = cfg->seed == SEED_FROM_FILE && (cfg->outputtype == otRF || cfg->outputtype == otRFmus);
+    const size_t halflen = nvox * maxgate * nsrcslot * ndetslot;
+    const size_t fieldlen = halflen * (isrf ? 2 : 1);
     const unsigned int partialdata = mcx_partialdata(cfg);
     const unsigned int reclen = mcx_detreclen(cfg);
 
@@ -340,6 +349,13 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     g.replaydet = cfg->replaydet;
     g.srcid = cfg->srcid;
     g.fieldlen = (uint32_t)fieldlen;
+    g.npattern = npattern;
+    g.nangle = cfg->nangle;
+    g.imoff = isrf ? (uint32_t)halflen : 0u;
+    g.omega = cfg->omega;
+    /* lane scratch: record columns, 2 tallies, w0, source id, one weight per
+       shared pattern (srcnum), the RF replay phase (cos, sin) */
+    g.pplen = g.w0offset + g.srcnum + (isrf ? 2u : 0u);
     memcpy(g.bc, cfg->bc, 12);
 
     {
@@ -363,10 +379,17 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
         lab.f = cfg->srcparam2.w;
         g.simple = cfg->srctype == MCX_SRC_PENCIL && cfg->srcdir.w == 0.f && (lab.u & MED_MASK) != 0 &&
                    cfg->extrasrclen == 0 && cfg->seed != SEED_FROM_FILE && cfg->issaveref == 0 && g.is2d == 0 &&
-                   cfg->nphase <= 2 && cfg->srcpos.w > 0.f && cfg->minenergy < cfg->srcpos.w &&
+                   cfg->nphase <= 2 && cfg->nangle == 0 && cfg->srcpos.w > 0.f && cfg->minenergy < cfg->srcpos.w &&
                    (cfg->outputtype == otFlux || cfg->outputtype == otFluence || cfg->outputtype == otEnergy ||
                     cfg->outputtype == otL);
     }
+
+    /* the full general path: launches and outputs the common general path leaves out */
+    g.full = !g.simple &&
+             (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->srctype == MCX_SRC_FOURIERX ||
+              cfg->srctype == MCX_SRC_FOURIERX2D || cfg->srctype == MCX_SRC_HYPERBOLOID_GAUSSIAN || cfg->nangle ||
+              cfg->outputtype == otRF || cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
+              cfg->outputtype == otWPTOF);
 
     /* reflection block rule (src/mcx_host.cpp:945-956) */
     {
@@ -381,7 +404,7 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
         }
     }
 
-    size_t pplen = g.w0offset + g.srcnum;
+    size_t pplen = g.pplen;
     size_t shmem = cfg->medianum * sizeof(float4) + ((cfg->nphase + 3) & ~3u) * sizeof(float) +
                    lblock * pplen * sizeof(float);
 
@@ -406,8 +429,8 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
             budget = (size_t)strtoul(env, NULL, 10);
         }
 
-        if (g.debugrng || !cfg->issave2pt || budget < elem) {
-            budget = 0;
+        if (g.debugrng || !cfg->issave2pt || budget < elem || npattern > 1) {
+            budget = 0;   /* shared patterns deposit straight to their interleaved slots */
         }
 
         const size_t cells = budget / elem;
@@ -475,6 +498,16 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     return shmem;
 }
 
+/* pixels of a pattern source: param1.w x param2.w (pattern), param1.x x
+   param1.y x param1.z (pattern3d) */
+static size_t mcx_pattern_pixels(const Config* cfg) {
+    if (cfg->srctype == MCX_SRC_PATTERN3D) {
+        return (size_t)((int)cfg->srcparam1.x * (int)cfg->srcparam1.y * (int)cfg->srcparam1.z);
+    }
+
+    return (size_t)((int)cfg->srcparam1.w * (int)cfg->srcparam2.w);
+}
+
 /* normalisation factor of the float output (src/mcx_host.cpp:1382-1465, after
    the energy tallies are known): flux = unitinmm / (E_launched Vvox tstep),
    fluence = flux x tstep, energy and path length = 1 / E_launched, replay
@@ -493,7 +526,9 @@ static float mcx_output_scale(const Config* cfg) {
         }
     } else if (cfg->outputtype == otEnergy || cfg->outputtype == otL) {
         scale = 1.f / (float)cfg->energytot;
-    } else if (cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS) {
+    } else if (cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS ||
+               cfg->outputtype == otRF || cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
+               cfg->outputtype == otWPTOF) {
         if (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
             return 0.f;   /* per-detector scaling after the read-back */
         }
@@ -533,7 +568,8 @@ static void mcx_plan_run(Config* cfg, int dev, int ncu, size_t* nthread_io, MCXK
     size_t nthread = *nthread_io;
     size_t shmem = mcx_kparam_setup(cfg, &g, nthread, lblock, &reflect, fast ? 0 : MCX_BOX_BYTES);
     {
-        const size_t bound = fast ? (size_t)MCX_FAST_BOUND(cfg->issavedet != 0, reflect != 0, g.simple != 0) : MCX_MAX_BLOCK;
+        const size_t bound = fast ? (size_t)MCX_FAST_BOUND_FULL(cfg->issavedet != 0, reflect != 0, g.simple != 0,
+                                 g.full != 0) : MCX_MAX_BLOCK;
 
         if (envblock) {
             lblock = (size_t)strtoul(envblock, NULL, 10);
@@ -541,7 +577,7 @@ static void mcx_plan_run(Config* cfg, int dev, int ncu, size_t* nthread_io, MCXK
             /* the wide workgroup holds the same lanes per CU as four 256-lane ones,
                so its lane scratch costs no more LDS per CU; it only has to fit in
                one workgroup's LDS (many media with full records may not) */
-            const size_t lane = ((size_t)g.w0offset + g.srcnum) * sizeof(float);
+            const size_t lane = (size_t)g.pplen * sizeof(float);
             lblock = (shmem + (bound - MCX_LAUNCH_BLOCK) * lane + MCX_LDS_SLACK <= MCX_LDS_LIMIT) ? bound
                      : MCX_LAUNCH_BLOCK;
         }
@@ -564,13 +600,13 @@ static void mcx_plan_run(Config* cfg, int dev, int ncu, size_t* nthread_io, MCXK
 
         if (hipDeviceGetAttribute(&ldscu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess &&
                 hipDeviceGetAttribute(&ldsblock, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess) {
-            nwg = mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, (int)g.simple);
+            nwg = mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, g.full ? 2 : (int)g.simple);
 
             if (nwg <= 0 && dynamic && !envblock && lblock > MCX_MAX_BLOCK) {
                 /* no CU holds the wide workgroup: fall back to 256 lanes */
                 lblock = MCX_MAX_BLOCK;
                 shmem = mcx_kparam_setup(cfg, &g, nthread, lblock, &reflect, 0);
-                nwg = mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, (int)g.simple);
+                nwg = mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, g.full ? 2 : (int)g.simple);
             }
         }
 
@@ -610,7 +646,7 @@ static void mcx_plan_run(Config* cfg, int dev, int ncu, size_t* nthread_io, MCXK
        mode keeps the reference's fixed photon-per-thread map. */
     {
         int nb = (dynamic && !g.debugrng) ?
-                 mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, (int)g.simple) : 0;
+                 mcx_fast_occupancy((int)lblock, shmem, cfg->issavedet != 0, reflect, g.full ? 2 : (int)g.simple) : 0;
 
         if (nb > 0 && ncu > 0) {
             nlaunch = (size_t)nb * (size_t)ncu;
@@ -732,6 +768,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     float4* dprop = NULL;
     float4* ddetpos = NULL;
     float* dinvcdf = NULL;
+    float* dangle = NULL;
+    float* dpattern = NULL;
     float4* dsrcdata = NULL;
     float* dreplayw = NULL;
     float* dreplaytof = NULL;
@@ -773,6 +811,17 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     if (cfg->nphase) {
         HIP_TRY(dalloc(bufs, &dinvcdf, cfg->nphase));
         HIP_TRY(hipMemcpyAsync(dinvcdf, cfg->invcdf, cfg->nphase * sizeof(float), hipMemcpyHostToDevice, stream));
+    }
+
+    if (cfg->nangle) {
+        HIP_TRY(dalloc(bufs, &dangle, cfg->nangle));
+        HIP_TRY(hipMemcpyAsync(dangle, cfg->angleinvcdf, cfg->nangle * sizeof(float), hipMemcpyHostToDevice, stream));
+    }
+
+    if ((cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D) && cfg->srcpattern) {
+        const size_t npat = mcx_pattern_pixels(cfg) * g.npattern;
+        HIP_TRY(dalloc(bufs, &dpattern, npat));
+        HIP_TRY(hipMemcpyAsync(dpattern, cfg->srcpattern, npat * sizeof(float), hipMemcpyHostToDevice, stream));
     }
 
     if (cfg->extrasrclen) {
@@ -850,6 +899,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     g.prop = dprop;
     g.detpos = ddetpos;
     g.invcdf = dinvcdf;
+    g.angleinvcdf = dangle;
+    g.srcpattern = dpattern;
     g.srcdata = dsrcdata;
     g.replayweight = dreplayw;
     g.replaytof = dreplaytof;
@@ -994,8 +1045,9 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         }
 
         if (wantfloat) {
-            HIP_TRY(mcx_launch_finalize(dfield, dout, fieldlen, g.nrep, (normalize && scale != 0.f) ? scale : 1.f,
-                                        stream));
+            /* shared patterns are scaled per slot on the host after the read-back */
+            HIP_TRY(mcx_launch_finalize(dfield, dout, fieldlen, g.nrep,
+                                        (normalize && scale != 0.f && g.npattern <= 1) ? scale : 1.f, stream));
         } else if (g.nrep > 1) {
             /* fold the replicas into replica 0 without a float output */
             HIP_TRY(mcx_launch_finalize(dfield, NULL, fieldlen, g.nrep, 1.f, stream));
@@ -1105,9 +1157,28 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
                         sd = cfg->unitinmm / sd;
                     }
 
-                    mcx_normalize(hfield + ((size_t)(s - 1) * cfg->detnum + (d - 1)) * nvox * maxgate, sd,
-                                  (int)(nvox * maxgate), 1, 0, 1);
+                    for (size_t part = 0; part < (g.imoff ? 2u : 1u); part++) {
+                        mcx_normalize(hfield + part * g.imoff + ((size_t)(s - 1) * cfg->detnum + (d - 1)) * nvox * maxgate,
+                                      sd, (int)(nvox * maxgate), 1, 0, 1);
+                    }
                 }
+            }
+        }
+
+        if (wantfloat && normalize && scale != 0.f && g.npattern > 1) {
+            /* pattern k: npixel / (its summed weights) x the unit-photon scale
+               (src/mcx_host.cpp:1351-1364, :1436-1447), Kahan sums in float */
+            const int psize = (int)mcx_pattern_pixels(cfg);
+
+            for (uint32_t k = 0; k < g.npattern; k++) {
+                float sum = 0.f, c = 0.f;
+
+                for (int j = 0; j < psize; j++) {
+                    mcx_kahanSum(&sum, &c, cfg->srcpattern[(size_t)j * g.npattern + k]);
+                }
+
+                mcx_normalize(hfield, (float)psize / sum * scale, (int)(fieldlen / g.npattern), cfg->isnormalized, (int)k,
+                              (int)g.npattern);
             }
         }
 
@@ -1392,6 +1463,11 @@ static void noalloc_run(Config* cfg, const uint32_t* media, size_t nmedia, doubl
 
     if (!media || !field64) {
         MCX_ERROR(-4, "mcx_run_simulation_noalloc needs the volume and the float64 field in device memory");
+    }
+
+    if (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->nangle) {
+        MCX_ERROR(-4, "mcx_run_simulation_noalloc does not stage pattern or launch-angle tables: use "
+                  "mcx_run_simulation for pattern sources and angle-CDF launches");
     }
 
     if (cfg->medianum > MCX_NOALLOC_MEDIA || cfg->nphase > MCX_NOALLOC_PHASE || cfg->extrasrclen > MCX_NOALLOC_SRC) {

@@ -1380,6 +1380,23 @@ void mcx_preprocess(Config* cfg) {
         MCX_ERROR(-4, "the 'srcpattern' field can not be empty when your 'srctype' is 'pattern'");
     }
 
+    /* the pixel index of a pattern launch is (int)(r * 0.9998 * n) per axis with
+       the row length (int)n (src/mcx_core.cl:1678-1727): it stays inside the
+       pattern only for whole, positive pixel counts */
+    if (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D) {
+        const float dims[3] = {cfg->srctype == MCX_SRC_PATTERN ? cfg->srcparam1.w : cfg->srcparam1.x,
+                               cfg->srctype == MCX_SRC_PATTERN ? cfg->srcparam2.w : cfg->srcparam1.y,
+                               cfg->srctype == MCX_SRC_PATTERN ? 1.f : cfg->srcparam1.z
+                              };
+
+        for (int i = 0; i < 3; i++) {
+            if (!(dims[i] >= 1.f) || dims[i] != floorf(dims[i]) || dims[i] > 65536.f) {
+                MCX_ERROR(-4, "pattern sources need whole, positive pixel counts (srcparam1.w/srcparam2.w for "
+                          "'pattern', srcparam1.x/y/z for 'pattern3d')");
+            }
+        }
+    }
+
     if (cfg->steps.x != 1.f && cfg->unitinmm == 1.f) {
         cfg->unitinmm = cfg->steps.x;
     }
@@ -1517,6 +1534,15 @@ void mcx_preprocess(Config* cfg) {
                 s->srcparam2.w = f2i.f;
             }
         }
+    }
+
+    /* the hyperboloid launch keeps srcparam2.z/.w as its launch voxel and label
+       (src/mcx_core.cl:1820-1852); they are the zero defaults, so the launch
+       finds its voxel by skipvoid.  Anything else there would index outside the
+       volume or the medium table, so it is cleared. */
+    if (cfg->srctype == MCX_SRC_HYPERBOLOID_GAUSSIAN) {
+        cfg->srcparam2.z = 0.f;
+        cfg->srcparam2.w = 0.f;
     }
 
     if (cfg->issavedet && cfg->mediabyte <= 4) {
@@ -1744,7 +1770,9 @@ void mcx_loadseedfile(Config* cfg) {
     replay_alloc(cfg, nrec);
 
     int hasdetid = SAVE_DETID(his.savedetflag);
-    int isreplayout = (cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS);
+    int isreplayout = (cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS ||
+                       cfg->outputtype == otRF || cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
+                       cfg->outputtype == otWPTOF);
 
     if (!isreplayout) {
         /* a plain replay relaunches every saved photon with its own weight */
@@ -1934,9 +1962,8 @@ void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
         MCX_ERROR(-6, "Jacobian output is only valid in the reply mode. Please define cfg.seed");
     }
 
-    if (cfg->outputtype == otRF || cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
-            cfg->outputtype == otWPTOF || cfg->omega > 0.f || cfg->srcid == -2) {
-        MCX_ERROR(-6, "RF, time-weighted replay outputs and srcid -2 are not supported by this build");
+    if ((cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE) || cfg->srcid == -2) {
+        MCX_ERROR(-6, "RF forward simulation (omega > 0 without replay) and srcid -2 are not supported by this build");
     }
 
     /* adjoint outputs: every detector becomes an extra disk source at its

@@ -27,6 +27,7 @@ SAVEFLAGS = "DSPMXVWI"
 # first adjoint output type (enum TOutputType, src/mcx_utils.h:58-62) and the
 # output keys of each adjoint type (src/pmcxcl.cpp:1426-1456)
 OT_ADJOINT = 11
+OT_RF, OT_RFMUS = 6, 8
 ADJOINT_KEYS = {11: ("jmua",), 12: ("jd",), 13: ("jmus",), 14: ("jmusp",), 15: ("jmua", "jd"), 16: ("jmua", "jmusp")}
 DEBUGFLAGS = "RMP"
 SEED_FROM_FILE = -999
@@ -192,7 +193,16 @@ def _parse_config(user, c):
         c.nphase = full.size
         c.invcdf = _carray(full, np.float32)
     if "angleinvcdf" in user:
-        raise ValueError("angleinvcdf launch distributions are not supported by this build")
+        # launch polar angle / pi at equal CDF steps (src/pmcxcl.cpp:963-983)
+        v = np.asarray(user["angleinvcdf"], dtype=np.float32).ravel(order="F")
+        if np.any(np.diff(v) < 0) or np.any(v > 1) or np.any(v < 0):
+            raise ValueError("cfg.angleinvcdf contains invalid data; it must be a monotonically increasing vector "
+                             "with all values between 0 and 1")
+        c.nangle = v.size
+        c.angleinvcdf = _carray(v, np.float32)
+    if "srcpattern" in user:
+        # pattern weights, Fortran order (pattern index fastest when srcnum > 1; src/pmcxcl.cpp:917-936)
+        c.srcpattern = _carray(np.asarray(user["srcpattern"], dtype=np.float32).ravel(order="F"), np.float32)
     if "shapes" in user:
         s = str(user["shapes"]).encode()
         ptr = _native.cmalloc(len(s) + 2)
@@ -264,7 +274,11 @@ class PreparedConfig:
         self.nvox = c.dim.x * c.dim.y * c.dim.z
         self.ndet = c.detnum if (c.seed == SEED_FROM_FILE and c.replaydet == -1) else 1
         self.nsrc = (c.extrasrclen + 1) if (c.extrasrclen and c.srcid < 0) else 1
-        self.fieldlen = self.nvox * self.ngate * self.ndet * self.nsrc
+        # several patterns share each photon: one interleaved slot per pattern
+        self.npattern = c.srcnum if (c.srctype[0] in (5, 15) and c.srcnum > 1) else 1
+        # complex RF replay outputs: the imaginary half after the real one
+        self.nrf = 2 if (c.seed == SEED_FROM_FILE and c.outputtype[0] in (OT_RF, OT_RFMUS)) else 1
+        self.fieldlen = self.nvox * self.ngate * self.ndet * self.nsrc * self.npattern * self.nrf
         self.raw = None
         self.raw32 = None
         if user.get("rawfield"):
@@ -310,7 +324,7 @@ class PreparedConfig:
         out = _collect(self.c, self.user, self.ngate, self.ndet, self.nsrc)
         if self.detw is not None:
             out["detw"] = self.detw[:self.c.detectedcount].copy()
-        dims = [self.c.dim.x, self.c.dim.y, self.c.dim.z, self.ngate * self.ndet * self.nsrc]
+        dims = [self.c.dim.x, self.c.dim.y, self.c.dim.z, -1]
         if self.raw is not None:
             out["flux64"] = self.raw.reshape(dims, order="F")   # the array the run filled, no copy
         if self.raw32 is not None:
@@ -451,10 +465,12 @@ def _collect(c, user, ngate, ndet, nsrc):
         out["detp"] = np.asfortranarray(det.reshape(n, reclen).T)
     if c.issave2pt and c.exportfield:
         dims = [c.srcnum * c.dim.x, c.dim.y, c.dim.z, ngate]
+        nrf = 2 if (c.seed == SEED_FROM_FILE and c.outputtype[0] in (OT_RF, OT_RFMUS)) else 1
         if ndet > 1:
             dims.append(ndet)
-        if nsrc > 1:
-            dims += [1] * (5 - len(dims)) + [nsrc]
+        if nsrc > 1 or nrf > 1:
+            # RF replay: real and imaginary halves along the last axis (src/pmcxcl.cpp:1339-1345)
+            dims += [1] * (5 - len(dims)) + [nsrc * nrf]
         total = int(np.prod(dims))
         flux = np.ctypeslib.as_array((ctypes.c_float * total).from_address(c.exportfield)).copy()
         flux = flux.reshape(dims, order="F")

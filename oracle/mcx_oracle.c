@@ -132,6 +132,12 @@ typedef struct {
     const int* replaydetid;
     const uint64_t* replayseed;
     const float* muamus;  /* optional per-voxel {mua, mus} in 1/grid unit (2 per voxel) */
+    const float* srcpattern;   /* pattern/pattern3d weights, srcnum per pixel (source index fastest) */
+    const float* angleinvcdf;  /* launch polar angle inverse CDF in units of pi (nangle entries) */
+    uint32_t nangle;
+    uint32_t npattern;         /* patterns sharing each photon (srcnum of a pattern source, else 1) */
+    uint32_t rfmul;            /* 2 for the complex RF replay outputs (imaginary half after the real) */
+    float omega;               /* RF replay: angular modulation frequency (rad/s) */
 } OParam;
 
 typedef struct {
@@ -467,6 +473,25 @@ static inline uint32_t photon_index(const OParam* g, int idx, float fw) {
     return (uint32_t)(idx * (int)g->threadphoton + (idx < odd ? idx : odd) + (int)fw);
 }
 
+/* launch weights of a pattern pixel (src/mcx_core.cl:1678-1727): one pattern
+   scales the photon weight; several share the photon (unit weight), each
+   pattern's weight kept per photon, the pixel index in the w0 slot as bits */
+static void pattern_weights(const OParam* g, OPhoton* s, float* pp, int pix, float srcw) {
+    if (g->srcnum <= 1) {
+        s->p.w = srcw * g->srcpattern[pix];
+        pp[4] = s->p.w;
+    } else {
+        const uint32_t bits = (uint32_t)pix;
+        memcpy(&pp[2], &bits, sizeof(bits));
+
+        for (uint32_t i = 0; i < g->srcnum; i++) {
+            pp[4 + i] = g->srcpattern[(size_t)pix * g->srcnum + i];
+        }
+
+        s->p.w = 1.f;
+    }
+}
+
 /* src/mcx_core.cl:1466-2275; returns 0 when a photon was launched, 1 when the
    thread's photon budget is used up, -1 when no launch succeeded */
 static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_t isdet) {
@@ -564,14 +589,31 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
         int st = (int)g->srctype;
         int positioned = 0;   /* sources that move the launch point recompute idx1d/mediaid */
 
-        if (st == MCX_SRC_PLANAR || st == MCX_SRC_FOURIER || st == MCX_SRC_PENCILARRAY) {
+        if (st == MCX_SRC_PLANAR || st == MCX_SRC_PATTERN || st == MCX_SRC_PATTERN3D || st == MCX_SRC_FOURIER ||
+                st == MCX_SRC_PENCILARRAY) {
             float rx = rng_next(s->t, w);
             float ry = rng_next(s->t, w);
-            p->x = p->x + rx * lp1.x + ry * lp2.x;
-            p->y = p->y + rx * lp1.y + ry * lp2.y;
-            p->z = p->z + rx * lp1.z + ry * lp2.z;
 
-            if (st == MCX_SRC_FOURIER) {
+            if (st == MCX_SRC_PATTERN3D) {
+                /* a box of param1.x x param1.y x param1.z voxels (src/mcx_core.cl:1661-1667) */
+                float rz = rng_next(s->t, w);
+                p->x = p->x + rx * lp1.x;
+                p->y = p->y + ry * lp1.y;
+                p->z = p->z + rz * lp1.z;
+                int pix = (int)(rz * JUST_BELOW_ONE * lp1.z) * (int)(lp1.y) * (int)(lp1.x) +
+                          (int)(ry * JUST_BELOW_ONE * lp1.y) * (int)(lp1.x) + (int)(rx * JUST_BELOW_ONE * lp1.x);
+                pattern_weights(g, s, pp, pix, lsrcpos.w);
+            } else {
+                p->x = p->x + rx * lp1.x + ry * lp2.x;
+                p->y = p->y + rx * lp1.y + ry * lp2.y;
+                p->z = p->z + rx * lp1.z + ry * lp2.z;
+            }
+
+            if (st == MCX_SRC_PATTERN) {
+                /* param1.w x param2.w pixels over the parallelogram (src/mcx_core.cl:1678-1693) */
+                int pix = (int)(ry * JUST_BELOW_ONE * lp2.w) * (int)(lp1.w) + (int)(rx * JUST_BELOW_ONE * lp1.w);
+                pattern_weights(g, s, pp, pix, lsrcpos.w);
+            } else if (st == MCX_SRC_FOURIER) {
                 p->w = lsrcpos.w * (pm_cos((floorf(lp1.w) * rx + floorf(lp2.w) * ry + lp1.w - floorf(lp1.w)) * TWO_PI) *
                                     (1.f - lp2.w + floorf(lp2.w)) + 1.f) * 0.5f;
             } else if (st == MCX_SRC_PENCILARRAY) {
@@ -585,6 +627,69 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
             s->prop.y = s->prop.y + (lp1.y + lp2.y) * 0.5f;
             s->prop.z = s->prop.z + (lp1.z + lp2.z) * 0.5f;
             s->prop.w = 0.f;
+        } else if (st == MCX_SRC_FOURIERX || st == MCX_SRC_FOURIERX2D) {
+            /* param1 = one edge (its .w the other edge's length relative to it), the
+               other edge = param1.w dir x param1/|param1|; param2 = the pattern
+               (src/mcx_core.cl:1776-1812) */
+            float rx = rng_next(s->t, w);
+            float ry = rng_next(s->t, w);
+            F4 v2 = lp1;
+            v2.w *= pm_rsqrt(lp1.x * lp1.x + lp1.y * lp1.y + lp1.z * lp1.z);
+            v2.x = v2.w * (lsrcdir.y * lp1.z - lsrcdir.z * lp1.y);
+            v2.y = v2.w * (lsrcdir.z * lp1.x - lsrcdir.x * lp1.z);
+            v2.z = v2.w * (lsrcdir.x * lp1.y - lsrcdir.y * lp1.x);
+            p->x = p->x + rx * lp1.x + ry * v2.x;
+            p->y = p->y + rx * lp1.y + ry * v2.y;
+            p->z = p->z + rx * lp1.z + ry * v2.z;
+
+            if (st == MCX_SRC_FOURIERX2D) {
+                p->w = lsrcpos.w * (pm_sin((lp2.x * rx + lp2.z) * TWO_PI) * pm_sin((lp2.y * ry + lp2.w) * TWO_PI) + 1.f) *
+                       0.5f;
+            } else {
+                p->w = lsrcpos.w * (pm_cos((lp2.x * rx + lp2.y * ry + lp2.z) * TWO_PI) * (1.f - lp2.w) + 1.f) * 0.5f;
+            }
+
+            positioned = 1;
+            s->prop.x = s->prop.x + (lp1.x + v2.x) * 0.5f;
+            s->prop.y = s->prop.y + (lp1.y + v2.y) * 0.5f;
+            s->prop.z = s->prop.z + (lp1.z + v2.z) * 0.5f;
+            s->prop.w = 0.f;
+        } else if (st == MCX_SRC_HYPERBOLOID_GAUSSIAN) {
+            /* rays on the one-sheet hyperboloid of a Gaussian beam: waist param1.x,
+               param1.y/param1.z the skew; the launch voxel is left as set
+               (src/mcx_core.cl:1820-1852) */
+            float sphi, cphi;
+            float r = TWO_PI * rng_next(s->t, w);
+            pm_sincos(r, &sphi, &cphi);
+            r = pm_sqrt(0.5f * rng_scatlen(s->t, w)) * lp1.x;
+            float hx = -lp1.y / lp1.z;
+            float hy = pm_rsqrt(r * r + lp1.z * lp1.z);
+            F4 q = {r * (cphi - hx * sphi), r * (sphi + hx * cphi), 0.f, p->w};
+            F4 d = {-r * sphi * hy, r * cphi * hy, lp1.z * hy, 0.f};
+
+            if (v->z > -1.f + K_EPS && v->z < 1.f - K_EPS) {
+                float t = 1.f - v->z * v->z;
+                float stheta = pm_sqrt(t);
+                t = pm_rsqrt(t);
+                cphi = v->x * t;
+                sphi = v->y * t;
+                F4 q2 = {q.x * cphi * v->z - q.y * sphi, q.x * sphi * v->z + q.y * cphi, -q.x * stheta, q.w};
+                F4 v2 = {d.x * cphi * v->z - d.y * sphi + d.z * cphi * stheta,
+                         d.x * sphi * v->z + d.y * cphi + d.z * sphi * stheta, -d.x * stheta + d.z * v->z, v->w};
+                q = q2;
+                *v = v2;
+            } else {
+                F4 v2 = {d.x, d.y, (v->z > 0.f) ? d.z : -d.z, v->w};
+                *v = v2;
+            }
+
+            p->x = q.x + lsrcpos.x;
+            p->y = q.y + lsrcpos.y;
+            p->z = q.z + lsrcpos.z;
+            p->w = q.w;
+            F4 c = {lsrcpos.x, lsrcpos.y, lsrcpos.z, 0.f};
+            s->prop = c;
+            s->Lmove = 0.f;
         } else if (st == MCX_SRC_DISK || st == MCX_SRC_GAUSSIAN || st == MCX_SRC_RING) {
             float sphi, cphi, phi, r;
 
@@ -729,7 +834,35 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
             continue;   /* a do-while continue re-tests the loop condition directly */
         }
 
-        if (s->Lmove < 0.f) {
+        if (g->nangle) {
+            /* sampled launch polar angle (src/mcx_core.cl:2098-2125): the table holds
+               theta/pi at nangle equal steps of the CDF, taken as steps
+               (srcdir.w > 0) or linearly interpolated; the azimuth is uniform
+               about srcdir (used as is when 0 <= srcdir.w < 1.5) */
+            float ang, stheta, ctheta, sphi, cphi;
+
+            if (lsrcdir.w > 0.f) {
+                ang = fminf(rng_next(s->t, w) * (float)g->nangle, (float)g->nangle - K_EPS);
+                cphi = g->angleinvcdf[(int)ang];
+            } else {
+                ang = fminf(rng_next(s->t, w) * (float)(g->nangle - 1), (float)(g->nangle - 1) - K_EPS);
+                sphi = ang - (float)((int)ang);
+                uint32_t i0 = ((uint32_t)ang >= g->nangle - 1) ? g->nangle - 1 : (uint32_t)(int)ang;
+                uint32_t i1 = ((uint32_t)ang + 1 >= g->nangle - 1) ? g->nangle - 1 : (uint32_t)((int)ang + 1);
+                cphi = (1.f - sphi) * g->angleinvcdf[i0] + sphi * g->angleinvcdf[i1];
+            }
+
+            cphi *= ONE_PI;
+            pm_sincos(cphi, &stheta, &ctheta);
+            ang = TWO_PI * rng_next(s->t, w);
+            pm_sincos(ang, &sphi, &cphi);
+
+            if (lsrcdir.w < 1.5f && lsrcdir.w >= 0.f) {
+                *v = lsrcdir;
+            }
+
+            rotatevector(v, stheta, ctheta, sphi, cphi);
+        } else if (s->Lmove < 0.f) {
             if (lsrcdir.w != lsrcdir.w) {   /* NaN focal length: isotropic launch */
                 float ang, stheta, ctheta, sphi, cphi;
                 ang = TWO_PI * rng_next(s->t, w);
@@ -813,6 +946,16 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
     pp[2] = (g->srcnum > 1) ? pp[2] : p->w;
     v->w = K_EPS;
     s->Lmove = 0.f;
+
+    if ((g->outputtype == otRF || g->outputtype == otRFmus) && g->seed == SEED_FROM_FILE) {
+        /* RF replay: the phase omega * tof of this photon (src/mcx_core.cl:2257-2263,
+           which reads the next photon's time of flight; here the photon's own) */
+        float rs, rc;
+        pm_sincos(g->omega * g->replaytof[photon_index(g, idx, f->w) - 1], &rs, &rc);
+        pp[4 + g->srcnum] = rc;
+        pp[5 + g->srcnum] = rs;
+    }
+
     OCOUNT(w, nlaunch);
     return 0;
 }
@@ -923,10 +1066,25 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                 v->w += 1.f;
                 OCOUNT(w, nscat);
 
-                if (g->outputtype == otWP || g->outputtype == otDCS) {
+                if (g->outputtype == otWP || g->outputtype == otDCS || g->outputtype == otWPTOF ||
+                        (g->seed == SEED_FROM_FILE && g->outputtype == otRFmus)) {
+                    /* replay deposits at each scattering site (src/mcx_core.cl:2568-2623):
+                       w (WP), (1 - cos theta) w (DCS), tof w (WPTOF), w e^{i omega tof} (RFmus) */
                     uint32_t pid = photon_index(g, idx, f->w) - 1;
+                    float im = 0.f;
                     tmp0 = (g->outputtype == otDCS) ? (1.f - ctheta) : 1.f;
-                    tmp0 *= g->replayweight[pid];
+
+                    if (g->outputtype == otRFmus) {
+                        const float rw = g->replayweight[pid];
+                        tmp0 = rw * s->pp[g->w0offset + g->srcnum].f;
+                        im = rw * s->pp[g->w0offset + g->srcnum + 1].f;
+                    } else if (g->outputtype == otWPTOF) {
+                        tmp0 = g->replaytof[pid];
+                        tmp0 *= g->replayweight[pid];
+                    } else {
+                        tmp0 *= g->replayweight[pid];
+                    }
+
                     int tshift = gate_of(g, g->replaytof[pid]) +
                                  ((g->replaydet == -1) ? (((g->replaydetid[pid] & 0xFFFF) - 1) * (int)g->maxgate) : 0);
 
@@ -935,6 +1093,10 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                                   (int)g->maxgate;
                     }
                     deposit(g, w, s->idx1d + (uint32_t)tshift * g->nvox, tmp0);
+
+                    if (g->outputtype == otRFmus) {
+                        deposit(g, w, s->idx1d + (uint32_t)tshift * g->nvox + g->fieldlen, im);
+                    }
                 }
             }
 
@@ -990,7 +1152,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
 
         if (s->idx1d != idx1dold && idx1dold < g->nvox && mediaidold) {
             if (g->save2pt && f->y >= g->twin0 && f->y < g->twin1) {
-                float weight = 0.f;
+                float weight = 0.f, weight_im = 0.f;
                 int tshift = gate_of(g, f->y);
 
                 if (g->outputtype == otEnergy) {
@@ -998,11 +1160,20 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                 } else if (g->outputtype == otFluence || g->outputtype == otFlux || MCX_IS_ADJOINT_TYPE(g->outputtype)) {
                     weight = (s->prop.x < K_EPS) ? (s->w0 * pathlen) : ((s->w0 - p->w) / s->prop.x);
                 } else if (g->seed == SEED_FROM_FILE) {
-                    if (g->outputtype == otJacobian) {
+                    if (g->outputtype == otJacobian || g->outputtype == otRF || g->outputtype == otWLTOF) {
+                        /* replay path-length deposits (src/mcx_core.cl:2845-2858): w L (J),
+                           -w L e^{i omega tof} (RF), tof w L (WLTOF) */
                         uint32_t pid = photon_index(g, idx, f->w) - 1;
                         weight = g->replayweight[pid] * pathlen;
                         tshift = gate_of(g, g->replaytof[pid]) +
                                  ((g->replaydet == -1) ? (((g->replaydetid[pid] & 0xFFFF) - 1) * (int)g->maxgate) : 0);
+
+                        if (g->outputtype == otRF) {
+                            weight_im = -weight * s->pp[g->w0offset + g->srcnum + 1].f;
+                            weight = -weight * s->pp[g->w0offset + g->srcnum].f;
+                        } else if (g->outputtype == otWLTOF) {
+                            weight = weight * g->replaytof[pid];
+                        }
                     }
                 } else if (g->outputtype == otL) {
                     weight = s->w0 * pathlen;
@@ -1013,8 +1184,22 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                               (int)g->maxgate;
                 }
 
-                if (pm_fabs(weight) > 0.f) {
+                if (g->npattern > 1) {
+                    /* several patterns share the photon: one slot per pattern, interleaved
+                       after the voxel (src/mcx_core.cl:2902-2914) */
+                    for (uint32_t i = 0; i < g->npattern; i++) {
+                        const float pw = s->pp[g->w0offset + i].f;
+
+                        if (pm_fabs(pw) > 0.f && pm_fabs(weight) > 0.f) {
+                            deposit(g, w, (idx1dold + (uint32_t)tshift * g->nvox) * g->npattern + i, weight * pw);
+                        }
+                    }
+                } else if (pm_fabs(weight) > 0.f) {
                     deposit(g, w, idx1dold + (uint32_t)tshift * g->nvox, weight);
+                }
+
+                if (g->outputtype == otRF && g->omega > 0.f && pm_fabs(weight_im) > 0.f) {
+                    deposit(g, w, idx1dold + (uint32_t)tshift * g->nvox + g->fieldlen, weight_im);
                 }
             }
 
@@ -1204,10 +1389,17 @@ static void oracle_setup(Config* cfg, OParam* gp, F4** props_out, float** muamus
     unsigned int maxgate = (unsigned int)((cfg->tend - cfg->tstart) / cfg->tstep + 0.5);
     cfg->maxgate = maxgate;
     gp->maxgate = maxgate;
-    unsigned int nsrcslot = (cfg->srcid < 0) ? cfg->extrasrclen + 1 : 1;
+    gp->npattern = ((cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D) && cfg->srcnum > 1) ?
+                   cfg->srcnum : 1;
+    unsigned int nsrcslot = (cfg->srcid < 0) ? cfg->extrasrclen + 1 : gp->npattern;
     unsigned int ndetslot = (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) ? cfg->detnum : 1;
     size_t fieldlen = (size_t)gp->nvox * maxgate * nsrcslot * ndetslot;
     gp->fieldlen = (uint32_t)fieldlen;
+    gp->rfmul = (cfg->seed == SEED_FROM_FILE && (cfg->outputtype == otRF || cfg->outputtype == otRFmus)) ? 2 : 1;
+    gp->srcpattern = cfg->srcpattern;
+    gp->nangle = cfg->nangle;
+    gp->angleinvcdf = cfg->angleinvcdf;
+    gp->omega = cfg->omega;
     float minstep = fminf(fminf(cfg->steps.x, cfg->steps.y), cfg->steps.z);
     gp->twin0 = cfg->tstart;
     gp->twin1 = cfg->tend;
@@ -1309,7 +1501,7 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
     float* muamus;
     oracle_setup(cfg, &g, &props, &muamus);
     const unsigned int maxgate = g.maxgate;
-    const size_t fieldlen = g.fieldlen;
+    const size_t fieldlen = (size_t)g.fieldlen * g.rfmul;
 
     /* respin: |respin| repetitions of nphoton/|respin| photons, each on the next
        nthread*4 rand() words (src/mcx_host.cpp:1089-1104, :1319-1332) */
@@ -1473,8 +1665,14 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
         }
 
         if (cfg->isnormalized && !(cfg->debuglevel & MCX_DEBUG_RNG)) {
+            /* src/mcx_host.cpp:1382-1465 */
+            const size_t half = g.fieldlen;   /* the real part; RF outputs have an imaginary half after it */
             float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
             float scale = 1.f;
+            const int isreplayout = cfg->outputtype == otJacobian || cfg->outputtype == otWP ||
+                                    cfg->outputtype == otDCS || cfg->outputtype == otRF ||
+                                    cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
+                                    cfg->outputtype == otWPTOF;
 
             if (cfg->outputtype == otFlux || cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
                 scale = cfg->unitinmm / (float)(cfg->energytot * Vvox * cfg->tstep);
@@ -1484,11 +1682,12 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
                 }
             } else if (cfg->outputtype == otEnergy || cfg->outputtype == otL) {
                 scale = 1.f / (float)cfg->energytot;
-            } else if (cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS) {
+            } else if (isreplayout) {
                 if (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) {
-                    /* per detector (src/mcx_host.cpp:1398-1422); with srcid < 0 per
-                       (source, detector) slice, each by its own photons' weights (the
-                       reference scales only the first source's slices) */
+                    /* per detector (src/mcx_host.cpp:1398-1422), both halves of RF
+                       outputs; with srcid < 0 per (source, detector) slice, each by its
+                       own photons' weights (the reference scales only the first
+                       source's slices) */
                     const int nsrc = (cfg->extrasrclen && cfg->srcid < 0) ? (int)cfg->extrasrclen + 1 : 1;
 
                     for (int sr = 1; sr <= nsrc; sr++) {
@@ -1504,8 +1703,11 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
                                 sd = cfg->unitinmm / sd;
                             }
 
-                            mcx_normalize(cfg->exportfield + ((size_t)(sr - 1) * cfg->detnum + (d - 1)) * g.nvox * maxgate, sd,
-                                          (int)(g.nvox * maxgate), 1, 0, 1);
+                            for (uint32_t part = 0; part < g.rfmul; part++) {
+                                mcx_normalize(cfg->exportfield + part * half +
+                                              ((size_t)(sr - 1) * cfg->detnum + (d - 1)) * g.nvox * maxgate, sd,
+                                              (int)(g.nvox * maxgate), 1, 0, 1);
+                            }
                         }
                     }
 
@@ -1528,7 +1730,30 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
             if (scale != 0.f) {
                 cfg->normalizer = scale;
                 cfg->his.normalizer = scale;
-                mcx_normalize(cfg->exportfield, scale, (int)fieldlen, cfg->isnormalized, 0, 1);
+
+                if (g.npattern > 1) {
+                    /* several patterns: pattern i scaled by npixel / (its summed weights)
+                       (src/mcx_host.cpp:1351-1364, :1436-1447), Kahan sums in float */
+                    const int psize = (cfg->srctype == MCX_SRC_PATTERN3D) ?
+                                      (int)cfg->srcparam1.x * (int)cfg->srcparam1.y * (int)cfg->srcparam1.z :
+                                      (int)cfg->srcparam1.w * (int)cfg->srcparam2.w;
+
+                    for (uint32_t i = 0; i < g.npattern; i++) {
+                        float sum = 0.f, c = 0.f;
+
+                        for (int j = 0; j < psize; j++) {
+                            float y = cfg->srcpattern[(size_t)j * g.npattern + i] - c;
+                            float t = sum + y;
+                            c = (t - sum) - y;
+                            sum = t;
+                        }
+
+                        mcx_normalize(cfg->exportfield, (float)psize / sum * scale, (int)(fieldlen / g.npattern),
+                                      cfg->isnormalized, (int)i, (int)g.npattern);
+                    }
+                } else {
+                    mcx_normalize(cfg->exportfield, scale, (int)fieldlen, cfg->isnormalized, 0, 1);
+                }
             }
         }
     }

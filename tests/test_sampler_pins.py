@@ -325,3 +325,114 @@ def test_russian_roulette_column():
     for k, ns in ((30, n1), (53, n2)):
         np.testing.assert_allclose(ratio[k], ns * (1 - 10 * math.exp(-0.1)) / (1 - math.exp(-0.1)) *
                                    (1 if k == 30 else 10), rtol=1e-4)
+
+
+# ---------------- pattern, fourierx, hyperboloid and angle-CDF launches ----------------
+
+JBO = np.float32(0.9998)   # JUST_BELOW_ONE of src/mcx_core.cl
+
+
+def _pixel(u, n):
+    """pixel index of a uniform coordinate u in [0, 1) over n pixels, as the
+    reference computes it: (int)(u * JUST_BELOW_ONE * n)"""
+    return np.floor(np.float32(u) * JBO * np.float32(n)).astype(int)
+
+
+def test_pattern_weights_follow_the_pixel():
+    """pattern (param1.w x param2.w pixels over the parallelogram param1, param2):
+    the launch weight is the pattern value of the pixel under the launch point
+    (src/mcx_core.cl:1678-1693); pixels are hit uniformly"""
+    pat = np.arange(1, 13, dtype=np.float32).reshape(4, 3, order="F")
+    s = _samples(srctype="pattern", srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 4], srcparam2=[0, 40, 0, 3],
+                 srcpattern=pat)
+    u = (s["p"][:, 0].astype(np.float64) - 10) / 40
+    v = (s["p"][:, 1].astype(np.float64) - 10) / 40
+    ix, iy = _pixel(u, 4), _pixel(v, 3)
+    ok = (np.abs(u * 4 - np.rint(u * 4)) > 1e-4) & (np.abs(v * 3 - np.rint(v * 3)) > 1e-4)   # off pixel edges
+    assert ok.mean() > 0.99
+    np.testing.assert_array_equal(s["w"][ok], pat[ix[ok], iy[ok]])
+    counts = np.bincount(ix * 3 + iy, minlength=12)
+    assert stats.chisquare(counts).pvalue > PMIN
+
+
+def test_pattern3d_box_and_weights():
+    """pattern3d (a param1.x x param1.y x param1.z box of voxels from srcpos): the
+    launch point is uniform in the box and carries the pattern value of its
+    voxel (src/mcx_core.cl:1661-1667, :1708-1727)"""
+    pat = np.random.default_rng(5).uniform(0.5, 2.0, (4, 3, 2)).astype(np.float32)
+    s = _samples(srctype="pattern3d", srcpos=[10, 12, 14], srcparam1=[4, 3, 2, 0], srcpattern=pat)
+    d = s["p"].astype(np.float64) - [10, 12, 14]
+    for k, n in enumerate((4, 3, 2)):
+        _ks(d[:, k] / n, _uniform(0, 1))
+    ix, iy, iz = _pixel(d[:, 0] / 4, 4), _pixel(d[:, 1] / 3, 3), _pixel(d[:, 2] / 2, 2)
+    ok = np.all(np.abs(d - np.rint(d)) > 1e-4, axis=1)
+    np.testing.assert_array_equal(s["w"][ok], pat[ix[ok], iy[ok], iz[ok]])
+
+
+@pytest.mark.parametrize("kind", ["fourierx", "fourierx2d"])
+def test_fourierx_patterns(kind):
+    """fourierx / fourierx2d: the parallelogram is param1 and param1.w dir x
+    param1/|param1| (length param1.w; src/mcx_core.cl:1776-1787); the weight is
+    (cos(2 pi (kx u + ky v + phase)) (1 - m) + 1) / 2 (fourierx, param2 = {kx, ky,
+    phase, m}) or (sin(2 pi (kx u + px)) sin(2 pi (ky v + py)) + 1) / 2
+    (fourierx2d, param2 = {kx, ky, px, py}) at the launch point (u, v)"""
+    p2 = [2.0, 1.0, 0.25, 0.5] if kind == "fourierx" else [2.0, 3.0, 0.1, 0.3]
+    s = _samples(srctype=kind, srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 20], srcparam2=p2)
+    u = (s["p"][:, 0].astype(np.float64) - 10) / 40
+    v = (s["p"][:, 1].astype(np.float64) - 10) / 20     # second edge: 20 along dir x param1 = +y
+    assert np.all(s["p"][:, 2] == 0)
+    _ks(u, _uniform(0, 1))
+    _ks(v, _uniform(0, 1))
+    if kind == "fourierx":
+        expect = (np.cos(2 * math.pi * (p2[0] * u + p2[1] * v + p2[2])) * (1 - p2[3]) + 1) / 2
+    else:
+        expect = (np.sin(2 * math.pi * (p2[0] * u + p2[2])) * np.sin(2 * math.pi * (p2[1] * v + p2[3])) + 1) / 2
+    np.testing.assert_allclose(s["w"], expect, atol=5e-5)
+    assert abs(s["w"].mean() - 0.5) < 4 * s["w"].std() / math.sqrt(N)
+
+
+@pytest.mark.parametrize("axis", [(0, 0, 1), (0, 0.6, 0.8)])
+def test_hyperboloid_gaussian_rays(axis):
+    """hyperboloid (param1 = {w, f, z0}): every ray passes the beam axis at its
+    closest distance r at f along the axis from srcpos, with r the waist sample
+    r = w sqrt(-log(U)/2): P(r <= x) = 1 - exp(-2 x^2 / w^2)
+    (src/mcx_core.cl:1820-1852)"""
+    w, f, z0 = 3.0, 10.0, 20.0
+    c = np.array([30.0, 30.0, 20.0])
+    s = _samples(srctype="hyperboloid", srcpos=list(c), srcdir=list(axis), srcparam1=[w, f, z0, 0])
+    a = np.array(axis, np.float64)
+    p, v = s["p"].astype(np.float64), s["v"].astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(v, axis=1), 1.0, atol=2e-6)
+    # closest approach of each ray to the axis line c + t a
+    w0 = p - c
+    b = v @ a
+    d_ = (w0 * v).sum(1)
+    e = w0 @ a
+    t_ray = (b * e - d_) / (1 - b * b)
+    t_axis = (e - b * d_) / (1 - b * b)
+    gap = np.linalg.norm(w0 + t_ray[:, None] * v - t_axis[:, None] * a, axis=1)
+    # (the closest point is ill-conditioned for rays almost parallel to the axis)
+    wide = gap > 0.5
+    assert wide.mean() > 0.9
+    np.testing.assert_allclose(t_axis[wide], f, atol=1e-2)
+    _ks(gap, lambda x: 1.0 - np.exp(-2.0 * x * x / (w * w)))
+
+
+@pytest.mark.parametrize("step", [False, True])
+def test_angle_cdf_launch(step):
+    """angleinvcdf (theta / pi at nangle equal CDF steps, src/mcx_core.cl:
+    2098-2125): interpolated linearly when srcdir.w <= 0, so theta/pi has the
+    piecewise-linear CDF through (table_k, k / (nangle - 1)); taken as steps when
+    srcdir.w > 0, so each entry is equally likely; the azimuth is uniform"""
+    table = np.array([0.0, 0.05, 0.08, 0.12, 0.2, 0.3, 0.45, 0.6], np.float32)
+    s = _samples(srctype="pencil", srcpos=[30, 30, 30], srcdir=[0, 0, 1, 1.0 if step else 0.0], angleinvcdf=table)
+    v = s["v"].astype(np.float64)
+    th = np.arccos(np.clip(v[:, 2], -1, 1)) / math.pi
+    if step:
+        k = np.argmin(np.abs(th[:, None] - table[None, :].astype(np.float64)), axis=1)
+        np.testing.assert_allclose(th, table[k], atol=2e-4)
+        assert stats.chisquare(np.bincount(k, minlength=table.size)).pvalue > PMIN
+    else:
+        _ks(th, lambda x: np.interp(x, table.astype(np.float64), np.linspace(0, 1, table.size)))
+    sel = th > 0.02
+    _ks(np.arctan2(v[sel, 1], v[sel, 0]), _uniform(-math.pi, math.pi))
