@@ -1442,7 +1442,9 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
 
                     s.idx1d = idx1dold;
                     keepold = true;
-                    s.mediaid = (g.media[s.idx1d] & K_MED_MASK);
+                    /* back in the voxel it came from, whose label mediaidold holds
+                       (the reference reloads it from the volume) */
+                    s.mediaid = mediaidold & K_MED_MASK;
                     s.prop = getprop(g, sh, s.mediaid, s.idx1d);
                     n1 = s.prop.w;
                     /* drop the boundary code so a later same-voxel step restores
