@@ -68,6 +68,7 @@ typedef MCXUInt3 uint3;
 #define VERY_BIG           1e10f
 #define SEED_FROM_FILE     -999
 #define FILL_MAXDETPHOTON  3
+#define SIGN_BIT           0x80000000U
 #define OUTSIDE_VOLUME_MIN 0xFFFFFFFFu
 #define OUTSIDE_VOLUME_MAX 0x7FFFFFFFu
 #define DET_MASK           0x80000000u
@@ -348,6 +349,9 @@ void mcx_maskdet(Config* cfg);                                              /* s
 void mcx_normalize(float field[], float scale, int fieldlen, int option, int pidx, int srcnum); /* src/mcx_utils.h:300 */
 void mcx_kahanSum(float* sum, float* kahanc, float input);                  /* src/mcx_utils.h:326 */
 float mcx_updatemua(unsigned int mediaid, Config* cfg);                     /* src/mcx_utils.h:327 */
+int  mcx_pack_media(unsigned int format, const float* in, size_t nvox, float unitinmm, unsigned int* out);
+                                                                            /* the packing steps of src/mcx_utils.c:3597-3681
+                                                                               and src/pmcxcl.cpp:243-389 (inline there) */
 void mcx_createfluence(float** fluence, Config* cfg);                       /* src/mcx_utils.h:306 */
 void mcx_clearfluence(float** fluence);                                     /* src/mcx_utils.h:307 */
 void mcx_convertrow2col(unsigned int** vol, MCXUInt4* dim);                 /* src/mcx_utils.h:309 */

@@ -189,6 +189,8 @@ lib.mcx_try_savedata.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _P(Config)]
 lib.mcx_try_savedata.restype = ctypes.c_int
 lib.mcx_try_savedetphoton.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _P(Config)]
 lib.mcx_try_savedetphoton.restype = ctypes.c_int
+lib.mcx_pack_media.argtypes = [ctypes.c_uint, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_void_p]
+lib.mcx_pack_media.restype = ctypes.c_int
 lib.mcx_try_run_simulation.argtypes = [_P(Config)]
 lib.mcx_try_run_simulation.restype = ctypes.c_int
 lib.mcx_last_error.restype = ctypes.c_char_p

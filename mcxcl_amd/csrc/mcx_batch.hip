@@ -376,8 +376,8 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
         }
 
         if (cfg->issavedet || cfg->seed == SEED_FROM_FILE || (cfg->debuglevel & MCX_DEBUG_RNG) || cfg->muamus ||
-                MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
-            MCX_ERROR(-4, "the batched launch takes no detected photons, replay, RNG dump, per-voxel media or adjoint "
+                cfg->mediabyte > 4 || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
+            MCX_ERROR(-4, "the batched launch takes no detected photons, replay, RNG dump, per-voxel or packed media or adjoint "
                       "outputs");
         }
 

@@ -386,7 +386,68 @@ __device__ __forceinline__ void deposit_voxel(const MCXKParam& g, const Shared& 
     }
 }
 
+/* binary16 bit pattern (sign bit clear) to float, exactly, subnormals included */
+__device__ __forceinline__ float half_bits_to_float(uint32_t h) {
+    const uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+
+    if (e == 0) {
+        return (float)m * 5.9604644775390625e-8f;   /* m x 2^-24 */
+    }
+
+    return __uint_as_float(e == 31 ? (0x7F800000u | (m << 13)) : (((e + 112u) << 23) | (m << 13)));
+}
+
+/* Properties of a voxel whose volume word packs them instead of naming a label
+   (src/mcx_core.cl:1079-1193).  t is the label table: t[0] the background,
+   t[1] and t[2] the two end members of the mixing formats.  The components a
+   format leaves unset keep t[1]'s, the value the reference's launch writes
+   (src/mcx_core.cl:2213) and its updates never overwrite; n is t[1]'s in a
+   non-void voxel.  Bit 31 (the detector mask) is not part of the word. */
+enum {kMediaLabelHalf = 99, kMediaAsF2H = 100, kMediaMuaFloat = 101, kMediaAsHalf = 102, kMediaAsgnByte = 103,
+      kMediaAsShort = 104};
+
+__device__ __forceinline__ float4 media_word_prop(uint32_t format, const float4* t, uint32_t mediaid) {
+    const uint32_t w = mediaid & K_MED_MASK;
+    float4 p = t[1];
+    const float nw = t[w != 0].w;
+
+    if (format == kMediaMuaFloat) {
+        p.x = __uint_as_float(w);                                  /* |mua| */
+        p.w = nw;
+    } else if (format == kMediaAsF2H || format == kMediaAsHalf) {
+        p.x = half_bits_to_float(w & 0x7FFFu);                     /* |mua|, |mus| as two halves */
+        p.y = half_bits_to_float(w >> 16);
+        p.w = nw;
+    } else if (format == kMediaLabelHalf) {
+        p = t[w & 0x3FFFu];                                        /* label, then one component replaced */
+        const float r = half_bits_to_float(w >> 16);
+        const uint32_t k = (w >> 14) & 3u;
+        p.x = (k == 0) ? r : p.x;
+        p.y = (k == 1) ? r : p.y;
+        p.z = (k == 2) ? r : p.z;
+        p.w = (k == 3) ? r : p.w;
+    } else if (format == kMediaAsgnByte) {
+        p.x = (float)(w & 0xFFu) * (1.f / 255.f) * (t[2].x - t[1].x) + t[1].x;
+        p.y = (float)((w >> 8) & 0xFFu) * (1.f / 255.f) * (t[2].y - t[1].y) + t[1].y;
+        p.z = (float)((w >> 16) & 0xFFu) * (1.f / 255.f) * (t[2].z - t[1].z) + t[1].z;
+        p.w = (float)(w >> 24) * (1.f / 127.f) * (t[2].w - t[1].w) + t[1].w;
+    } else if (format == kMediaAsShort) {
+        p.x = (float)(w & 0xFFFFu) * (1.f / 65535.f) * (t[2].x - t[1].x) + t[1].x;
+        p.y = (float)(w >> 16) * (1.f / 65535.f) * (t[2].y - t[1].y) + t[1].y;
+        p.w = nw;
+    }
+
+    return p;
+}
+
+/* the properties of the photon's voxel: label table, per-voxel mua/mus
+   replacing the label's, or (full general path) a packed media word */
+template <bool FULL = false>
 __device__ __forceinline__ float4 getprop(const MCXKParam& g, const Shared& sh, uint32_t mediaid, uint32_t idx1d) {
+    if (FULL && g.mediaformat > 4) {
+        return media_word_prop(g.mediaformat, sh.prop, mediaid);
+    }
+
     float4 p = sh.prop[mediaid & K_MED_MASK];
 
     /* per-voxel absorption and scattering: g and n stay those of the label */
@@ -400,6 +461,7 @@ __device__ __forceinline__ float4 getprop(const MCXKParam& g, const Shared& sh, 
 }
 
 /* src/mcx_core.cl:1350-1455 */
+template <bool FULL>
 static __device__ int skipvoid(const MCXKParam& g, Photon& s) {
     int count = 1, idx1d;
     float4& p = s.p;
@@ -458,7 +520,10 @@ static __device__ int skipvoid(const MCXKParam& g, Photon& s) {
                     return -1;   /* entry search failed: the reference would read outside the volume */
                 }
 
-                float nin = g.prop[g.media[idx1d] & K_MED_MASK].w;
+                /* n of the entry voxel (a packed media word: its decoded n; the reference
+                   indexes the label table with the word, src/mcx_core.cl:1424) */
+                float nin = (FULL && g.mediaformat > 4) ? media_word_prop(g.mediaformat, g.prop, g.media[idx1d]).w :
+                            g.prop[g.media[idx1d] & K_MED_MASK].w;
 
                 if (g.isspecular && nin != g.prop[0].w) {
                     p.w *= 1.f - reflectcoeff(v, g.prop[0].w, nin, s.fw);
@@ -662,7 +727,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
         s.fy = to_short((int)floorf(p.y));
         s.fz = to_short((int)floorf(p.z));
         f.w += 1.f;
-        s.prop = getprop(g, sh, s.mediaid, s.idx1d);
+        s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
         pp[pd + 1] += p.w;
         s.w0 = p.w;
         pp[pd + 2] = (g.srcnum > 1) ? pp[pd + 2] : p.w;
@@ -1049,7 +1114,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
         }
 
         if ((s.mediaid & K_MED_MASK) == 0) {
-            int id = skipvoid(g, s);
+            int id = skipvoid<FULL>(g, s);
 
             if (id >= 0) {
                 s.idx1d = (uint32_t)id;
@@ -1068,7 +1133,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
     } while ((s.mediaid & K_MED_MASK) == 0 || fabsf(p.w) <= g.minenergy);
 
     f.w += 1.f;
-    s.prop = getprop(g, sh, s.mediaid, s.idx1d);
+    s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
     ppl[1] += p.w;
     s.w0 = p.w;
     ppl[2] = (g.srcnum > 1) ? ppl[2] : p.w;
@@ -1234,7 +1299,7 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
         }
 
         n1 = s.prop.w;
-        s.prop = getprop(g, sh, s.mediaid, s.idx1d);
+        s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
         int face;
         f.z = hitgrid(p, v, s.fx, s.fy, s.fz, &face);
         s.fw = face;
@@ -1394,7 +1459,7 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
         }
 
         if (REFLECT) {
-            s.prop = getprop(g, sh, s.mediaid, s.idx1d);
+            s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
 
             if (((s.mediaid && g.doreflect) ||
                     (s.mediaid == 0 && (((isdet & 0xF) == 0 && g.doreflect) || ((isdet & 0xF) == kReflect ||
@@ -1445,7 +1510,7 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
                     /* back in the voxel it came from, whose label mediaidold holds
                        (the reference reloads it from the volume) */
                     s.mediaid = mediaidold & K_MED_MASK;
-                    s.prop = getprop(g, sh, s.mediaid, s.idx1d);
+                    s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
                     n1 = s.prop.w;
                     /* drop the boundary code so a later same-voxel step restores
                        the voxel's own label (the reference ORs the code in) */

@@ -237,8 +237,20 @@ static hipError_t dalloc(DevBuf& bufs, T** p, size_t n) {
 /* features the transport kernels do not implement stop here, before any
    device work, instead of running with different semantics */
 void mcx_check_supported(const Config* cfg) {
-    if (cfg->mediabyte > 4) {
-        MCX_ERROR(-4, "unsupported media format for the GPU kernel");
+    /* packed media words: the property formats of src/mcx_core.cl:1079-1193;
+       asgn_float, svmc and mixlabel are not built */
+    if (cfg->mediabyte > 4 && (cfg->mediabyte < MEDIA_LABEL_HALF || cfg->mediabyte > MEDIA_AS_SHORT)) {
+        MCX_ERROR(-4, "unsupported media format for the GPU kernel (label byte/short/integer, labelplus, "
+                  "muamus_float, mua_float, muamus_half, asgn_byte and muamus_short are supported)");
+    }
+
+    if (cfg->mediabyte > 4 && (cfg->seed == SEED_FROM_FILE || cfg->muamus || MCX_IS_ADJOINT_TYPE(cfg->outputtype))) {
+        MCX_ERROR(-4, "replay, per-voxel mua/mus arrays and adjoint outputs need a label volume");
+    }
+
+    if (cfg->mediabyte == MEDIA_LABEL_HALF && (cfg->savedetflag & 0xEu)) {
+        MCX_ERROR(-4, "scattering counts, partial path lengths and momentum transfer of detected photons need a "
+                  "label volume (labelplus words carry a replaced property)");
     }
 
     if ((cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D) && !cfg->srcpattern) {
@@ -378,6 +390,7 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
         } lab;
         lab.f = cfg->srcparam2.w;
         g.simple = cfg->srctype == MCX_SRC_PENCIL && cfg->srcdir.w == 0.f && (lab.u & MED_MASK) != 0 &&
+                   cfg->mediabyte <= 4 &&
                    cfg->extrasrclen == 0 && cfg->seed != SEED_FROM_FILE && cfg->issaveref == 0 && g.is2d == 0 &&
                    cfg->nphase <= 2 && cfg->nangle == 0 && cfg->srcpos.w > 0.f && cfg->minenergy < cfg->srcpos.w &&
                    (cfg->outputtype == otFlux || cfg->outputtype == otFluence || cfg->outputtype == otEnergy ||
@@ -389,7 +402,7 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
              (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->srctype == MCX_SRC_FOURIERX ||
               cfg->srctype == MCX_SRC_FOURIERX2D || cfg->srctype == MCX_SRC_HYPERBOLOID_GAUSSIAN || cfg->nangle ||
               cfg->outputtype == otRF || cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
-              cfg->outputtype == otWPTOF);
+              cfg->outputtype == otWPTOF || cfg->mediabyte > 4);
 
     /* reflection block rule (src/mcx_host.cpp:945-956) */
     {
@@ -1484,7 +1497,7 @@ static void noalloc_run(Config* cfg, const uint32_t* media, size_t nmedia, doubl
     mcx_check_supported(cfg);
 
     if (cfg->issavedet || cfg->seed == SEED_FROM_FILE || MCX_IS_ADJOINT_TYPE(cfg->outputtype) ||
-            (cfg->debuglevel & MCX_DEBUG_RNG) || cfg->muamus || !cfg->issave2pt) {
+            (cfg->debuglevel & MCX_DEBUG_RNG) || cfg->muamus || cfg->mediabyte > 4 || !cfg->issave2pt) {
         MCX_ERROR(-4, "mcx_run_simulation_noalloc runs forward fluence simulations only: no detected photons, "
                   "replay, adjoint outputs, RNG dump or per-voxel media, and the field must be saved");
     }

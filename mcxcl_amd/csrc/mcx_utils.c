@@ -287,6 +287,102 @@ void mcx_convertrow2col(unsigned int** vol, MCXUInt4* dim) {
     *vol = out;
 }
 
+/* ---------------- packed media words ---------------- */
+
+/* float32 bits to the binary16 pattern the reference's packers write: the
+   exponent rebiased and clamped at zero (wrapping above the half range), the
+   mantissa truncated.  With withsub, a value whose rebiased exponent is zero
+   and whose top ten mantissa bits are below 0x10 is rounded to a subnormal
+   half (src/mcx_utils.c:3625-3646); the labelplus packer has no such path
+   (src/pmcxcl.cpp:283-288).  A right shift of 32 or more counts as zero: such
+   values are below 2^-45 and the reference's shift there is undefined. */
+static unsigned short mcx_half_bits(unsigned int i, int withsub) {
+    const unsigned int m = (i >> 13) & 0x3FFu, e = (i >> 23) & 0xFFu;
+    const unsigned int t = (e > 0x70u) ? ((e - 0x70u) & 0x1Fu) : 0u;
+
+    if (withsub && m < 0x10u && t == 0) {
+        const unsigned int mm = ((i >> 12) & 0x7FFu) | 0x800u;
+        const unsigned int s1 = 114u - e, s0 = 113u - e;   /* e <= 0x70 here: both positive */
+        const unsigned int a = (s1 < 32u) ? (mm >> s1) : 0u;
+        const unsigned int b = (s0 < 32u) ? ((mm >> s0) & 1u) : 0u;
+        return (unsigned short)(((i >> 16) & 0x8000u) | ((a + b) & 0xFFFFu));
+    }
+
+    return (unsigned short)(((i >> 31) << 15) | (t << 10) | m);
+}
+
+/* Pack per-voxel properties into the words of a packed media format, as the
+   reference's volume reader (src/mcx_utils.c:3597-3681) and its Python module
+   (src/pmcxcl.cpp:243-389) do.  in holds ncomp floats per voxel, voxel-major:
+     MEDIA_MUA_FLOAT  (1): mua (1/mm) x unitinmm as float32 bits; a zero word
+                           becomes EPS, NaN and -0 become 0 (void)
+     MEDIA_AS_F2H     (2): {mua, mus} (1/mm) x unitinmm as two halves, mua in the
+                           low half; NaN in either gives 0, an all-zero word
+                           0x00010000 (mus = the smallest half), -0 gives 0
+     MEDIA_LABEL_HALF (3): {value, component 0..3, label}: label | component << 14
+                           in the low half, the value's half in the high half;
+                           the value is scaled by unitinmm for components 1 and 2,
+                           as src/pmcxcl.cpp:273-275 does
+   Returns 0; -1 for another format; -2 for a labelplus component outside 0..3
+   or a label outside 0..16383 (the reference truncates such labels into the
+   component bits). */
+int mcx_pack_media(unsigned int format, const float* in, size_t nvox, float unitinmm, unsigned int* out) {
+    union {
+        float f;
+        unsigned int u;
+    } a, b;
+
+    if (format == MEDIA_MUA_FLOAT) {
+        for (size_t i = 0; i < nvox; i++) {
+            a.f = in[i] * unitinmm;
+
+            if (a.u == 0) {
+                a.f = EPS;
+            }
+
+            if (in[i] != in[i] || a.u == SIGN_BIT) {
+                a.u = 0;
+            }
+
+            out[i] = a.u;
+        }
+    } else if (format == MEDIA_AS_F2H) {
+        for (size_t i = 0; i < nvox; i++) {
+            a.f = in[2 * i] * unitinmm;
+            b.f = in[2 * i + 1] * unitinmm;
+
+            if (a.f != a.f || b.f != b.f) {
+                out[i] = 0;
+                continue;
+            }
+
+            unsigned int w = (unsigned int)mcx_half_bits(a.u, 1) | ((unsigned int)mcx_half_bits(b.u, 1) << 16);
+            w = (w == 0) ? 0x00010000u : w;
+            out[i] = (w == SIGN_BIT) ? 0u : w;
+        }
+    } else if (format == MEDIA_LABEL_HALF) {
+        for (size_t i = 0; i < nvox; i++) {
+            a.f = in[3 * i];
+            const float comp = in[3 * i + 1], label = in[3 * i + 2];
+
+            if (!(comp >= 0.f && comp < 4.f) || !(label >= 0.f && label < 16384.f)) {
+                return -2;
+            }
+
+            if (comp >= 1.f && comp <= 2.f) {
+                a.f *= unitinmm;
+            }
+
+            out[i] = ((((unsigned int)comp & 3u) << 14) | (unsigned int)label) |
+                     ((unsigned int)mcx_half_bits(a.u, 0) << 16);
+        }
+    } else {
+        return -1;
+    }
+
+    return 0;
+}
+
 /* ---------------- volume file (reference src/mcx_utils.c:3524-3698) ---------------- */
 static void mcx_loadvolume(const char* filename, Config* cfg) {
     size_t n = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
@@ -297,10 +393,10 @@ static void mcx_loadvolume(const char* filename, Config* cfg) {
     }
 
     unsigned int bytes = (cfg->mediabyte <= 4) ? cfg->mediabyte : 4;
-    int pairs = (cfg->mediabyte == MEDIA_AS_F2H || cfg->mediabyte == MEDIA_2LABEL_SPLIT ||
-                 cfg->mediabyte == MEDIA_MUA_MUS_FLOAT);
+    /* two words per voxel kept as read */
+    int pairs = (cfg->mediabyte == MEDIA_2LABEL_SPLIT || cfg->mediabyte == MEDIA_MUA_MUS_FLOAT);
 
-    if (pairs) {
+    if (pairs || cfg->mediabyte == MEDIA_AS_F2H) {
         bytes = 8;
     }
 
@@ -324,6 +420,9 @@ static void mcx_loadvolume(const char* filename, Config* cfg) {
         for (size_t i = 0; i < n; i++) {
             cfg->vol[i] = ((unsigned short*)raw)[i];
         }
+    } else if (cfg->mediabyte == MEDIA_MUA_FLOAT || cfg->mediabyte == MEDIA_AS_F2H) {
+        /* float mua, or {mua, mus} pairs, in 1/mm: packed into one word per voxel */
+        mcx_pack_media(cfg->mediabyte, (const float*)raw, n, cfg->unitinmm, cfg->vol);
     } else {
         memcpy(cfg->vol, raw, n * bytes);
     }
@@ -1490,9 +1589,30 @@ void mcx_preprocess(Config* cfg) {
             if (cfg->medianum <= maxlabel) {
                 MCX_ERROR(-4, "input media optical properties are less than the labels in the volume");
             }
-        } else if (cfg->mediabyte != MEDIA_MUA_MUS_FLOAT && cfg->mediabyte != MEDIA_MUA_FLOAT) {
-            MCX_ERROR(-4, "this build supports label (byte/short/integer), mua_float and muamus_f32 media formats");
+        } else if (cfg->mediabyte == MEDIA_LABEL_HALF) {
+            /* the label in the low 14 bits indexes the media table (the reference
+               reads past the table for a larger one, src/mcx_core.cl:1145) */
+            for (size_t i = 0; i < n; i++) {
+                if ((cfg->vol[i] & MED_MASK) && (cfg->vol[i] & 0x3FFFu) >= cfg->medianum) {
+                    MCX_ERROR(-4, "input media optical properties are less than the labels in the volume");
+                }
+            }
+        } else if (cfg->mediabyte != MEDIA_MUA_MUS_FLOAT && cfg->mediabyte != MEDIA_MUA_FLOAT &&
+                   cfg->mediabyte != MEDIA_AS_F2H && cfg->mediabyte != MEDIA_AS_HALF &&
+                   cfg->mediabyte != MEDIA_ASGN_BYTE && cfg->mediabyte != MEDIA_AS_SHORT) {
+            MCX_ERROR(-4, "this build supports the label (byte/short/integer), labelplus, muamus_float, mua_float, "
+                      "muamus_half, asgn_byte, muamus_short and muamus_f32 media formats");
         }
+    }
+
+    /* src/mcx_utils.c:1760-1766 */
+    if ((cfg->mediabyte == MEDIA_AS_F2H || cfg->mediabyte == MEDIA_MUA_FLOAT || cfg->mediabyte == MEDIA_AS_HALF) &&
+            cfg->medianum < 2) {
+        MCX_ERROR(-4, "the 'prop' field must contain at least 2 rows for the requested media format");
+    }
+
+    if ((cfg->mediabyte == MEDIA_ASGN_BYTE || cfg->mediabyte == MEDIA_AS_SHORT) && cfg->medianum < 3) {
+        MCX_ERROR(-4, "the 'prop' field must contain at least 3 rows for the requested media format");
     }
 
     /* point sources: pre-compute the launch voxel and its label, stored as raw
@@ -1545,7 +1665,9 @@ void mcx_preprocess(Config* cfg) {
         cfg->srcparam2.w = 0.f;
     }
 
-    if (cfg->issavedet && cfg->mediabyte <= 4) {
+    /* one word per voxel: the detector mask marks non-zero words (src/mcx_utils.c:1751-1753) */
+    if (cfg->issavedet && (cfg->mediabyte <= 4 || (cfg->mediabyte >= MEDIA_LABEL_HALF &&
+                           cfg->mediabyte <= MEDIA_AS_SHORT))) {
         mcx_maskdet(cfg);
     }
 

@@ -54,18 +54,51 @@ def _carray(arr, dtype):
     return ptr
 
 
+# packed media formats (include/mcx_utils.h)
+MEDIA_LABEL_HALF, MEDIA_AS_F2H, MEDIA_MUA_FLOAT, MEDIA_ASGN_BYTE, MEDIA_AS_SHORT = 99, 100, 101, 103, 104
+
+
+def _pack_float_media(vol, fmt, unitinmm):
+    """float32 [k, Nx, Ny, Nz] per-voxel values to one packed word per voxel (the
+    library's mcx_pack_media, the packing of src/pmcxcl.cpp:243-389)"""
+    src = np.ascontiguousarray(np.asfortranarray(vol).ravel(order="F"), dtype=np.float32)
+    out = np.empty(int(np.prod(vol.shape[1:])), dtype=np.uint32)     # voxels, x fastest
+    rc = _native.lib.mcx_pack_media(fmt, src.ctypes.data, out.size, unitinmm, out.ctypes.data)
+    if rc:
+        raise ValueError("the 2nd volume must have an integer value between 0 and 3, and labels below 16384"
+                         if rc == -2 else "unsupported packed media format %d" % fmt)
+    return out
+
+
 def _parse_volume(user, c):
-    """reference parseVolume (src/pmcxcl.cpp:93-427): dtype selects the media format"""
+    """reference parseVolume (src/pmcxcl.cpp:93-427): dtype and leading dimension
+    select the media format.  3-D integer arrays are label volumes; 4-D arrays are
+    per-voxel properties packed into one word per voxel: float32 [1,...] mua
+    (mua_float), float32 [2,...] {mua, mus} as two halves (muamus_float),
+    float32 [3,...] {value, component, label} (labelplus), int8 [4,...] four
+    mixing bytes (asgn_byte), int16 [2,...] two mixing shorts (muamus_short).
+    Per-voxel float32 {mua, mus} without half rounding go in 'muamus' beside a
+    label volume."""
     if "vol" not in user:
         raise ValueError("Configuration must specify a 2/3/4D volume.")
     vol = np.asarray(user["vol"])
-    if vol.ndim == 4 and vol.shape[0] == 2 and vol.dtype == np.float32:
-        # per-voxel {mua, mus} in 1/mm (the reference stores these as two halves,
-        # src/pmcxcl.cpp parseVolume, MEDIA_AS_F2H); here they stay float32, and a
-        # voxel with both zero is background (label 0), any other is label 1
-        vol = (np.any(vol != 0, axis=0)).astype(np.uint8)
-    elif vol.ndim == 4:
-        raise ValueError("4-D volumes must be float32 [2, Nx, Ny, Nz] per-voxel {mua, mus}")
+    if vol.ndim == 4:
+        k = vol.shape[0]
+        if vol.dtype == np.float32 and k in (1, 2, 3):
+            c.mediabyte = {1: MEDIA_MUA_FLOAT, 2: MEDIA_AS_F2H, 3: MEDIA_LABEL_HALF}[k]
+            words = _pack_float_media(vol, c.mediabyte, float(c.unitinmm))
+        elif vol.dtype == np.int8 and k == 4:
+            c.mediabyte = MEDIA_ASGN_BYTE
+            words = np.ascontiguousarray(np.asfortranarray(vol).ravel(order="F")).view(np.uint32)
+        elif vol.dtype == np.int16 and k == 2:
+            c.mediabyte = MEDIA_AS_SHORT
+            words = np.ascontiguousarray(np.asfortranarray(vol).ravel(order="F")).view(np.uint32)
+        else:
+            raise ValueError("4-D volumes must be float32 [1|2|3, Nx, Ny, Nz], int8 [4, Nx, Ny, Nz] or int16 "
+                             "[2, Nx, Ny, Nz] (the svmc and mixlabel formats are not supported)")
+        c.dim.x, c.dim.y, c.dim.z = (int(d) for d in vol.shape[1:])
+        c.vol = _carray(words.astype(np.uint32), np.uint32)
+        return
     if vol.ndim != 3:
         raise ValueError("the 'vol' field must be a 3-D array")
     if vol.dtype in (np.uint8, np.int8):
@@ -292,13 +325,9 @@ class PreparedConfig:
         self._set_detweight(user)
 
     def _set_pervoxel(self, user):
-        """per-voxel media ('vol' as float32 [2,Nx,Ny,Nz], or 'muamus' beside a label
-        volume), replay weights ('replayweight') and exit-weight output ('savedetweight')"""
+        """per-voxel float32 media ('muamus' beside a label volume), replay weights ('replayweight') and exit-weight output ('savedetweight')"""
         c = self.c
         mm = None
-        v = np.asarray(user["vol"]) if "vol" in user else None
-        if v is not None and v.ndim == 4 and v.shape[0] == 2 and v.dtype == np.float32:
-            mm = v
         if "muamus" in user:
             mm = np.asarray(user["muamus"], dtype=np.float32)
         if mm is not None:

@@ -266,13 +266,89 @@ static inline int inside(const OParam* g, short x, short y, short z) {
     return (unsigned short)x < g->maxidx.x && (unsigned short)y < g->maxidx.y && (unsigned short)z < g->maxidx.z;
 }
 
+/* binary16 bits (sign clear) to float: the value m x 2^(e-25) for e > 0,
+   m x 2^-24 for the subnormals, exactly */
+static float half_to_float(uint32_t h) {
+    const uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+
+    if (e == 31) {
+        return m ? NAN : INFINITY;
+    }
+
+    return e ? ldexpf((float)(m | 0x400u), (int)e - 25) : ldexpf((float)m, -24);
+}
+
+/* src/mcx_core.cl:1079-1193 (updateproperty): a packed media word decoded
+   against the label table; the launch sets the properties to t[1] first
+   (src/mcx_core.cl:2213) and the formats below overwrite only the components
+   they carry, so the others are t[1]'s */
+static F4 media_word_prop(uint32_t format, const F4* t, uint32_t mediaid) {
+    union {
+        uint32_t i;
+        float f;
+        uint16_t s[2];
+        uint8_t c[4];
+    } w;
+    w.i = mediaid & MED_MASK;
+    F4 p = t[1];
+    const float nw = t[w.i ? 1 : 0].w;
+
+    switch (format) {
+        case MEDIA_MUA_FLOAT:             /* :1089-1096 */
+            p.x = fabsf(w.f);
+            p.w = nw;
+            break;
+
+        case MEDIA_AS_F2H:                /* :1099-1123 */
+        case MEDIA_AS_HALF:
+            p.x = fabsf(half_to_float(w.s[0] & 0x7FFFu));
+            p.y = fabsf(half_to_float(w.s[1] & 0x7FFFu));
+            p.w = nw;
+            break;
+
+        case MEDIA_LABEL_HALF: {          /* :1126-1150 */
+            p = t[w.s[0] & 0x3FFF];
+            const float r = fabsf(half_to_float(w.s[1] & 0x7FFFu));
+            const int k = (w.s[0] & 0xC000) >> 14;
+            if (k == 0) p.x = r;
+            if (k == 1) p.y = r;
+            if (k == 2) p.z = r;
+            if (k == 3) p.w = r;
+            break;
+        }
+
+        case MEDIA_ASGN_BYTE:             /* :1153-1169 */
+            p.x = w.c[0] * (1.f / 255.f) * (t[2].x - t[1].x) + t[1].x;
+            p.y = w.c[1] * (1.f / 255.f) * (t[2].y - t[1].y) + t[1].y;
+            p.z = w.c[2] * (1.f / 255.f) * (t[2].z - t[1].z) + t[1].z;
+            p.w = w.c[3] * (1.f / 127.f) * (t[2].w - t[1].w) + t[1].w;
+            break;
+
+        case MEDIA_AS_SHORT:              /* :1172-1186 */
+            p.x = w.s[0] * (1.f / 65535.f) * (t[2].x - t[1].x) + t[1].x;
+            p.y = w.s[1] * (1.f / 65535.f) * (t[2].y - t[1].y) + t[1].y;
+            p.w = nw;
+            break;
+    }
+
+    return p;
+}
+
 static inline F4 getprop(const OParam* g, uint32_t mediaid) {
+    if (g->mediaformat > 4) {
+        return media_word_prop(g->mediaformat, g->prop, mediaid);
+    }
+
     return g->prop[mediaid & MED_MASK];
 }
 
 /* the properties of the photon's voxel: with per-voxel media the label's
    mua and mus are replaced by the voxel's own, g and n stay the label's */
 static inline F4 getvoxprop(const OParam* g, uint32_t mediaid, uint32_t idx1d) {
+    if (g->mediaformat > 4) {
+        return media_word_prop(g->mediaformat, g->prop, mediaid);
+    }
+
     F4 p = g->prop[mediaid & MED_MASK];
 
     if (g->muamus && (mediaid & MED_MASK)) {
@@ -340,11 +416,15 @@ static int skipvoid(const OParam* g, F4* p, F4* v, F4* f, S4* fd) {
 
                 F4 htime = getprop(g, g->media[idx1d]);
 
+                /* the entry voxel's n: a packed word's decoded n (the reference
+                   indexes the label table with the word, src/mcx_core.cl:1424-1428) */
+                const float nin = (g->mediaformat > 4) ? htime.w : g->prop[g->media[idx1d] & MED_MASK].w;
+
                 if (g->isspecular && htime.w != g->prop[0].w) {
-                    p->w *= 1.f - reflectcoeff(v, g->prop[0].w, g->prop[g->media[idx1d] & MED_MASK].w, fd->w);
+                    p->w *= 1.f - reflectcoeff(v, g->prop[0].w, nin, fd->w);
 
                     if (p->w > K_EPS) {
-                        transmit(v, g->prop[0].w, g->prop[g->media[idx1d] & MED_MASK].w, fd->w);
+                        transmit(v, g->prop[0].w, nin, fd->w);
                     }
                 }
 
