@@ -67,7 +67,10 @@ struct MCXKParam {
     uint32_t simple;              /* 1: pencil beam from a non-void voxel without focus, extra sources,
                                      replay, 2D, phase table, reflectance tally or replay output types */
     uint32_t full;                /* 1: the run uses a pattern, fourierx or hyperboloid launch, an
-                                     angle-CDF launch, shared patterns or RF/time-weighted replay outputs */
+                                     angle-CDF launch, shared patterns, RF/time-weighted replay outputs,
+                                     packed media words or trajectory records */
+    uint32_t savetraj;            /* 1: photon positions are recorded at launch, scattering and termination */
+    uint32_t maxjumpdebug;        /* trajectory records the buffer holds */
     unsigned char bc[12];
 
     const uint32_t* media;        /* label volume, x fastest; bit 31 marks detector-adjacent voxels */
@@ -89,6 +92,8 @@ struct MCXKParam {
     float* detrec;                /* detected-photon records, reclen floats each */
     uint64_t* detseed;            /* 2 words per detected photon */
     uint32_t* detcount;
+    float* debugdata;             /* trajectory records: {photon id bits, x, y, z, w, source id} */
+    uint32_t* jumpdebug;          /* trajectory records written, counted past maxjumpdebug */
 };
 
 /* per-instance output of a batched step: grid, float output and normalisation */

@@ -561,6 +561,29 @@ __device__ __forceinline__ uint32_t photon_index(const MCXKParam& g, int idx, fl
     return (uint32_t)(idx * (int)g.threadphoton + (idx < odd ? idx : odd) + (int)fw);
 }
 
+/* trajectory record (src/mcx_core.cl:929-944): {photon id as bits, x, y, z, w,
+   source id}; positions past maxjumpdebug are counted, not stored */
+__device__ __forceinline__ void save_traj(const MCXKParam& g, const float4& p, uint32_t id, float srcid) {
+    const uint32_t pos = atomicAdd(g.jumpdebug, 1u);
+
+    if (pos < g.maxjumpdebug) {
+        float* r = g.debugdata + (size_t)pos * 6;
+        r[0] = __uint_as_float(id);
+        r[1] = p.x;
+        r[2] = p.y;
+        r[3] = p.z;
+        r[4] = p.w;
+        r[5] = (float)(int)srcid;
+    }
+}
+
+/* the photon id the reference writes into launch and scattering records
+   (src/mcx_core.cl:2247, :2630): threads past oddphoton add 0 where the
+   termination record (:1501) and the photon count add oddphoton */
+__device__ __forceinline__ uint32_t traj_index(const MCXKParam& g, int idx, float fw) {
+    return (uint32_t)(idx * (int)g.threadphoton + (idx < g.oddphoton ? idx : 0) + (int)fw);
+}
+
 __device__ __forceinline__ int gate_of(const MCXKParam& g, float t) {
     int gate = (int)floorf((t - g.twin0) * g.Rtstep);
     return gate > (int)g.maxgate - 1 ? (int)g.maxgate - 1 : gate;
@@ -675,6 +698,10 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
 
     if (fabsf(p.w) >= 0.f) {
         pp[pd] += p.w;
+
+        if (FULL && g.savetraj) {
+            save_traj(g, p, photon_index(g, idx, f.w), pp[g.w0offset - 1]);
+        }
 
         if (!SIMPLE && g.issaveref == 1 && s.mediaid == 0 && s.idx1d != K_OUT_MIN && s.idx1d != K_OUT_MAX &&
                 p.w > 0.f) {
@@ -1134,6 +1161,11 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
 
     f.w += 1.f;
     s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
+
+    if (FULL && g.savetraj) {
+        save_traj(g, p, traj_index(g, idx, f.w), ppl[3]);   /* the source-id slot */
+    }
+
     ppl[1] += p.w;
     s.w0 = p.w;
     ppl[2] = (g.srcnum > 1) ? ppl[2] : p.w;
@@ -1292,6 +1324,10 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
                     if (FULL && g.outputtype == kRFmus) {
                         deposit(g, sh.repoff, s.idx1d + (uint32_t)tshift * g.nvox + g.imoff, im);
                     }
+                }
+
+                if (FULL && g.savetraj) {
+                    save_traj(g, p, traj_index(g, idx, f.w), sh.pp[g.w0offset - 1]);
                 }
             }
 

@@ -274,9 +274,9 @@ void mcx_check_supported(const Config* cfg) {
     }
 
     if (cfg->polmedianum || cfg->issaveref > 1 || cfg->issavedet == FILL_MAXDETPHOTON ||
-            (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY))) {
-        MCX_ERROR(-4, "polarization, issaveref>1, fill-max mode and trajectory saving are not supported by the "
-                  "MI355X kernel");
+            (cfg->debuglevel & MCX_DEBUG_MOVE_ONLY)) {
+        MCX_ERROR(-4, "polarization, issaveref>1, fill-max mode and move-only trajectories (debug flag T) are not "
+                  "supported by the MI355X kernel");
     }
 }
 
@@ -381,8 +381,8 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     }
 
     /* the specialised launch/scatter/deposit path: a pencil beam starting inside a
-       non-void voxel, no focus or angular launch, no extra sources, no replay and
-       only forward output types */
+       non-void voxel of a label volume, no focus or angular launch, no extra
+       sources, no replay, no trajectory records and only forward output types */
     {
         union {
             float f;
@@ -390,7 +390,7 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
         } lab;
         lab.f = cfg->srcparam2.w;
         g.simple = cfg->srctype == MCX_SRC_PENCIL && cfg->srcdir.w == 0.f && (lab.u & MED_MASK) != 0 &&
-                   cfg->mediabyte <= 4 &&
+                   cfg->mediabyte <= 4 && !(cfg->debuglevel & MCX_DEBUG_MOVE) &&
                    cfg->extrasrclen == 0 && cfg->seed != SEED_FROM_FILE && cfg->issaveref == 0 && g.is2d == 0 &&
                    cfg->nphase <= 2 && cfg->nangle == 0 && cfg->srcpos.w > 0.f && cfg->minenergy < cfg->srcpos.w &&
                    (cfg->outputtype == otFlux || cfg->outputtype == otFluence || cfg->outputtype == otEnergy ||
@@ -402,7 +402,11 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
              (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->srctype == MCX_SRC_FOURIERX ||
               cfg->srctype == MCX_SRC_FOURIERX2D || cfg->srctype == MCX_SRC_HYPERBOLOID_GAUSSIAN || cfg->nangle ||
               cfg->outputtype == otRF || cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
-              cfg->outputtype == otWPTOF || cfg->mediabyte > 4);
+              cfg->outputtype == otWPTOF || cfg->mediabyte > 4 || (cfg->debuglevel & MCX_DEBUG_MOVE));
+
+    /* trajectory records (src/mcx_core.cl:929-944) */
+    g.savetraj = (cfg->debuglevel & MCX_DEBUG_MOVE) ? 1u : 0u;
+    g.maxjumpdebug = cfg->maxjumpdebug;
 
     /* reflection block rule (src/mcx_host.cpp:945-956) */
     {
@@ -601,8 +605,10 @@ static void mcx_plan_run(Config* cfg, int dev, int ncu, size_t* nthread_io, MCXK
     const char* envdyn = getenv("MCXHIP_DYNAMIC");
     const int fast = cfg->mathmode == 0 && !(cfg->debuglevel & MCX_DEBUG_RNG);
     /* fast mode hands photons out at run time (below) unless replaying, where
-       each lane must rerun the photons of its recorded seeds */
-    const int dynamic = fast && cfg->seed != SEED_FROM_FILE && !(envdyn && envdyn[0] == '0');
+       each lane must rerun the photons of its recorded seeds, or recording
+       trajectories, whose photon ids follow the fixed map */
+    const int dynamic = fast && cfg->seed != SEED_FROM_FILE && !(cfg->debuglevel & MCX_DEBUG_MOVE) &&
+                        !(envdyn && envdyn[0] == '0');
     size_t lblock = MCX_LAUNCH_BLOCK;
     MCXKParam& g = *gp;
     int reflect = 0;
@@ -824,6 +830,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     float* ddetw = NULL;
     float2* dmuamus = NULL;
     uint32_t* ddetcount = NULL;
+    float* ddebug = NULL;
+    uint32_t* djump = NULL;
     float* dout = NULL;
     float* draw32 = NULL;
     float* djacob = NULL;
@@ -901,6 +909,12 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     HIP_TRY(dalloc(bufs, &ddetcount, 1));
     HIP_TRY(hipMemsetAsync(ddetcount, 0, sizeof(uint32_t), stream));
 
+    if (g.savetraj) {
+        HIP_TRY(dalloc(bufs, &ddebug, (size_t)cfg->maxjumpdebug * MCX_DEBUG_REC_LEN));
+        HIP_TRY(dalloc(bufs, &djump, 1));
+        HIP_TRY(hipMemsetAsync(djump, 0, sizeof(uint32_t), stream));
+    }
+
     if (cfg->issavedet) {
         HIP_TRY(dalloc(bufs, &ddetrec, (size_t)cfg->maxdetphoton * (reclen ? reclen : 1)));
 
@@ -954,6 +968,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     g.detseed = ddetseed;
     g.detcount = ddetcount;
     g.detweight = ddetw;
+    g.debugdata = ddebug;
+    g.jumpdebug = djump;
     g.muamus = dmuamus;
 
     /* the post-processing buffers are allocated before the timed region: a
@@ -1035,6 +1051,34 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     }
 
     cfg->energyabs = cfg->energytot - cfg->energyesc;
+
+    /* trajectory records, appended to the caller's (src/mcx_host.cpp:1173-1193);
+       the counter runs over all repetitions, so each record is read once */
+    if (g.savetraj) {
+        uint32_t njump = 0;
+        HIP_TRY(hipMemcpy(&njump, djump, sizeof(uint32_t), hipMemcpyDeviceToHost));
+
+        if (njump > cfg->maxjumpdebug) {
+            fprintf(cfg->flog ? cfg->flog : stdout,
+                    "WARNING: the saved trajectory positions are more than what your have specified (%u > %u), please "
+                    "use the --maxjumpdebug option to specify a greater number\n", njump, cfg->maxjumpdebug);
+            njump = cfg->maxjumpdebug;
+        }
+
+        if (njump) {
+            float* buf = (float*)realloc(cfg->exportdebugdata, ((size_t)cfg->debugdatalen + njump) *
+                                         MCX_DEBUG_REC_LEN * sizeof(float));
+
+            if (!buf) {
+                HIP_TRY(hipErrorOutOfMemory);
+            }
+
+            cfg->exportdebugdata = buf;
+            HIP_TRY(hipMemcpy(buf + (size_t)cfg->debugdatalen * MCX_DEBUG_REC_LEN, ddebug,
+                              (size_t)njump * MCX_DEBUG_REC_LEN * sizeof(float), hipMemcpyDeviceToHost));
+            cfg->debugdatalen += njump;
+        }
+    }
 
     /* detected photons (src/mcx_host.cpp:1195-1237) */
     if (cfg->issavedet) {
@@ -1248,6 +1292,19 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             cfg->his.seedbyte = (cfg->issaveseed && cfg->seeddata) ? 16 : 0;
             cfg->his.detected = cfg->detectedcount;
             mcx_savedetphoton(cfg->exportdetected, cfg->seeddata, (int)cfg->detectedcount, 0, cfg);
+        }
+
+        /* trajectories: a .mct file, the .mch layout with 6 columns (src/mcx_host.cpp:1666-1672) */
+        if (g.savetraj && cfg->exportdebugdata) {
+            const unsigned int colcount = cfg->his.colcount, seedbyte = cfg->his.seedbyte;
+            cfg->his.colcount = MCX_DEBUG_REC_LEN;
+            cfg->his.savedphoton = cfg->debugdatalen;
+            cfg->his.totalphoton = (unsigned int)cfg->nphoton;
+            cfg->his.detected = 0;
+            cfg->his.seedbyte = 0;
+            mcx_savedetphoton(cfg->exportdebugdata, NULL, (int)cfg->debugdatalen, 0, cfg);
+            cfg->his.colcount = colcount;
+            cfg->his.seedbyte = seedbyte;
         }
     }
 
@@ -1497,9 +1554,10 @@ static void noalloc_run(Config* cfg, const uint32_t* media, size_t nmedia, doubl
     mcx_check_supported(cfg);
 
     if (cfg->issavedet || cfg->seed == SEED_FROM_FILE || MCX_IS_ADJOINT_TYPE(cfg->outputtype) ||
-            (cfg->debuglevel & MCX_DEBUG_RNG) || cfg->muamus || cfg->mediabyte > 4 || !cfg->issave2pt) {
+            (cfg->debuglevel & (MCX_DEBUG_RNG | MCX_DEBUG_MOVE)) || cfg->muamus || cfg->mediabyte > 4 ||
+            !cfg->issave2pt) {
         MCX_ERROR(-4, "mcx_run_simulation_noalloc runs forward fluence simulations only: no detected photons, "
-                  "replay, adjoint outputs, RNG dump or per-voxel media, and the field must be saved");
+                  "replay, adjoint outputs, RNG dump, trajectories or per-voxel media, and the field must be saved");
     }
 
     if (!media || !field64) {

@@ -138,6 +138,9 @@ typedef struct {
     uint32_t npattern;         /* patterns sharing each photon (srcnum of a pattern source, else 1) */
     uint32_t rfmul;            /* 2 for the complex RF replay outputs (imaginary half after the real) */
     float omega;               /* RF replay: angular modulation frequency (rad/s) */
+    uint32_t savetraj, maxjumpdebug;  /* trajectory records (src/mcx_core.cl:929-944) */
+    float* debugdata;                 /* maxjumpdebug records of 6 floats */
+    uint32_t* jumpdebug;              /* records written, counted past maxjumpdebug */
 } OParam;
 
 typedef struct {
@@ -553,6 +556,30 @@ static inline uint32_t photon_index(const OParam* g, int idx, float fw) {
     return (uint32_t)(idx * (int)g->threadphoton + (idx < odd ? idx : odd) + (int)fw);
 }
 
+/* the photon id of launch and scattering trajectory records (src/mcx_core.cl:2247,
+   :2630): min(idx, (idx < oddphoton) * idx), where termination records (:1501)
+   use min(idx, oddphoton) as photon_index does */
+static inline uint32_t traj_index(const OParam* g, int idx, float fw) {
+    return (uint32_t)(idx * (int)g->threadphoton + (idx < g->oddphoton ? idx : 0) + (int)fw);
+}
+
+/* src/mcx_core.cl:929-944: {id as bits, x, y, z, w, (int) source id} */
+static void save_traj(const OParam* g, const F4* p, uint32_t id, float srcid) {
+    uint32_t pos;
+#pragma omp atomic capture
+    pos = (*g->jumpdebug)++;
+
+    if (pos < g->maxjumpdebug) {
+        float* r = g->debugdata + (size_t)pos * 6;
+        memcpy(&r[0], &id, sizeof(id));
+        r[1] = p->x;
+        r[2] = p->y;
+        r[3] = p->z;
+        r[4] = p->w;
+        r[5] = (float)(int)srcid;
+    }
+}
+
 /* launch weights of a pattern pixel (src/mcx_core.cl:1678-1727): one pattern
    scales the photon weight; several share the photon (unit weight), each
    pattern's weight kept per photon, the pixel index in the w0 slot as bits */
@@ -583,6 +610,10 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
 
     if (pm_fabs(s->p.w) >= 0.f) {    /* false only for the NaN of the first call */
         s->pp[pd].f += s->p.w;
+
+        if (g->savetraj) {
+            save_traj(g, &s->p, photon_index(g, idx, s->f.w), s->pp[g->w0offset - 1].f);
+        }
 
         if (s->mediaid == 0 && s->idx1d != OUTSIDE_VOLUME_MIN && s->idx1d != OUTSIDE_VOLUME_MAX && g->issaveref &&
                 s->p.w > 0.f && g->issaveref == 1) {
@@ -1021,6 +1052,12 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
 
     f->w += 1.f;
     s->prop = getvoxprop(g, s->mediaid, s->idx1d);
+
+    if (g->savetraj) {
+        /* ppath[3] of the shifted pointer: the source-id slot (src/mcx_core.cl:2247) */
+        save_traj(g, p, traj_index(g, idx, f->w), pp[3]);
+    }
+
     pp[1] += p->w;
     s->w0 = p->w;
     pp[2] = (g->srcnum > 1) ? pp[2] : p->w;
@@ -1177,6 +1214,10 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                     if (g->outputtype == otRFmus) {
                         deposit(g, w, s->idx1d + (uint32_t)tshift * g->nvox + g->fieldlen, im);
                     }
+                }
+
+                if (g->savetraj) {    /* src/mcx_core.cl:2625-2633 */
+                    save_traj(g, p, traj_index(g, idx, f->w), s->pp[g->w0offset - 1].f);
                 }
             }
 
@@ -1620,6 +1661,13 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
         works[i].field = (OTally*)calloc(fieldlen, sizeof(OTally));
     }
 
+    /* trajectory records (src/mcx_core.cl:929-944), one buffer over all repetitions */
+    uint32_t njump = 0;
+    g.savetraj = (cfg->debuglevel & MCX_DEBUG_MOVE) ? 1u : 0u;
+    g.maxjumpdebug = cfg->maxjumpdebug;
+    g.jumpdebug = &njump;
+    g.debugdata = g.savetraj ? (float*)calloc((size_t)cfg->maxjumpdebug * MCX_DEBUG_REC_LEN + 1, sizeof(float)) : NULL;
+
     double tic = now_sec();
 
     for (unsigned int pass = 0; pass < respin; pass++) {
@@ -1680,6 +1728,20 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
         res->cnt.nbnd += works[i].cnt.nbnd;
         res->cnt.nlaunch += works[i].cnt.nlaunch;
         res->cnt.nrand += works[i].cnt.nrand;
+    }
+
+    if (g.savetraj) {
+        const uint32_t nj = njump < cfg->maxjumpdebug ? njump : cfg->maxjumpdebug;
+
+        if (nj) {
+            cfg->exportdebugdata = (float*)realloc(cfg->exportdebugdata, ((size_t)cfg->debugdatalen + nj) *
+                                                   MCX_DEBUG_REC_LEN * sizeof(float));
+            memcpy(cfg->exportdebugdata + (size_t)cfg->debugdatalen * MCX_DEBUG_REC_LEN, g.debugdata,
+                   (size_t)nj * MCX_DEBUG_REC_LEN * sizeof(float));
+            cfg->debugdatalen += nj;
+        }
+
+        free(g.debugdata);
     }
 
     /* detected photons */
