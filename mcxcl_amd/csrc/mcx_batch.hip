@@ -376,9 +376,9 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
         }
 
         if (cfg->issavedet || cfg->seed == SEED_FROM_FILE || (cfg->debuglevel & (MCX_DEBUG_RNG | MCX_DEBUG_MOVE)) ||
-                cfg->muamus || cfg->mediabyte > 4 || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
-            MCX_ERROR(-4, "the batched launch takes no detected photons, replay, RNG dump, trajectories, per-voxel "
-                      "or packed media or adjoint outputs");
+                cfg->muamus || cfg->mediabyte > 4 || cfg->omega > 0.f || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
+            MCX_ERROR(-4, "the batched launch takes no detected photons, replay, RNG dump, trajectories, RF, "
+                      "per-voxel or packed media or adjoint outputs");
         }
 
         if (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->nangle) {

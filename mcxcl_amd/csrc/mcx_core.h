@@ -60,8 +60,10 @@ struct MCXKParam {
     uint32_t nangle;              /* entries of the launch polar-angle table (0: none) */
     uint32_t pplen;               /* floats of per-lane LDS scratch (records, tallies, w0, source id,
                                      pattern weights, the RF replay phase) */
-    uint32_t imoff;               /* complex RF replay outputs: offset of the imaginary half (0: real output) */
-    float omega;                  /* RF replay: angular modulation frequency, rad/s */
+    uint32_t imoff;               /* complex outputs (RF replay, RF forward): offset of the imaginary half
+                                     (0: real output) */
+    float omega;                  /* angular modulation frequency, rad/s */
+    uint32_t rfforward;           /* 1: complex photon weights and fluence (omega > 0 without replay) */
     uint32_t dynamic;             /* 1: photons are handed out per workgroup at run time
                                      (threadphoton/oddphoton then split photons over workgroups) */
     uint32_t simple;              /* 1: pencil beam from a non-void voxel without focus, extra sources,

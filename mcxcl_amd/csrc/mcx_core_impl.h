@@ -1172,7 +1172,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
     v.w = K_EPS;
     s.Lmove = 0.f;
 
-    if (FULL && g.imoff) {
+    if (FULL && g.imoff && g.seed == K_SEED_FROM_FILE) {
         /* RF replay: the phase omega * tof of this photon (src/mcx_core.cl:2257-2263,
            which reads the next photon's time of flight; here the photon's own) */
         float rs, rc;
@@ -1214,6 +1214,10 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
     DepCache dcache;
     dcache.a0 = dcache.a1 = 0xFFFFFFFFu;
     dcache.v0 = dcache.v1 = 0.0;
+    /* RF forward: the complex photon weight and its value at the last voxel exit */
+    struct {
+        float wre, wim, w0re, w0im;
+    } rf = {0.f, 0.f, 0.f, 0.f};
 
     for (;;) {
         if (relaunch) {
@@ -1225,6 +1229,11 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
             s.mediaid &= K_MED_MASK;
             relaunch = false;
             keepold = false;
+
+            if (FULL && g.rfforward) {
+                rf.wre = rf.w0re = s.p.w;
+                rf.wim = rf.w0im = 0.f;
+            }
         }
 
         float4& p = s.p;
@@ -1367,7 +1376,21 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
             s.fz += (slen == f.x) ? 0 : (v.z > 0.f ? 1 : -1);
         }
 
-        p.w *= kexp(-s.prop.x * f.z);
+        if (FULL && g.rfforward) {
+            /* RF forward (src/mcx_core.cl:2750-2760): w <- w exp(-(mua + i omega n / c0) ds),
+               p.w its magnitude */
+            const float att = kexp(-s.prop.x * f.z);
+            float rs, rc;
+            ksincos(g.omega * s.prop.w * g.oneoverc0 * f.z, &rs, &rc);
+            const float tre = att * (rf.wre * rc + rf.wim * rs);
+            const float tim = att * (-rf.wre * rs + rf.wim * rc);
+            rf.wre = tre;
+            rf.wim = tim;
+            p.w = ksqrt(rf.wre * rf.wre + rf.wim * rf.wim);
+        } else {
+            p.w *= kexp(-s.prop.x * f.z);
+        }
+
         f.x -= slen;
         f.y += f.z * s.prop.w * g.oneoverc0;
 
@@ -1395,7 +1418,14 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
                 float weight = 0.f, weight_im = 0.f;
                 int tshift = gate_of(g, f.y);
 
-                if (g.outputtype == kEnergy) {
+                if (FULL && g.rfforward) {
+                    /* RF forward (src/mcx_core.cl:2833-2839): (w0 - w) / (mua + i omega n / c0) */
+                    const float dre = rf.w0re - rf.wre, dim = rf.w0im - rf.wim;
+                    const float aim = g.omega * s.prop.w * g.oneoverc0;
+                    const float amag2 = s.prop.x * s.prop.x + aim * aim;
+                    weight = (amag2 < K_EPS) ? (rf.w0re * pathlen) : kdiv(dre * s.prop.x + dim * aim, amag2);
+                    weight_im = (amag2 < K_EPS) ? (rf.w0im * pathlen) : kdiv(dim * s.prop.x - dre * aim, amag2);
+                } else if (g.outputtype == kEnergy) {
                     weight = s.w0 - p.w;
                 } else if (g.outputtype == kFluence || g.outputtype == kFlux) {
                     weight = (s.prop.x < K_EPS) ? (s.w0 * pathlen) : kdiv(s.w0 - p.w, s.prop.x);
@@ -1438,14 +1468,25 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
                     }
                 } else if (fabsf(weight) > 0.f) {
                     deposit_voxel(g, sh, dcache, ox, oy, oz, idx1dold, tshift, weight);
+
+                    if (FULL && g.rfforward) {
+                        deposit(g, sh.repoff, idx1dold + (uint32_t)tshift * g.nvox + g.imoff, weight_im);
+                    }
                 }
 
-                if (FULL && g.outputtype == kRF && g.omega > 0.f && fabsf(weight_im) > 0.f) {
+                if (FULL && g.outputtype == kRF && g.seed == K_SEED_FROM_FILE && g.omega > 0.f &&
+                        fabsf(weight_im) > 0.f) {
                     deposit(g, sh.repoff, idx1dold + (uint32_t)tshift * g.nvox + g.imoff, weight_im);
                 }
             }
 
             s.w0 = p.w;
+
+            if (FULL && g.rfforward) {
+                rf.w0re = rf.wre;
+                rf.w0im = rf.wim;
+            }
+
             pathlen = 0.f;
         } else {
             s.mediaid = mediaidold;
@@ -1487,6 +1528,13 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
         if (fabsf(p.w) < g.minenergy) {
             if (rng.next() * K_ROULETTE <= 1.f) {
                 p.w *= K_ROULETTE;
+
+                if (FULL && g.rfforward) {
+                    rf.wre *= K_ROULETTE;
+                    rf.wim *= K_ROULETTE;
+                    rf.w0re *= K_ROULETTE;
+                    rf.w0im *= K_ROULETTE;
+                }
             } else {
                 launchflag = mediaidold & K_DET_MASK;
                 relaunch = true;

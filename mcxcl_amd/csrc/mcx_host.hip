@@ -248,6 +248,14 @@ void mcx_check_supported(const Config* cfg) {
         MCX_ERROR(-4, "replay, per-voxel mua/mus arrays and adjoint outputs need a label volume");
     }
 
+    /* RF forward (complex photon weights): one source, real outputs after the
+       complex field only (the reference's multi-source and adjoint RF layouts
+       are not built) */
+    if (cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE &&
+            ((cfg->extrasrclen && cfg->srcid < 0) || cfg->srcnum > 1 || MCX_IS_ADJOINT_TYPE(cfg->outputtype))) {
+        MCX_ERROR(-4, "RF forward simulations (omega > 0) take one source and no adjoint outputs");
+    }
+
     if (cfg->mediabyte == MEDIA_LABEL_HALF && (cfg->savedetflag & 0xEu)) {
         MCX_ERROR(-4, "scattering counts, partial path lengths and momentum transfer of detected photons need a "
                   "label volume (labelplus words carry a replaced property)");
@@ -293,8 +301,10 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
                                    cfg->srcnum > 1) ? cfg->srcnum : 1;
     const unsigned int nsrcslot = (cfg->srcid < 0) ? cfg->extrasrclen + 1 : npattern;
     const unsigned int ndetslot = (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) ? cfg->detnum : 1;
-    /* complex RF replay outputs: the imaginary half follows the real one */
-    const int isrf = cfg->seed == SEED_FROM_FILE && (cfg->outputtype == otRF || cfg->outputtype == otRFmus);
+    /* complex outputs (RF replay, RF forward): the imaginary half follows the real one */
+    const int isrfreplay = cfg->seed == SEED_FROM_FILE && (cfg->outputtype == otRF || cfg->outputtype == otRFmus);
+    const int isrfforward = cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE;
+    const int isrf = isrfreplay || isrfforward;
     const size_t halflen = nvox * maxgate * nsrcslot * ndetslot;
     const size_t fieldlen = halflen * (isrf ? 2 : 1);
     const unsigned int partialdata = mcx_partialdata(cfg);
@@ -367,7 +377,8 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     g.omega = cfg->omega;
     /* lane scratch: record columns, 2 tallies, w0, source id, one weight per
        shared pattern (srcnum), the RF replay phase (cos, sin) */
-    g.pplen = g.w0offset + g.srcnum + (isrf ? 2u : 0u);
+    g.pplen = g.w0offset + g.srcnum + (isrfreplay ? 2u : 0u);
+    g.rfforward = isrfforward ? 1u : 0u;
     memcpy(g.bc, cfg->bc, 12);
 
     {
@@ -390,7 +401,7 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
         } lab;
         lab.f = cfg->srcparam2.w;
         g.simple = cfg->srctype == MCX_SRC_PENCIL && cfg->srcdir.w == 0.f && (lab.u & MED_MASK) != 0 &&
-                   cfg->mediabyte <= 4 && !(cfg->debuglevel & MCX_DEBUG_MOVE) &&
+                   cfg->mediabyte <= 4 && !(cfg->debuglevel & MCX_DEBUG_MOVE) && !(cfg->omega > 0.f) &&
                    cfg->extrasrclen == 0 && cfg->seed != SEED_FROM_FILE && cfg->issaveref == 0 && g.is2d == 0 &&
                    cfg->nphase <= 2 && cfg->nangle == 0 && cfg->srcpos.w > 0.f && cfg->minenergy < cfg->srcpos.w &&
                    (cfg->outputtype == otFlux || cfg->outputtype == otFluence || cfg->outputtype == otEnergy ||
@@ -402,7 +413,8 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
              (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->srctype == MCX_SRC_FOURIERX ||
               cfg->srctype == MCX_SRC_FOURIERX2D || cfg->srctype == MCX_SRC_HYPERBOLOID_GAUSSIAN || cfg->nangle ||
               cfg->outputtype == otRF || cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
-              cfg->outputtype == otWPTOF || cfg->mediabyte > 4 || (cfg->debuglevel & MCX_DEBUG_MOVE));
+              cfg->outputtype == otWPTOF || cfg->mediabyte > 4 || (cfg->debuglevel & MCX_DEBUG_MOVE) ||
+              (cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE));
 
     /* trajectory records (src/mcx_core.cl:929-944) */
     g.savetraj = (cfg->debuglevel & MCX_DEBUG_MOVE) ? 1u : 0u;
@@ -563,7 +575,8 @@ static float mcx_output_scale(const Config* cfg) {
     const float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
     float scale = 1.f;
 
-    if (cfg->outputtype == otFlux || cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
+    if (cfg->outputtype == otFlux || cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype) ||
+            (cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE)) {
         scale = cfg->unitinmm / (float)(cfg->energytot * Vvox * cfg->tstep);
 
         if (cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
@@ -1555,9 +1568,9 @@ static void noalloc_run(Config* cfg, const uint32_t* media, size_t nmedia, doubl
 
     if (cfg->issavedet || cfg->seed == SEED_FROM_FILE || MCX_IS_ADJOINT_TYPE(cfg->outputtype) ||
             (cfg->debuglevel & (MCX_DEBUG_RNG | MCX_DEBUG_MOVE)) || cfg->muamus || cfg->mediabyte > 4 ||
-            !cfg->issave2pt) {
+            cfg->omega > 0.f || !cfg->issave2pt) {
         MCX_ERROR(-4, "mcx_run_simulation_noalloc runs forward fluence simulations only: no detected photons, "
-                  "replay, adjoint outputs, RNG dump, trajectories or per-voxel media, and the field must be saved");
+                  "replay, adjoint outputs, RNG dump, trajectories, RF or per-voxel media, and the field must be saved");
     }
 
     if (!media || !field64) {

@@ -2084,8 +2084,8 @@ void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
         MCX_ERROR(-6, "Jacobian output is only valid in the reply mode. Please define cfg.seed");
     }
 
-    if ((cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE) || cfg->srcid == -2) {
-        MCX_ERROR(-6, "RF forward simulation (omega > 0 without replay) and srcid -2 are not supported by this build");
+    if (cfg->srcid == -2) {
+        MCX_ERROR(-6, "srcid -2 is not supported by this build");
     }
 
     /* adjoint outputs: every detector becomes an extra disk source at its

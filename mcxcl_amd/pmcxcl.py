@@ -310,8 +310,8 @@ class PreparedConfig:
         self.nsrc = (c.extrasrclen + 1) if (c.extrasrclen and c.srcid < 0) else 1
         # several patterns share each photon: one interleaved slot per pattern
         self.npattern = c.srcnum if (c.srctype[0] in (5, 15) and c.srcnum > 1) else 1
-        # complex RF replay outputs: the imaginary half after the real one
-        self.nrf = 2 if (c.seed == SEED_FROM_FILE and c.outputtype[0] in (OT_RF, OT_RFMUS)) else 1
+        # complex outputs (RF replay, RF forward): the imaginary half after the real one
+        self.nrf = 2 if _is_complex(c) else 1
         self.fieldlen = self.nvox * self.ngate * self.ndet * self.nsrc * self.npattern * self.nrf
         self.raw = None
         self.raw32 = None
@@ -480,6 +480,12 @@ def run_batch(cfgs, nstep=1, **opts):
         return b.collect(), b.info
 
 
+def _is_complex(c):
+    """complex outputs: RF replay (outputtype r/s) or RF forward (omega > 0 without replay)"""
+    return (c.seed == SEED_FROM_FILE and c.outputtype[0] in (OT_RF, OT_RFMUS)) or \
+        (c.omega > 0 and c.seed != SEED_FROM_FILE)
+
+
 def _collect(c, user, ngate, ndet, nsrc):
     out = {}
     reclen = lib.mcx_detreclen(ctypes.byref(c))
@@ -500,7 +506,7 @@ def _collect(c, user, ngate, ndet, nsrc):
         out["traj"] = np.asfortranarray(traj.reshape(c.debugdatalen, 6).T)
     if c.issave2pt and c.exportfield:
         dims = [c.srcnum * c.dim.x, c.dim.y, c.dim.z, ngate]
-        nrf = 2 if (c.seed == SEED_FROM_FILE and c.outputtype[0] in (OT_RF, OT_RFMUS)) else 1
+        nrf = 2 if _is_complex(c) else 1
         if ndet > 1:
             dims.append(ndet)
         if nsrc > 1 or nrf > 1:
@@ -518,6 +524,11 @@ def _collect(c, user, ngate, ndet, nsrc):
             fl[~inside] = 0.0
             out["dref"] = dref.reshape(dims, order="F")
             flux = fl.reshape(dims, order="F")
+        if c.omega > 0 and c.seed != SEED_FROM_FILE:
+            # RF forward: complex fluence, real + 1j * imaginary half (src/pmcxcl.cpp:1387-1408)
+            flux = (flux[..., 0] + 1j * flux[..., 1]).astype(np.complex64)
+            if flux.ndim == 5 and flux.shape[4] == 1:
+                flux = flux[..., 0]
         out["flux"] = flux
     ot = c.outputtype[0]
     if ot >= OT_ADJOINT and c.exportjacob and c.detnum and c.extrasrclen + 1 > c.detnum:

@@ -136,7 +136,8 @@ typedef struct {
     const float* angleinvcdf;  /* launch polar angle inverse CDF in units of pi (nangle entries) */
     uint32_t nangle;
     uint32_t npattern;         /* patterns sharing each photon (srcnum of a pattern source, else 1) */
-    uint32_t rfmul;            /* 2 for the complex RF replay outputs (imaginary half after the real) */
+    uint32_t rfmul;            /* 2 for complex outputs, RF replay or forward (imaginary half after the real) */
+    uint32_t rfforward;        /* complex photon weights: omega > 0 without replay */
     float omega;               /* RF replay: angular modulation frequency (rad/s) */
     uint32_t savetraj, maxjumpdebug;  /* trajectory records (src/mcx_core.cl:929-944) */
     float* debugdata;                 /* maxjumpdebug records of 6 floats */
@@ -1146,12 +1147,24 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
         return;   /* handled by the caller */
     }
 
+    /* RF forward (omega > 0 without replay): the complex photon weight and its
+       value at the last voxel exit, reset at every launch (src/mcx_core.cl:2427-2430) */
+    float rwre = 0.f, rwim = 0.f, rw0re = 0.f, rw0im = 0.f;
+#define RF_RESET()                       \
+    do {                                 \
+        if (g->rfforward) {              \
+            rwre = rw0re = s->p.w;       \
+            rwim = rw0im = 0.f;          \
+        }                                \
+    } while (0)
+
     if (launch_photon(g, w, s, idx, 0)) {
         return;
     }
 
     isdet = s->mediaid & DET_MASK;
     s->mediaid &= MED_MASK;
+    RF_RESET();
 
     while (f->w <= (float)nlimit) {
         OCOUNT(w, nstep);
@@ -1248,7 +1261,19 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
             fd->z += (slen == f->x) ? 0 : (v->z > 0.f ? 1 : -1);
         }
 
-        p->w *= pm_exp(-s->prop.x * f->z);
+        if (g->rfforward) {
+            /* src/mcx_core.cl:2750-2760: w <- w exp(-(mua + i omega n / c0) ds), p.w = |w| */
+            const float att = pm_exp(-s->prop.x * f->z);
+            float rs, rc;
+            pm_sincos(g->omega * s->prop.w * g->oneoverc0 * f->z, &rs, &rc);
+            const float tre = att * (rwre * rc + rwim * rs);
+            const float tim = att * (-rwre * rs + rwim * rc);
+            rwre = tre;
+            rwim = tim;
+            p->w = pm_sqrt(rwre * rwre + rwim * rwim);
+        } else {
+            p->w *= pm_exp(-s->prop.x * f->z);
+        }
         f->x -= slen;
         f->y += f->z * s->prop.w * g->oneoverc0;
 
@@ -1276,7 +1301,14 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                 float weight = 0.f, weight_im = 0.f;
                 int tshift = gate_of(g, f->y);
 
-                if (g->outputtype == otEnergy) {
+                if (g->rfforward) {
+                    /* src/mcx_core.cl:2833-2839: (w0 - w) / (mua + i omega n / c0) */
+                    const float dre = rw0re - rwre, dim = rw0im - rwim;
+                    const float aim = g->omega * s->prop.w * g->oneoverc0;
+                    const float amag2 = s->prop.x * s->prop.x + aim * aim;
+                    weight = (amag2 < K_EPS) ? (rw0re * pathlen) : (dre * s->prop.x + dim * aim) / amag2;
+                    weight_im = (amag2 < K_EPS) ? (rw0im * pathlen) : (dim * s->prop.x - dre * aim) / amag2;
+                } else if (g->outputtype == otEnergy) {
                     weight = s->w0 - p->w;
                 } else if (g->outputtype == otFluence || g->outputtype == otFlux || MCX_IS_ADJOINT_TYPE(g->outputtype)) {
                     weight = (s->prop.x < K_EPS) ? (s->w0 * pathlen) : ((s->w0 - p->w) / s->prop.x);
@@ -1317,14 +1349,24 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                     }
                 } else if (pm_fabs(weight) > 0.f) {
                     deposit(g, w, idx1dold + (uint32_t)tshift * g->nvox, weight);
+
+                    if (g->rfforward) {    /* the imaginary part, gated by the real one (:2884-2893) */
+                        deposit(g, w, idx1dold + (uint32_t)tshift * g->nvox + g->fieldlen, weight_im);
+                    }
                 }
 
-                if (g->outputtype == otRF && g->omega > 0.f && pm_fabs(weight_im) > 0.f) {
+                if (g->outputtype == otRF && g->seed == SEED_FROM_FILE && g->omega > 0.f && pm_fabs(weight_im) > 0.f) {
                     deposit(g, w, idx1dold + (uint32_t)tshift * g->nvox + g->fieldlen, weight_im);
                 }
             }
 
             s->w0 = p->w;
+
+            if (g->rfforward) {
+                rw0re = rwre;
+                rw0im = rwim;
+            }
+
             pathlen = 0.f;
         } else {
             s->mediaid = mediaidold;
@@ -1370,12 +1412,20 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
 
             isdet = s->mediaid & DET_MASK;
             s->mediaid &= MED_MASK;
+            RF_RESET();
             continue;
         }
 
         if (pm_fabs(p->w) < g->minenergy) {
             if (rng_next(s->t, w) * ROULETTE_SIZE <= 1.f) {
                 p->w *= ROULETTE_SIZE;
+
+                if (g->rfforward) {    /* src/mcx_core.cl:3033-3040 */
+                    rwre *= ROULETTE_SIZE;
+                    rwim *= ROULETTE_SIZE;
+                    rw0re *= ROULETTE_SIZE;
+                    rw0im *= ROULETTE_SIZE;
+                }
             } else {
                 if (launch_photon(g, w, s, idx, mediaidold & DET_MASK)) {
                     break;
@@ -1383,6 +1433,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
 
                 isdet = s->mediaid & DET_MASK;
                 s->mediaid &= MED_MASK;
+                RF_RESET();
                 continue;
             }
         }
@@ -1427,6 +1478,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
 
                         isdet = s->mediaid & DET_MASK;
                         s->mediaid &= MED_MASK;
+                        RF_RESET();
                         continue;
                     }
                 } else {
@@ -1458,6 +1510,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
 
     genergy[idx * 2] = s->pp[pd].f;
     genergy[idx * 2 + 1] = s->pp[pd + 1].f;
+#undef RF_RESET
 }
 
 /* ---------------- host driver ---------------- */
@@ -1516,7 +1569,9 @@ static void oracle_setup(Config* cfg, OParam* gp, F4** props_out, float** muamus
     unsigned int ndetslot = (cfg->seed == SEED_FROM_FILE && cfg->replaydet == -1) ? cfg->detnum : 1;
     size_t fieldlen = (size_t)gp->nvox * maxgate * nsrcslot * ndetslot;
     gp->fieldlen = (uint32_t)fieldlen;
-    gp->rfmul = (cfg->seed == SEED_FROM_FILE && (cfg->outputtype == otRF || cfg->outputtype == otRFmus)) ? 2 : 1;
+    gp->rfforward = cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE;
+    gp->rfmul = ((cfg->seed == SEED_FROM_FILE && (cfg->outputtype == otRF || cfg->outputtype == otRFmus)) ||
+                 gp->rfforward) ? 2 : 1;
     gp->srcpattern = cfg->srcpattern;
     gp->nangle = cfg->nangle;
     gp->angleinvcdf = cfg->angleinvcdf;
@@ -1816,7 +1871,8 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
                                     cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
                                     cfg->outputtype == otWPTOF;
 
-            if (cfg->outputtype == otFlux || cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
+            if (cfg->outputtype == otFlux || cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype) ||
+                    g.rfforward) {
                 scale = cfg->unitinmm / (float)(cfg->energytot * Vvox * cfg->tstep);
 
                 if (cfg->outputtype == otFluence || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
