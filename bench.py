@@ -496,7 +496,7 @@ def main():
             "unit": "Tops/s",
             "frac": round(achieved / VALU_PEAK_TOPS, 4),
             "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
-            "kernel": "transport_kernel<savedet,reflect,simple>",
+            "kernel": "transport_kernel<savedet,reflect,simple,full>",
             "kernel_ms": round(transport_ms, 3),
             "ops_per_photon": round(opp, 1),
             "frac_of_fp32_vector_peak": round(achieved / FP32_PEAK_TFLOPS, 4),
