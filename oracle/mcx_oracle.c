@@ -21,6 +21,18 @@
  *   skipvoid              <- skipvoid                 (src/mcx_core.cl:1350-1455)
  *   save_detected         <- finddetector/savedetphoton (src/mcx_core.cl:838-926)
  *   rng_next              <- xorshift128p_nextf       (src/mcx_core.cl:684-698)
+ *   media_word_prop       <- updateproperty, packed media formats (src/mcx_core.cl:1079-1193)
+ *   pattern_weights       <- the pattern/pattern3d pixel weights (src/mcx_core.cl:1656-1766)
+ *   save_traj             <- savedebugdata            (src/mcx_core.cl:929-944)
+ *   (in run_thread)       <- RF forward weights and deposits (src/mcx_core.cl:2750-2760, :2833-2839),
+ *                            RF/time-weighted replay deposits (:2568-2623, :2845-2858)
+ *
+ * Every function is pinned by a test that a plausible restatement mistake
+ * fails (closed forms, distributions, identities between runs, libm bounds,
+ * the reference's published absorbed fractions): tests/test_oracle_pins.py,
+ * test_sampler_pins.py, test_output_pins.py, test_replay.py,
+ * test_replay_outputs.py, test_media_formats.py, test_trajectory.py,
+ * test_rf_forward.py, test_respin.py, test_multisource.py, test_anchors.py.
  *
  * Arithmetic: every transcendental goes through the deterministic FP32
  * functions of mcx_fpmath.h (shared with the kernel's parity mode and pinned
@@ -34,7 +46,8 @@
  * host in thread order (src/mcx_host.cpp:1303-1306).
  *
  * Deliberate deviations from the reference (also applied by the kernel); each
- * removes an out-of-bounds access of the reference and changes nothing else:
+ * removes an out-of-bounds access or a value attributed to the wrong photon,
+ * gate or slot by the reference, and changes nothing else:
  *  - the deposit gate index is clamped to maxgate-1 (the reference can write
  *    one gate past the end when (t-twin0)*Rtstep rounds up to maxgate);
  *  - skipvoid returns "not found" when its 3-step entry search ends outside
@@ -50,7 +63,16 @@
  *    voxel restores mediaid = label | code (src/mcx_core.cl:2796, :2928), e.g.
  *    1|3 = 3 after a mirror face, and reads past the property table;
  *  - a photon whose time is NaN leaves the time window (the test is
- *    !(t <= tend) instead of t > tend), so no lane can loop forever.
+ *    !(t <= tend) instead of t > tend), so no lane can loop forever;
+ *  - scattering-site replay deposits clamp the gate before adding the detector
+ *    and source offsets (the reference clamps the sum, src/mcx_core.cl:2591);
+ *  - the RF replay phase is the replayed photon's own omega*tof (the reference
+ *    reads the next photon's, src/mcx_core.cl:2257-2263), and the imaginary
+ *    part of the RFmus deposit goes to the imaginary half (the reference writes
+ *    it past its buffer, :2598);
+ *  - packed media: the specular entry loss of a source outside the volume uses
+ *    the decoded n of the entry voxel (the reference indexes the label table
+ *    with the packed word, :1424-1428).
  */
 #include <float.h>
 #include <math.h>
