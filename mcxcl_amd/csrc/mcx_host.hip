@@ -351,8 +351,10 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     /* the adjoint outputs deposit fluence per source; their products are formed
        after the run (src/mcx_core.cl:2843) */
     g.outputtype = MCX_IS_ADJOINT_TYPE(cfg->outputtype) ? (uint32_t)otFluence : (uint32_t)cfg->outputtype;
-    g.adjsrc = (MCX_IS_ADJOINT_TYPE(cfg->outputtype) && cfg->seed != SEED_FROM_FILE && cfg->detdir && cfg->detnum &&
-                cfg->extrasrclen + 1 > cfg->detnum) ? cfg->extrasrclen + 1 - cfg->detnum : 0;
+    /* detectors appended as sources (adjoint outputs, srcid -2) launch from their disks */
+    g.adjsrc = ((MCX_IS_ADJOINT_TYPE(cfg->outputtype) || cfg->srcid == -2) && cfg->seed != SEED_FROM_FILE &&
+                cfg->detdir && cfg->detnum && cfg->extrasrclen + 1 > cfg->detnum) ?
+               cfg->extrasrclen + 1 - cfg->detnum : 0;
     g.savedetflag = cfg->savedetflag;
     g.reclen = reclen;
     g.partialdata = partialdata;

@@ -2084,17 +2084,13 @@ void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
         MCX_ERROR(-6, "Jacobian output is only valid in the reply mode. Please define cfg.seed");
     }
 
-    if (cfg->srcid == -2) {
-        MCX_ERROR(-6, "srcid -2 is not supported by this build");
-    }
-
-    /* adjoint outputs: every detector becomes an extra disk source at its
-       position and direction, radius detpos.w, and the field is stacked per
-       source (src/mcx_utils.c:1902-1925).  A binding does this itself after
+    /* adjoint outputs and srcid -2: every detector becomes an extra disk source
+       at its position and direction, radius detpos.w, and the field is stacked
+       per source (src/mcx_utils.c:1902-1925).  A binding does this itself after
        validation (src/pmcxcl.cpp:1179-1198), so only a stand-alone run does it
        here. */
-    if (cfg->parentid == mpStandalone && MCX_IS_ADJOINT_TYPE(cfg->outputtype) && cfg->seed != SEED_FROM_FILE &&
-            cfg->detnum > 0 && cfg->detdir != NULL) {
+    if (cfg->parentid == mpStandalone && (MCX_IS_ADJOINT_TYPE(cfg->outputtype) || cfg->srcid == -2) &&
+            cfg->seed != SEED_FROM_FILE && cfg->detnum > 0 && cfg->detdir != NULL) {
         mcx_append_detector_sources(cfg);
     }
 

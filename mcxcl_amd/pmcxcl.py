@@ -295,8 +295,9 @@ class PreparedConfig:
                 # (src/pmcxcl.cpp:1523-1525)
                 raise RuntimeError(_native.last_error())
             c0 = self.c
-            if (c0.outputtype[0] >= OT_ADJOINT and c0.seed != SEED_FROM_FILE and c0.detnum > 0 and c0.detdir):
-                # adjoint outputs: detectors become disk sources (src/pmcxcl.cpp:1179-1198)
+            if ((c0.outputtype[0] >= OT_ADJOINT or c0.srcid == -2) and c0.seed != SEED_FROM_FILE and c0.detnum > 0
+                    and c0.detdir):
+                # adjoint outputs and srcid -2: detectors become disk sources (src/pmcxcl.cpp:1179-1198)
                 lib.mcx_append_detector_sources(ctypes.byref(c0))
             if not self.c.vol or self.c.medianum == 0:
                 raise ValueError("You must define 'vol' and 'prop' field.")

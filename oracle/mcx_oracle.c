@@ -1630,8 +1630,10 @@ static void oracle_setup(Config* cfg, OParam* gp, F4** props_out, float** muamus
     gp->invcdf = cfg->invcdf;
     gp->debuglevel = cfg->debuglevel;
     gp->extrasrclen = cfg->extrasrclen;
-    gp->adjsrc = (MCX_IS_ADJOINT_TYPE(cfg->outputtype) && cfg->seed != SEED_FROM_FILE && cfg->detdir && cfg->detnum &&
-                cfg->extrasrclen + 1 > cfg->detnum) ? cfg->extrasrclen + 1 - cfg->detnum : 0;
+    /* src/mcx_core.cl:2159-2160: adjoint outputs or srcid -2 */
+    gp->adjsrc = ((MCX_IS_ADJOINT_TYPE(cfg->outputtype) || cfg->srcid == -2) && cfg->seed != SEED_FROM_FILE &&
+                  cfg->detdir && cfg->detnum && cfg->extrasrclen + 1 > cfg->detnum) ?
+                 cfg->extrasrclen + 1 - cfg->detnum : 0;
     gp->srcid = cfg->srcid;
     gp->srcdata = cfg->srcdata;
     gp->replaydet = cfg->replaydet;

@@ -134,3 +134,19 @@ def test_adjoint_outputs_device_parity(outputtype, tstep):
         scale = np.abs(ref[k]).max()
         assert scale > 0
         np.testing.assert_allclose(gpu[k], ref[k], rtol=1e-5, atol=1e-6 * scale)
+
+
+def test_srcid_minus_two_stacks_the_detector_sources():
+    """srcid -2 (src/mcx_utils.c:1902-1925, src/mcx_core.cl:2159-2160): the detectors
+    are appended as disk sources as for the adjoint outputs, srcid stays -2, and the
+    run returns one fluence per source with no adjoint products: bit for bit the
+    per-source fluence an adjoint-output run computes from the same photons"""
+    fl = _case(outputtype="fluence", srcid=-2)
+    with pmcxcl.prepare(fl) as pc:
+        assert pc.c.extrasrclen == 2 and pc.c.srcid == -2 and pc.nsrc == 3
+    a = oracle.run(dict(fl, autopilot=0, nthread=NT, nblocksize=64, rawfield=1), nthread=NT)
+    b = oracle.run(dict(_case(), autopilot=0, nthread=NT, nblocksize=64, rawfield=1), nthread=NT)
+    assert "jmua" not in a and "jmua" in b
+    assert np.array_equal(a["flux64"], b["flux64"])
+    assert a["flux"].shape == (60, 60, 60, 1, 1, 3)      # [x, y, z, gate, 1, source] (src/pmcxcl.cpp:1339-1345)
+    np.testing.assert_array_equal(a["flux"], b["flux"])

@@ -160,3 +160,19 @@ def test_multisource_replay_parity(multisource_forward, outputtype):
     ref = oracle.run(cfg, nthread=NT)
     np.testing.assert_allclose(gpu["flux64"], ref["flux64"], rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(gpu["flux"], ref["flux"], rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_srcid_minus_two_device_parity():
+    """srcid -2: the detectors launch as disk sources beside the user's source, one
+    fluence per source, device equal to the oracle (bit-reproducible mode)"""
+    cfg = workloads.homogeneous_cube_workload(nphoton=30000, srcpos=[29.5, 29.5, 0], tstep=1e-9)
+    cfg.update(outputtype="fluence", srcid=-2, detpos=[[20, 29.5, 0, 2], [39, 29.5, 0, 2]],
+               detdir=[[0, 0, 1, 0], [0, 0, 1, 0]], issavedet=0, autopilot=0, nthread=2048, nblocksize=64,
+               mathmode=1, rawfield=1)
+    gpu, ref = mcxcl_amd.run(cfg), oracle.run(cfg, nthread=2048)
+    assert gpu["stat"]["energytot"] == ref["stat"]["energytot"]
+    assert gpu["stat"]["energyabs"] == ref["stat"]["energyabs"]
+    np.testing.assert_allclose(gpu["flux64"], ref["flux64"], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(gpu["flux"], ref["flux"], rtol=1e-6, atol=0)
+    assert gpu["flux"].shape[-1] == 3
