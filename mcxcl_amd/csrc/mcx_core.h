@@ -119,10 +119,11 @@ hipError_t mcx_launch_fast(const MCXKParam* prm, int nblock, int nthreadblock, s
 /* RNG stream dump: field[i] for i = idx; i < fieldlen; i += nthread */
 /* resident workgroups per CU of the fast transport kernel variant for this launch shape */
 /* adjoint outputs from the normalised per-source field (kind 0: phi_s phi_d,
-   kind 1: grad phi_s . grad phi_d), scaled, optionally by a per-label factor */
-hipError_t mcx_launch_adjoint(const float* field, float* out, uint32_t nvox, uint32_t nx, uint32_t ny, uint32_t maxgate,
-                              uint32_t ns, uint32_t nd, int kind, float scale, const float* opscale,
-                              const uint32_t* media, hipStream_t stream);
+   kind 1: grad phi_s . grad phi_d), scaled, optionally by a per-label factor;
+   complex when field_im is given (RF), the imaginary part into out_im */
+hipError_t mcx_launch_adjoint(const float* field, const float* field_im, float* out, float* out_im, uint32_t nvox,
+                              uint32_t nx, uint32_t ny, uint32_t maxgate, uint32_t ns, uint32_t nd, int kind,
+                              float scale, const float* opscale, const uint32_t* media, hipStream_t stream);
 int mcx_fast_occupancy(int nthreadblock, size_t shmem, int savedet, int reflect, int simple);
 hipError_t mcx_launch_atomic_probe(double* grid, uint32_t n, uint32_t nblock, uint32_t k, hipStream_t s);
 hipError_t mcx_launch_rngdump(const MCXKParam* prm, int nblock, int nthreadblock, hipStream_t stream);

@@ -137,7 +137,13 @@ __device__ __forceinline__ float fd_grad(const float* __restrict__ f, uint32_t v
     return (cw_sum(f, vox + stride, slot, maxgate, nvox) - cw_sum(f, vox - stride, slot, maxgate, nvox)) * 0.5f;
 }
 
-__global__ void adjoint_kernel(const float* __restrict__ field, float* __restrict__ out, uint32_t nvox, uint32_t nx,
+/* the adjoint outputs of one voxel for every (source, detector) pair: kind 0
+   sum_t phi_s * sum_t phi_d (src/mcx_core.cl:3466-3512), kind 1 grad phi_s .
+   grad phi_d (:3390-3464); complex for RF fields (im != NULL): re = re.re -
+   im.im, im = re.im + im.re, the imaginary part to out_im.  Both times scale
+   and the label's factor. */
+__global__ void adjoint_kernel(const float* __restrict__ field, const float* __restrict__ field_im,
+                               float* __restrict__ out, float* __restrict__ out_im, uint32_t nvox, uint32_t nx,
                                uint32_t ny, uint32_t maxgate, uint32_t ns, uint32_t nd, int kind, float scale,
                                const float* __restrict__ opscale, const uint32_t* __restrict__ media) {
     const uint32_t vox = blockIdx.x * blockDim.x + threadIdx.x;
@@ -152,43 +158,75 @@ __global__ void adjoint_kernel(const float* __restrict__ field, float* __restric
 
     for (uint32_t s = 0; s < ns; s++) {
         float sx = 0.f, sy = 0.f, sz = 0.f, sc = 0.f;
+        float sxi = 0.f, syi = 0.f, szi = 0.f, sci = 0.f;
 
         if (kind == 0) {
             sc = cw_sum(field, vox, s, maxgate, nvox);
+
+            if (field_im) {
+                sci = cw_sum(field_im, vox, s, maxgate, nvox);
+            }
         } else {
             sx = fd_grad(field, vox, s, maxgate, nvox, ix, nx, 1);
             sy = fd_grad(field, vox, s, maxgate, nvox, iy, ny, nx);
             sz = fd_grad(field, vox, s, maxgate, nvox, iz, nz, nxy);
+
+            if (field_im) {
+                sxi = fd_grad(field_im, vox, s, maxgate, nvox, ix, nx, 1);
+                syi = fd_grad(field_im, vox, s, maxgate, nvox, iy, ny, nx);
+                szi = fd_grad(field_im, vox, s, maxgate, nvox, iz, nz, nxy);
+            }
         }
 
         for (uint32_t d = 0; d < nd; d++) {
-            float v;
+            float v, vi = 0.f;
 
             if (kind == 0) {
-                v = sc * cw_sum(field, vox, ns + d, maxgate, nvox);
+                const float dc = cw_sum(field, vox, ns + d, maxgate, nvox);
+                v = sc * dc;
+
+                if (field_im) {
+                    const float dci = cw_sum(field_im, vox, ns + d, maxgate, nvox);
+                    v = v - sci * dci;
+                    vi = sc * dci + sci * dc;
+                }
             } else {
                 const float dx = fd_grad(field, vox, ns + d, maxgate, nvox, ix, nx, 1);
                 const float dy = fd_grad(field, vox, ns + d, maxgate, nvox, iy, ny, nx);
                 const float dz = fd_grad(field, vox, ns + d, maxgate, nvox, iz, nz, nxy);
                 v = sx * dx + sy * dy + sz * dz;
+
+                if (field_im) {
+                    const float dxi = fd_grad(field_im, vox, ns + d, maxgate, nvox, ix, nx, 1);
+                    const float dyi = fd_grad(field_im, vox, ns + d, maxgate, nvox, iy, ny, nx);
+                    const float dzi = fd_grad(field_im, vox, ns + d, maxgate, nvox, iz, nz, nxy);
+                    v = v - (sxi * dxi + syi * dyi + szi * dzi);
+                    vi = sx * dxi + sy * dyi + sz * dzi + sxi * dx + syi * dy + szi * dz;
+                }
             }
 
             v = v * scale;
+            vi = vi * scale;
 
             if (opscale) {
                 v = v * op;
+                vi = vi * op;
             }
 
             out[vox + (size_t)(s * nd + d) * nvox] = v;
+
+            if (out_im) {
+                out_im[vox + (size_t)(s * nd + d) * nvox] = vi;
+            }
         }
     }
 }
 
-extern "C" hipError_t mcx_launch_adjoint(const float* field, float* out, uint32_t nvox, uint32_t nx, uint32_t ny,
-        uint32_t maxgate, uint32_t ns, uint32_t nd, int kind, float scale,
+extern "C" hipError_t mcx_launch_adjoint(const float* field, const float* field_im, float* out, float* out_im,
+        uint32_t nvox, uint32_t nx, uint32_t ny, uint32_t maxgate, uint32_t ns, uint32_t nd, int kind, float scale,
         const float* opscale, const uint32_t* media, hipStream_t stream) {
-    hipLaunchKernelGGL(adjoint_kernel, dim3((nvox + 255) / 256), dim3(256), 0, stream, field, out, nvox, nx, ny, maxgate,
-                       ns, nd, kind, scale, opscale, media);
+    hipLaunchKernelGGL(adjoint_kernel, dim3((nvox + 255) / 256), dim3(256), 0, stream, field, field_im, out, out_im,
+                       nvox, nx, ny, maxgate, ns, nd, kind, scale, opscale, media);
     return hipGetLastError();
 }
 

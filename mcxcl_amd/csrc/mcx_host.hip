@@ -248,12 +248,10 @@ void mcx_check_supported(const Config* cfg) {
         MCX_ERROR(-4, "replay, per-voxel mua/mus arrays and adjoint outputs need a label volume");
     }
 
-    /* RF forward (complex photon weights): one source, real outputs after the
-       complex field only (the reference's multi-source and adjoint RF layouts
-       are not built) */
-    if (cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE &&
-            ((cfg->extrasrclen && cfg->srcid < 0) || cfg->srcnum > 1 || MCX_IS_ADJOINT_TYPE(cfg->outputtype))) {
-        MCX_ERROR(-4, "RF forward simulations (omega > 0) take one source and no adjoint outputs");
+    /* RF forward (complex photon weights) with patterns sharing photons: the
+       complex field has no per-pattern layout here */
+    if (cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE && cfg->srcnum > 1) {
+        MCX_ERROR(-4, "RF forward simulations (omega > 0) take no patterns sharing photons (srcnum > 1)");
     }
 
     if (cfg->mediabyte == MEDIA_LABEL_HALF && (cfg->savedetflag & 0xEu)) {
@@ -1005,7 +1003,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             adjdual = MCX_IS_DUAL_ADJOINT_TYPE(cfg->outputtype);
             adjmusp = cfg->outputtype == otAdjointMusp || cfg->outputtype == otAdjointMuaMusp;
             adjop = adjmusp || cfg->outputtype == otAdjointMus;
-            HIP_TRY(dalloc(bufs, &djacob, adjlen * (adjdual ? 2 : 1)));
+            /* RF forward: a real and an imaginary block per output (src/mcx_host.cpp:1474-1476) */
+            HIP_TRY(dalloc(bufs, &djacob, adjlen * (adjdual ? 2 : 1) * (g.rfforward ? 2 : 1)));
 
             if (adjop) {
                 HIP_TRY(dalloc(bufs, &dopscale, 256));
@@ -1181,15 +1180,22 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
                 HIP_TRY(hipMemcpyAsync(dopscale, hop, sizeof(hop), hipMemcpyHostToDevice, stream));
             }
 
+            /* RF forward: complex products of the real and imaginary halves; the
+               layout is [re] [im] for one output, [re1] [re2] [im1] [im2] for two
+               (src/mcx_host.cpp:1559-1568) */
+            const float* fim = g.rfforward ? dout + g.imoff : NULL;
+            const size_t imblock = g.rfforward ? adjlen * (dual ? 2 : 1) : 0;
+
             if (dual || cfg->outputtype == otAdjoint) {
-                HIP_TRY(mcx_launch_adjoint(dout, djacob, (uint32_t)nvox, cfg->dim.x, cfg->dim.y, maxgate, ns, nd, 0, -Vvox,
-                                           NULL, dmedia, stream));
+                HIP_TRY(mcx_launch_adjoint(dout, fim, djacob, fim ? djacob + imblock : NULL, (uint32_t)nvox, cfg->dim.x,
+                                           cfg->dim.y, maxgate, ns, nd, 0, -Vvox, NULL, dmedia, stream));
             }
 
             if (dual || cfg->outputtype != otAdjoint) {
-                HIP_TRY(mcx_launch_adjoint(dout, djacob + (dual ? adjlen : 0), (uint32_t)nvox, cfg->dim.x, cfg->dim.y,
-                                           maxgate, ns, nd, 1, -cfg->unitinmm, needop ? dopscale : NULL, dmedia,
-                                           stream));
+                float* o = djacob + (dual ? adjlen : 0);
+                HIP_TRY(mcx_launch_adjoint(dout, fim, o, fim ? o + imblock : NULL, (uint32_t)nvox, cfg->dim.x,
+                                           cfg->dim.y, maxgate, ns, nd, 1, -cfg->unitinmm, needop ? dopscale : NULL,
+                                           dmedia, stream));
             }
         }
 
@@ -1202,14 +1208,14 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
 
         if (wantadj) {
             free(cfg->exportjacob);
-            cfg->exportjacob = (float*)malloc(adjlen * (adjdual ? 2 : 1) * sizeof(float));
+            const size_t jlen = adjlen * (adjdual ? 2 : 1) * (g.rfforward ? 2 : 1);
+            cfg->exportjacob = (float*)malloc(jlen * sizeof(float));
 
             if (!cfg->exportjacob) {
                 HIP_TRY(hipErrorOutOfMemory);
             }
 
-            HIP_TRY(hipMemcpyAsync(cfg->exportjacob, djacob, adjlen * (adjdual ? 2 : 1) * sizeof(float),
-                                   hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipMemcpyAsync(cfg->exportjacob, djacob, jlen * sizeof(float), hipMemcpyDeviceToHost, stream));
         }
 
         if (cfg->exportfield32) {

@@ -526,10 +526,14 @@ def _collect(c, user, ngate, ndet, nsrc):
             out["dref"] = dref.reshape(dims, order="F")
             flux = fl.reshape(dims, order="F")
         if c.omega > 0 and c.seed != SEED_FROM_FILE:
-            # RF forward: complex fluence, real + 1j * imaginary half (src/pmcxcl.cpp:1387-1408)
-            flux = (flux[..., 0] + 1j * flux[..., 1]).astype(np.complex64)
-            if flux.ndim == 5 and flux.shape[4] == 1:
+            # RF forward: complex fluence, real half + 1j * imaginary half (src/pmcxcl.cpp:1387-1408);
+            # with pooled sources one complex field per source along the last axis
+            n = flux.shape[-1] // 2
+            flux = (flux[..., :n] + 1j * flux[..., n:]).astype(np.complex64)
+            if n == 1:
                 flux = flux[..., 0]
+                if flux.ndim == 5 and flux.shape[4] == 1:
+                    flux = flux[..., 0]
         out["flux"] = flux
     ot = c.outputtype[0]
     if ot >= OT_ADJOINT and c.exportjacob and c.detnum and c.extrasrclen + 1 > c.detnum:
@@ -537,9 +541,15 @@ def _collect(c, user, ngate, ndet, nsrc):
         npair = (c.extrasrclen + 1 - c.detnum) * c.detnum
         adjlen = c.dim.x * c.dim.y * c.dim.z * npair
         keys = ADJOINT_KEYS.get(ot, ())
-        buf = np.ctypeslib.as_array((ctypes.c_float * (adjlen * len(keys))).from_address(c.exportjacob)).copy()
+        # RF forward: complex64, the imaginary blocks after the real ones (src/pmcxcl.cpp:1459-1485)
+        nrf = 2 if (c.omega > 0 and c.seed != SEED_FROM_FILE) else 1
+        buf = np.ctypeslib.as_array((ctypes.c_float * (adjlen * len(keys) * nrf)).from_address(c.exportjacob)).copy()
         for k, name in enumerate(keys):
-            out[name] = buf[k * adjlen:(k + 1) * adjlen].reshape((c.dim.x, c.dim.y, c.dim.z, npair), order="F")
+            v = buf[k * adjlen:(k + 1) * adjlen]
+            if nrf == 2:
+                im = buf[(len(keys) + k) * adjlen:(len(keys) + k + 1) * adjlen]
+                v = (v + 1j * im).astype(np.complex64)
+            out[name] = v.reshape((c.dim.x, c.dim.y, c.dim.z, npair), order="F")
     out["stat"] = {
         "runtime": int(c.runtime),
         "nphoton": int(c.nphoton) * (c.respin if c.respin > 1 else 1),

@@ -145,10 +145,17 @@ def adjoint_outputs(c):
     nvox = int(np.prod(dims))
     maxgate = int((c.tend - c.tstart) / c.tstep + 0.5)
     ns, nd = c.extrasrclen + 1 - c.detnum, c.detnum
-    field = np.ctypeslib.as_array((ctypes.c_float * (nvox * maxgate * (ns + nd))).from_address(c.exportfield))
-    field = field.reshape((nvox, -1), order="F")
-    cw = [_cw_sum(field, k, maxgate) for k in range(ns + nd)]
-    grads = [[_fd_grad(v.reshape(dims, order="F"), ax).reshape(-1, order="F") for ax in range(3)] for v in cw]
+    # RF forward: the imaginary half of the field follows the real one, and every
+    # output is complex: re = re.re - im.im, im = re.im + im.re (src/mcx_core.cl:3430-3511)
+    rf = c.omega > 0 and c.seed != pmcxcl.SEED_FROM_FILE
+    half = nvox * maxgate * (ns + nd)
+    flat = np.ctypeslib.as_array((ctypes.c_float * (half * (2 if rf else 1))).from_address(c.exportfield))
+    parts = [flat[:half].reshape((nvox, -1), order="F")]
+    if rf:
+        parts.append(flat[half:].reshape((nvox, -1), order="F"))
+    cws = [[_cw_sum(f, k, maxgate) for k in range(ns + nd)] for f in parts]
+    gradss = [[[_fd_grad(v.reshape(dims, order="F"), ax).reshape(-1, order="F") for ax in range(3)] for v in cw]
+              for cw in cws]
     vvox = np.float32(c.steps.x) * np.float32(c.steps.y) * np.float32(c.steps.z)
     prop = np.ctypeslib.as_array((ctypes.c_float * (4 * c.medianum)).from_address(c.prop)).reshape(-1, 4)
     labels = np.ctypeslib.as_array((ctypes.c_uint32 * nvox).from_address(c.vol)) & 0xFF
@@ -163,17 +170,32 @@ def adjoint_outputs(c):
         return op[labels]
 
     def block(kind, scale, op=None):
-        out = np.empty((nvox, ns * nd), np.float32)
+        out = np.empty((nvox, ns * nd), np.complex64 if rf else np.float32)
         for s in range(ns):
             for d in range(nd):
                 if kind == 0:
+                    cw = cws[0]
                     v = cw[s] * cw[ns + d]
+                    if rf:
+                        cwi = cws[1]
+                        vi = cw[s] * cwi[ns + d] + cwi[s] * cw[ns + d]
+                        v = v - cwi[s] * cwi[ns + d]
                 else:
-                    gs, gd = grads[s], grads[ns + d]
+                    gs, gd = gradss[0][s], gradss[0][ns + d]
                     v = gs[0] * gd[0] + gs[1] * gd[1] + gs[2] * gd[2]
+                    if rf:
+                        gsi, gdi = gradss[1][s], gradss[1][ns + d]
+                        vi = gs[0] * gdi[0] + gs[1] * gdi[1] + gs[2] * gdi[2] + gsi[0] * gd[0] + gsi[1] * gd[1] + \
+                            gsi[2] * gd[2]
+                        v = v - (gsi[0] * gdi[0] + gsi[1] * gdi[1] + gsi[2] * gdi[2])
                 v = v * np.float32(scale)
                 if op is not None:
                     v = v * op
+                if rf:
+                    vi = vi * np.float32(scale)
+                    if op is not None:
+                        vi = vi * op
+                    v = v + 1j * vi
                 out[:, s * nd + d] = v
         return out.reshape(dims + (ns * nd,), order="F")
 

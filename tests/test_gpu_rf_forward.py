@@ -57,3 +57,25 @@ def test_rf_forward_slab_closed_form():
     a = 0.01 + 1j * om * 1.37 * R_C0
     want = 1000 * np.exp(-a * z) * (1 - np.exp(-a)) / a
     np.testing.assert_allclose(f[29, 29, :, 0] + 1j * f[29, 29, :, 1], want, rtol=2e-4)
+
+
+@pytest.mark.parametrize("outputtype", ["adjoint_mua_d", "adjoint_mua_musp", "adjoint_mus"])
+def test_complex_adjoint_device_parity(outputtype):
+    """RF forward with the detectors as disk sources: the complex per-source field
+    and the complex adjoint products, device against the oracle (bit-reproducible
+    mode): the grid within 1e-9, the products within float32 rounding"""
+    cfg = workloads.homogeneous_cube_workload(nphoton=20000, srcpos=[29.5, 29.5, 0], tstep=5e-9,
+                                              prop=[[0, 0, 1, 1], [0.005, 1.0, 0.8, 1.37]])
+    cfg.update(outputtype=outputtype, detpos=[[20, 29.5, 0, 2], [39, 29.5, 0, 2]],
+               detdir=[[0, 0, 1, 0], [0, 0, 1, 0]], issavedet=0, omega=OMEGA, autopilot=0, nthread=NT,
+               nblocksize=64, mathmode=1, rawfield=1)
+    gpu, ref = mcxcl_amd.run(cfg), oracle.run(cfg, nthread=NT)
+    assert gpu["stat"]["energytot"] == ref["stat"]["energytot"]
+    np.testing.assert_allclose(gpu["flux64"], ref["flux64"], rtol=1e-9, atol=1e-12 * np.abs(ref["flux64"]).max())
+    keys = [k for k in ("jmua", "jd", "jmus", "jmusp") if k in ref]
+    assert keys and [k for k in ("jmua", "jd", "jmus", "jmusp") if k in gpu] == keys
+    for k in keys:
+        assert gpu[k].dtype == np.complex64
+        scale = np.abs(ref[k]).max()
+        assert scale > 0 and np.abs(ref[k].imag).max() > 0
+        np.testing.assert_allclose(gpu[k], ref[k], rtol=1e-5, atol=1e-6 * scale)

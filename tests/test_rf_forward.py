@@ -70,8 +70,56 @@ def test_complex_output_is_the_normalised_grid():
 
 
 def test_rf_forward_restrictions():
-    cfg = workloads.sphere_cube_workload(nphoton=100)
+    cfg = workloads.homogeneous_cube_workload(nphoton=100, srctype="pattern", srcpos=[10, 15, 0],
+                                              srcparam1=[40, 0, 0, 4], srcparam2=[0, 30, 0, 3], srcnum=2,
+                                              srcpattern=np.ones((2, 4, 3), np.float32))
     with pytest.raises(RuntimeError, match="RF forward"):
-        # pooled sources (srcid -1) stack one field per source: not built for complex fields
-        mcxcl_amd.run(dict(cfg, omega=1e8, srcpos=[[29, 29, 0], [20, 20, 0]], srcdir=[[0, 0, 1], [0, 0, 1]],
-                           srcid=-1))
+        # patterns sharing photons keep one slot per pattern: not built for complex fields
+        mcxcl_amd.run(dict(cfg, omega=1e8))
+
+
+def _adjoint_case(outputtype, omega, nphoton=40000):
+    cfg = workloads.homogeneous_cube_workload(nphoton=nphoton, srcpos=[29.5, 29.5, 0], tstep=5e-9)
+    cfg.update(outputtype=outputtype, detpos=[[20, 29.5, 0, 2], [39, 29.5, 0, 2]],
+               detdir=[[0, 0, 1, 0], [0, 0, 1, 0]], issavedet=0, omega=omega)
+    return cfg
+
+
+def test_pooled_sources_slices_add_to_the_pooled_field():
+    """srcid -1 with complex weights: the per-source complex fields add up to the
+    srcid 0 field of the same photons (the source is drawn per photon either way)"""
+    base = workloads.homogeneous_cube_workload(nphoton=20000, tstep=5e-9, srcpos=[[29, 29, 0], [15, 40, 0]],
+                                               srcdir=[[0, 0, 1], [0, 0, 1]], omega=2 * np.pi * 200e6)
+    pooled, split = _run(dict(base, srcid=0)), _run(dict(base, srcid=-1))
+    assert split["flux64"].shape == (60, 60, 60, 4)          # [re s1, re s2, im s1, im s2]
+    re = split["flux64"][..., 0] + split["flux64"][..., 1]
+    im = split["flux64"][..., 2] + split["flux64"][..., 3]
+    np.testing.assert_allclose(re, pooled["flux64"][..., 0], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(im, pooled["flux64"][..., 1], rtol=1e-9, atol=1e-12)
+    assert split["flux"].dtype == np.complex64 and split["flux"].shape == (60, 60, 60, 1, 1, 2)
+
+
+def test_complex_adjoint_products():
+    """RF adjoint outputs (src/mcx_core.cl:3390-3511, src/mcx_host.cpp:1467-1642): the
+    plain (not conjugated) complex products of the normalised complex fluences,
+    against a complex128 computation from the returned per-source field"""
+    out = _run(_adjoint_case("adjoint_mua_d", 2 * np.pi * 300e6))
+    flux = out["flux"][:, :, :, 0, 0, :]                      # [x, y, z, source], complex, one gate
+    assert flux.shape == (60, 60, 60, 3) and np.abs(flux.imag).max() > 0
+    phi = flux.astype(np.complex128)
+    jmua = -1.0 * phi[..., :1, None] * phi[..., None, 1:]     # Vvox = 1
+    jmua = jmua.reshape(60, 60, 60, 2)
+    assert out["jmua"].dtype == np.complex64
+    np.testing.assert_allclose(out["jmua"], jmua, rtol=1e-4, atol=1e-6 * np.abs(jmua).max())
+    grad = lambda f: np.stack(np.gradient(f, edge_order=2), axis=-1)
+    gs = grad(phi[..., 0])
+    for d in range(2):
+        jd = -1.0 * (gs * grad(phi[..., 1 + d])).sum(-1)
+        np.testing.assert_allclose(out["jd"][..., d], jd, rtol=2e-3, atol=1e-4 * np.abs(jd).max())
+
+
+def test_complex_adjoint_zero_frequency_limit():
+    cw = _run(_adjoint_case("adjoint", 0.0))
+    rf = _run(_adjoint_case("adjoint", 1e-3))
+    np.testing.assert_allclose(rf["jmua"].real, cw["jmua"], rtol=1e-4, atol=1e-7 * np.abs(cw["jmua"]).max())
+    assert np.abs(rf["jmua"].imag).max() <= 1e-8 * np.abs(cw["jmua"]).max()
