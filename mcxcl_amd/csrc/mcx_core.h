@@ -73,6 +73,8 @@ struct MCXKParam {
                                      packed media words or trajectory records */
     uint32_t savetraj;            /* 1: photon positions are recorded at launch, scattering and termination */
     uint32_t maxjumpdebug;        /* trajectory records the buffer holds */
+    uint32_t svmc;                /* 1: split-voxel media, two words per voxel (the second block of
+                                     nvox words holds each voxel's plane) */
     unsigned char bc[12];
 
     const uint32_t* media;        /* label volume, x fastest; bit 31 marks detector-adjacent voxels */

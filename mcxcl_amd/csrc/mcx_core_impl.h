@@ -296,6 +296,11 @@ struct Photon {
     uint32_t idx1d, mediaid;
     float w0, Lmove;
     uint64_t seed0, seed1;       /* RNG state at launch (for seed saving) */
+    /* split voxel (svmc, src/mcx_core.cl:594-598): the interface plane n.x = pd of
+       the current voxel, its normal pointing away from the photon's side, and
+       lower label | upper label << 8 | split << 16 | on the upper side << 17 */
+    float snx, sny, snz, spd;
+    uint32_t sv;
 };
 
 struct Shared {
@@ -440,6 +445,150 @@ __device__ __forceinline__ float4 media_word_prop(uint32_t format, const float4*
     return p;
 }
 
+/* ---- split-voxel media (svmc, src/mcx_core.cl:1229-1348) ---- */
+#define K_SV_LOWER(sv)   ((sv) & 0xFFu)
+#define K_SV_UPPER(sv)   (((sv) >> 8) & 0xFFu)
+#define K_SV_ISSPLIT(sv) (((sv) >> 16) & 1u)
+#define K_SV_ISUPPER(sv) (((sv) >> 17) & 1u)
+#define K_SV_CURLABEL(sv) (K_SV_ISUPPER(sv) ? K_SV_UPPER(sv) : K_SV_LOWER(sv))
+enum {kMediaSvmc = 97};
+
+/* the properties of a split voxel at position p (its integer corner fx, fy, fz):
+   the second volume word holds the plane normal (three bytes, 2c/255 - 1 each)
+   and the reference point's z, the first word the reference point's y and x and
+   the upper and lower labels; a voxel whose upper label is 0 is not split.  The
+   side is the sign of p.n - pd; on the upper side the normal is turned to face
+   the lower one.  Outside the grid: the background. */
+__device__ __forceinline__ float4 svmc_prop(const MCXKParam& g, const float4* t, uint32_t mediaid, uint32_t idx1d,
+        const float4& p, int fx, int fy, int fz, Photon& s) {
+    if (idx1d == K_OUT_MIN || idx1d == K_OUT_MAX) {
+        return t[0];
+    }
+
+    const uint32_t lo = g.media[idx1d + g.nvox];           /* bytes c0..c3: nz, ny, nx, pz */
+    const uint32_t hi = mediaid & K_MED_MASK;                /* bytes c4..c7: py, px, upper, lower */
+    const uint32_t c6 = (hi >> 16) & 0xFFu, c7 = hi >> 24;
+    uint32_t sv = c7 | (c6 << 8);
+    float4 prop;
+
+    if (c6) {
+        const float rx = (float)((hi >> 8) & 0xFFu) * (1.f / 255.f) + (float)fx;
+        const float ry = (float)(hi & 0xFFu) * (1.f / 255.f) + (float)fy;
+        const float rz = (float)(lo >> 24) * (1.f / 255.f) + (float)fz;
+        float nx = (float)((lo >> 16) & 0xFFu) * (2.f / 255.f) - 1.f;
+        float ny = (float)((lo >> 8) & 0xFFu) * (2.f / 255.f) - 1.f;
+        float nz = (float)(lo & 0xFFu) * (2.f / 255.f) - 1.f;
+        const float rn = krsqrt(nx * nx + ny * ny + nz * nz);
+        nx = nx * rn;
+        ny = ny * rn;
+        nz = nz * rn;
+        float pd = rx * nx + ry * ny + rz * nz;
+
+        if (p.x * nx + p.y * ny + p.z * nz > pd) {
+            prop = t[K_SV_UPPER(sv)];
+            sv |= 1u << 17;
+            nx = -nx;
+            ny = -ny;
+            nz = -nz;
+            pd = -pd;
+        } else {
+            prop = t[K_SV_LOWER(sv)];
+        }
+
+        sv |= 1u << 16;
+        s.snx = nx;
+        s.sny = ny;
+        s.snz = nz;
+        s.spd = pd;
+    } else {
+        prop = t[c7];
+    }
+
+    s.sv = sv;
+    return prop;
+}
+
+/* src/mcx_core.cl:1280-1299: where the step of length len crosses the voxel's
+   plane towards the other side, shorten it to the plane and recompute the
+   scattering length it uses; returns whether it does */
+__device__ __forceinline__ bool svmc_ray_plane(const Photon& s, const float4& v, float gfac, float* len, float* slen) {
+    const float vdotn = v.x * s.snx + v.y * s.sny + v.z * s.snz;
+
+    if (vdotn <= 0.f) {
+        return false;
+    }
+
+    const float d0 = s.p.x * s.snx + s.p.y * s.sny + s.p.z * s.snz - s.spd;
+    const float d1 = d0 + (*len) * vdotn;
+
+    if (d0 * d1 > 0.f) {
+        return false;
+    }
+
+    const float len0 = kdiv((*len) * d0, d0 - d1);
+    *len = (len0 > 0.f) ? len0 : *len;
+    *slen = (*len) * s.prop.y * gfac;
+    return true;
+}
+
+/* src/mcx_core.cl:1300-1345: Fresnel reflection or refraction at the plane
+   inside a split voxel, n1 the side the photon leaves; returns 1 when it
+   refracts into label 0 (the photon leaves the medium) */
+__device__ __forceinline__ int svmc_reflect(float n1, float4& v, Photon& s, const float4* t, Rng& rng) {
+    float cx = v.x, cy = v.y, cz = v.z;
+    const float icos = fabsf(cx * s.snx + cy * s.sny + cz * s.snz);
+    const float n2 = K_SV_ISUPPER(s.sv) ? t[K_SV_UPPER(s.sv)].w : t[K_SV_LOWER(s.sv)].w;
+    const float tmp0 = n1 * n1, tmp1 = n2 * n2;
+    float tmp2 = 1.f - kdiv(tmp0, tmp1) * (1.f - icos * icos);
+
+    if (tmp2 > 0.f) {
+        float re = tmp0 * icos * icos + tmp1 * tmp2;
+        tmp2 = ksqrt(tmp2);
+        const float im = 2.f * n1 * n2 * icos * tmp2;
+        float rtotal = kdiv(re - im, re + im);
+        re = tmp1 * icos * icos + tmp0 * tmp2 * tmp2;
+        rtotal = (rtotal + kdiv(re - im, re + im)) * 0.5f;
+
+        if (rng.next() <= rtotal) {
+            const float k = -2.f * icos;
+            cx += k * s.snx;
+            cy += k * s.sny;
+            cz += k * s.snz;
+            s.sv ^= 1u << 17;
+        } else {
+            const float k = -icos, r = kdiv(n1, n2);
+            cx += k * s.snx;
+            cy += k * s.sny;
+            cz += k * s.snz;
+            cx = tmp2 * s.snx + r * cx;
+            cy = tmp2 * s.sny + r * cy;
+            cz = tmp2 * s.snz + r * cz;
+            s.snx = -s.snx;
+            s.sny = -s.sny;
+            s.snz = -s.snz;
+            s.spd = -s.spd;
+
+            if (K_SV_CURLABEL(s.sv) == 0) {
+                return 1;
+            }
+
+            s.prop = t[K_SV_CURLABEL(s.sv)];
+        }
+    } else {
+        const float k = -2.f * icos;
+        cx += k * s.snx;
+        cy += k * s.sny;
+        cz += k * s.snz;
+        s.sv ^= 1u << 17;
+    }
+
+    const float rn = krsqrt(cx * cx + cy * cy + cz * cz);
+    v.x = cx * rn;
+    v.y = cy * rn;
+    v.z = cz * rn;
+    return 0;
+}
+
 /* the properties of the photon's voxel: label table, per-voxel mua/mus
    replacing the label's, or (full general path) a packed media word */
 template <bool FULL = false>
@@ -522,8 +671,14 @@ static __device__ int skipvoid(const MCXKParam& g, Photon& s) {
 
                 /* n of the entry voxel (a packed media word: its decoded n; the reference
                    indexes the label table with the word, src/mcx_core.cl:1424) */
-                float nin = (FULL && g.mediaformat > 4) ? media_word_prop(g.mediaformat, g.prop, g.media[idx1d]).w :
-                            g.prop[g.media[idx1d] & K_MED_MASK].w;
+                float nin;
+
+                if (FULL && g.svmc) {
+                    nin = svmc_prop(g, g.prop, g.media[idx1d], (uint32_t)idx1d, p, s.fx, s.fy, s.fz, s).w;
+                } else {
+                    nin = (FULL && g.mediaformat > 4) ? media_word_prop(g.mediaformat, g.prop, g.media[idx1d]).w :
+                          g.prop[g.media[idx1d] & K_MED_MASK].w;
+                }
 
                 if (g.isspecular && nin != g.prop[0].w) {
                     p.w *= 1.f - reflectcoeff(v, g.prop[0].w, nin, s.fw);
@@ -718,7 +873,8 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
             }
         }
 
-        if (SAVEDET && (isdet & K_DET_MASK) == K_DET_MASK && s.mediaid == 0 && g.issaveref < 2) {
+        if (SAVEDET && (isdet & K_DET_MASK) == K_DET_MASK &&
+                (s.mediaid == 0 || (FULL && g.svmc && K_SV_CURLABEL(s.sv) == 0)) && g.issaveref < 2) {
             save_detected<SAVEDET>(g, sh, s, isdet);
         }
     }
@@ -810,6 +966,10 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
         s.idx1d = __float_as_uint(lp2.z);
         s.mediaid = __float_as_uint(lp2.w);
         s.prop = make_float4(lpos.x, lpos.y, lpos.z, 0.f);
+
+        if (FULL && g.svmc) {
+            s.sv = 0;
+        }
 
         if (!SIMPLE) {
             const int st = (int)g.srctype;
@@ -1160,7 +1320,8 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
     } while ((s.mediaid & K_MED_MASK) == 0 || fabsf(p.w) <= g.minenergy);
 
     f.w += 1.f;
-    s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
+    s.prop = (FULL && g.svmc) ? svmc_prop(g, sh.prop, s.mediaid, s.idx1d, p, s.fx, s.fy, s.fz, s) :
+             getprop<FULL>(g, sh, s.mediaid, s.idx1d);
 
     if (FULL && g.savetraj) {
         save_traj(g, p, traj_index(g, idx, f.w), ppl[3]);   /* the source-id slot */
@@ -1218,6 +1379,12 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
     struct {
         float wre, wim, w0re, w0im;
     } rf = {0.f, 0.f, 0.f, 0.f};
+    /* split voxels: test the plane on this step (cleared once the photon crossed
+       it, set by a scattering event, a voxel change or some relaunches), and
+       whether the step ended on it (src/mcx_core.cl:2342-2343) */
+    bool testint = false, hitintf = false;
+    s.sv = 0;
+    s.snx = s.sny = s.snz = s.spd = 0.f;
 
     for (;;) {
         if (relaunch) {
@@ -1280,15 +1447,18 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
                     }
                 }
 
-                if (SAVEDET && (s.mediaid & K_MED_MASK) > 0) {
+                /* per-medium tallies: a split voxel's current side (src/mcx_core.cl:2502-2540) */
+                const uint32_t tlab = (FULL && g.svmc) ? K_SV_CURLABEL(s.sv) : (s.mediaid & K_MED_MASK);
+
+                if (SAVEDET && tlab > 0) {
                     if ((g.savedetflag >> 1) & 1) {
                         uint32_t* cnt = (uint32_t*)sh.pp;
-                        cnt[(s.mediaid & K_MED_MASK) - 1]++;
+                        cnt[tlab - 1]++;
                     }
 
                     if ((g.savedetflag >> 3) & 1) {
-                        sh.pp[g.maxmedia * (((g.savedetflag >> 1) & 1) + ((g.savedetflag >> 2) & 1)) +
-                                         (s.mediaid & K_MED_MASK) - 1] += 1.f - ctheta;
+                        sh.pp[g.maxmedia * (((g.savedetflag >> 1) & 1) + ((g.savedetflag >> 2) & 1)) + tlab - 1] +=
+                            1.f - ctheta;
                     }
                 }
 
@@ -1341,10 +1511,17 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
             }
 
             v.w = (float)(int)v.w;
+
+            if (FULL && g.svmc) {
+                testint = true;
+            }
         }
 
         n1 = s.prop.w;
-        s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
+        /* split voxels: the side is taken again from the position at every step
+           (src/mcx_core.cl:2656-2672) */
+        s.prop = (FULL && g.svmc) ? svmc_prop(g, sh.prop, s.mediaid, s.idx1d, p, s.fx, s.fy, s.fz, s) :
+                 getprop<FULL>(g, sh, s.mediaid, s.idx1d);
         int face;
         f.z = hitgrid(p, v, s.fx, s.fy, s.fz, &face);
         s.fw = face;
@@ -1352,6 +1529,17 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
         float slen = f.z * s.prop.y * gfac;
         slen = fminf(slen, f.x);
         f.z = kdiv(slen, s.prop.y * gfac);
+
+        if (FULL && g.svmc) {
+            /* the step ends on the voxel's plane when it crosses it (src/mcx_core.cl:2684-2701) */
+            hitintf = false;
+
+            if (K_SV_ISSPLIT(s.sv) && testint) {
+                float len = f.z;
+                hitintf = svmc_ray_plane(s, v, gfac, &len, &slen);
+                f.z = len;
+            }
+        }
         if (!keepold) {
             ox = s.fx;
             oy = s.fy;
@@ -1364,16 +1552,18 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
         p.y = p.y + f.z * v.y;
         p.z = p.z + f.z * v.z;
 
+        const bool stay = (slen == f.x) || (FULL && hitintf);
+
         if (face == 0) {
-            s.fx += (slen == f.x) ? 0 : (v.x > 0.f ? 1 : -1);
+            s.fx += stay ? 0 : (v.x > 0.f ? 1 : -1);
         }
 
         if (face == 1) {
-            s.fy += (slen == f.x) ? 0 : (v.y > 0.f ? 1 : -1);
+            s.fy += stay ? 0 : (v.y > 0.f ? 1 : -1);
         }
 
         if (face == 2) {
-            s.fz += (slen == f.x) ? 0 : (v.z > 0.f ? 1 : -1);
+            s.fz += stay ? 0 : (v.z > 0.f ? 1 : -1);
         }
 
         if (FULL && g.rfforward) {
@@ -1394,8 +1584,12 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
         f.x -= slen;
         f.y += f.z * s.prop.w * g.oneoverc0;
 
-        if (SAVEDET && ((g.savedetflag >> 2) & 1) && (s.mediaid & K_MED_MASK) > 0) {
-            sh.pp[g.maxmedia * ((g.savedetflag >> 1) & 1) + (s.mediaid & K_MED_MASK) - 1] += f.z;
+        if (SAVEDET && ((g.savedetflag >> 2) & 1)) {
+            const uint32_t plab = (FULL && g.svmc) ? K_SV_CURLABEL(s.sv) : (s.mediaid & K_MED_MASK);
+
+            if (plab > 0) {
+                sh.pp[g.maxmedia * ((g.savedetflag >> 1) & 1) + plab - 1] += f.z;
+            }
         }
 
         mediaidold = s.mediaid | isdet;
@@ -1413,7 +1607,7 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
             s.mediaid &= K_MED_MASK;
         }
 
-        if (s.idx1d != idx1dold && idx1dold < g.nvox && mediaidold) {
+        if ((s.idx1d != idx1dold || (FULL && hitintf)) && idx1dold < g.nvox && mediaidold) {
             if (g.save2pt && f.y >= g.twin0 && f.y < g.twin1) {
                 float weight = 0.f, weight_im = 0.f;
                 int tshift = gate_of(g, f.y);
@@ -1492,8 +1686,29 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
             s.mediaid = mediaidold;
         }
 
+        if (FULL && g.svmc) {
+            /* split voxels (src/mcx_core.cl:2937-2949): a new voxel brings its own
+               plane and side; a plane crossing moves the photon to the other side */
+            if (s.idx1d != idx1dold) {
+                s.prop = svmc_prop(g, sh.prop, s.mediaid, s.idx1d, p, s.fx, s.fy, s.fz, s);
+                testint = true;
+            } else if (hitintf) {
+                s.snx = -s.snx;
+                s.sny = -s.sny;
+                s.snz = -s.snz;
+                s.spd = -s.spd;
+                s.sv ^= 1u << 17;
+                testint = false;
+            }
+        }
+
+        /* split voxels also end the photon on entering label 0 through a voxel face
+           or the plane, unless it can reflect there (src/mcx_core.cl:2959-2966) */
+        const bool svexit = FULL && g.svmc && (s.idx1d != idx1dold || hitintf) && !K_SV_ISUPPER(s.sv) &&
+                            !K_SV_LOWER(s.sv) && (!g.doreflect || n1 == sh.prop[0].w);
+
         if ((s.mediaid == 0 && ((isdet & 0xF) == kAbsorb || (isdet & 0xF) == kCyclic ||
-                                ((isdet & 0xF) == kReflect && n1 == sh.prop[0].w))) || !(f.y <= g.twin1)) {
+                                ((isdet & 0xF) == kReflect && n1 == sh.prop[0].w))) || svexit || !(f.y <= g.twin1)) {
             if (isdet == kCyclic) {
                 if (face == 0) {
                     p.x = nextafter_ref(rintf((s.idx1d == K_OUT_MIN) ? g.maxidx_x : 0.f), (v.x > 0.f) - (v.x < 0.f));
@@ -1522,6 +1737,7 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
             launchflag = (((s.idx1d == K_OUT_MAX && g.bc[9 + face]) || (s.idx1d == K_OUT_MIN && g.bc[6 + face])) ?
                           K_OUT_MIN : (mediaidold & K_DET_MASK));
             relaunch = true;
+            testint = testint || (FULL && g.svmc);
             continue;
         }
 
@@ -1543,9 +1759,36 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
         }
 
         if (REFLECT) {
-            s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
+            bool atface = true;
 
-            if (((s.mediaid && g.doreflect) ||
+            if (FULL && g.svmc) {
+                /* split voxels keep the properties of the side taken above; at the
+                   plane, Fresnel reflection or refraction between the two labels
+                   (src/mcx_core.cl:3072-3110) replaces the voxel-face test */
+                if (hitintf) {
+                    atface = false;
+
+                    if (sh.prop[K_SV_LOWER(s.sv)].w != sh.prop[K_SV_UPPER(s.sv)].w) {
+                        s.snx = -s.snx;
+                        s.sny = -s.sny;
+                        s.snz = -s.snz;
+                        s.spd = -s.spd;
+
+                        if (svmc_reflect(n1, v, s, sh.prop, rng)) {
+                            launchflag = mediaidold & K_DET_MASK;
+                            relaunch = true;
+                            testint = true;
+                            continue;
+                        }
+                    } else {
+                        s.prop = sh.prop[K_SV_CURLABEL(s.sv)];
+                    }
+                }
+            } else {
+                s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
+            }
+
+            if (atface && ((s.mediaid && g.doreflect) ||
                     (s.mediaid == 0 && (((isdet & 0xF) == 0 && g.doreflect) || ((isdet & 0xF) == kReflect ||
                                         (isdet & 0xF) == kMirror)))) &&
                     (((isdet & 0xF) == kMirror) || n1 != s.prop.w)) {
@@ -1594,7 +1837,21 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
                     /* back in the voxel it came from, whose label mediaidold holds
                        (the reference reloads it from the volume) */
                     s.mediaid = mediaidold & K_MED_MASK;
-                    s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
+
+                    if (FULL && g.svmc) {
+                        /* the side of the plane the reflected photon lands on; label 0
+                           there ends it (src/mcx_core.cl:3221-3243) */
+                        s.prop = svmc_prop(g, sh.prop, s.mediaid, s.idx1d, p, s.fx, s.fy, s.fz, s);
+
+                        if (K_SV_CURLABEL(s.sv) == 0) {
+                            launchflag = mediaidold & K_DET_MASK;
+                            relaunch = true;
+                            continue;
+                        }
+                    } else {
+                        s.prop = getprop<FULL>(g, sh, s.mediaid, s.idx1d);
+                    }
+
                     n1 = s.prop.w;
                     /* drop the boundary code so a later same-voxel step restores
                        the voxel's own label (the reference ORs the code in) */

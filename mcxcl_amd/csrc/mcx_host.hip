@@ -237,10 +237,11 @@ static hipError_t dalloc(DevBuf& bufs, T** p, size_t n) {
 /* features the transport kernels do not implement stop here, before any
    device work, instead of running with different semantics */
 void mcx_check_supported(const Config* cfg) {
-    /* packed media words: the property formats of src/mcx_core.cl:1079-1193;
-       asgn_float, svmc and mixlabel are not built */
-    if (cfg->mediabyte > 4 && (cfg->mediabyte < MEDIA_LABEL_HALF || cfg->mediabyte > MEDIA_AS_SHORT)) {
-        MCX_ERROR(-4, "unsupported media format for the GPU kernel (label byte/short/integer, labelplus, "
+    /* packed media words (the property formats of src/mcx_core.cl:1079-1193) and
+       split voxels (svmc, :1229-1348); asgn_float and mixlabel are not built */
+    if (cfg->mediabyte > 4 && cfg->mediabyte != MEDIA_2LABEL_SPLIT &&
+            (cfg->mediabyte < MEDIA_LABEL_HALF || cfg->mediabyte > MEDIA_AS_SHORT)) {
+        MCX_ERROR(-4, "unsupported media format for the GPU kernel (label byte/short/integer, svmc, labelplus, "
                   "muamus_float, mua_float, muamus_half, asgn_byte and muamus_short are supported)");
     }
 
@@ -419,6 +420,7 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     /* trajectory records (src/mcx_core.cl:929-944) */
     g.savetraj = (cfg->debuglevel & MCX_DEBUG_MOVE) ? 1u : 0u;
     g.maxjumpdebug = cfg->maxjumpdebug;
+    g.svmc = (cfg->mediabyte == MEDIA_2LABEL_SPLIT) ? 1u : 0u;
 
     /* reflection block rule (src/mcx_host.cpp:945-956) */
     {
@@ -860,8 +862,10 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     HIP_TRY(hipEventCreate(&ev1));
     HIP_TRY(hipEventCreate(&ev2));
 
-    HIP_TRY(dalloc(bufs, &dmedia, nvox));
-    HIP_TRY(hipMemcpyAsync(dmedia, cfg->vol, nvox * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    /* split voxels: the label words, then one plane word per voxel */
+    HIP_TRY(dalloc(bufs, &dmedia, nvox * (cfg->mediabyte == MEDIA_2LABEL_SPLIT ? 2 : 1)));
+    HIP_TRY(hipMemcpyAsync(dmedia, cfg->vol, nvox * (cfg->mediabyte == MEDIA_2LABEL_SPLIT ? 2 : 1) * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, stream));
     HIP_TRY(dalloc(bufs, &dprop, cfg->medianum));
     HIP_TRY(hipMemcpyAsync(dprop, cfg->prop, cfg->medianum * sizeof(float4), hipMemcpyHostToDevice, stream));
 

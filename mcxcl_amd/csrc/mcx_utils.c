@@ -287,6 +287,45 @@ void mcx_convertrow2col(unsigned int** vol, MCXUInt4* dim) {
     *vol = out;
 }
 
+/* the same for volumes of 8 bytes per voxel (svmc and muamus_float as read,
+   reference src/mcx_utils.c:1606-1608) */
+static void mcx_convertrow2col64(unsigned int** vol, MCXUInt4* dim) {
+    size_t nx = dim->x, ny = dim->y, nz = dim->z;
+    unsigned long long* in = (unsigned long long*)(*vol);
+    unsigned long long* out = (unsigned long long*)malloc(nx * ny * nz * sizeof(unsigned long long));
+
+    for (size_t i = 0; i < nx; i++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t k = 0; k < nz; k++) {
+                out[k * nx * ny + j * nx + i] = in[i * ny * nz + j * nz + k];
+            }
+
+    free(*vol);
+    *vol = (unsigned int*)out;
+}
+
+/* split-voxel volumes (svmc): 8 bytes per voxel as read, {lower label, upper
+   label, reference point x, y, z, normal x, y, z}, become two blocks of nvox
+   words: the first holds py | px << 8 | upper << 16 | lower << 24 (bit 31 is
+   also the detector flag), the second nz | ny << 8 | nx << 16 | pz << 24
+   (reference src/mcx_utils.c:1688-1712) */
+static void mcx_split_svmc(Config* cfg) {
+    const size_t n = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
+    const unsigned char* val = (const unsigned char*)cfg->vol;
+    unsigned int* out = (unsigned int*)malloc(n * 2 * sizeof(unsigned int));
+
+    for (size_t i = 0; i < n; i++) {
+        const unsigned char* b = val + (i << 3);
+        out[i] = (unsigned int)b[3] | ((unsigned int)b[2] << 8) | ((unsigned int)b[1] << 16) |
+                 ((unsigned int)b[0] << 24);
+        out[i + n] = (unsigned int)b[7] | ((unsigned int)b[6] << 8) | ((unsigned int)b[5] << 16) |
+                     ((unsigned int)b[4] << 24);
+    }
+
+    free(cfg->vol);
+    cfg->vol = out;
+}
+
 /* ---------------- packed media words ---------------- */
 
 /* float32 bits to the binary16 pattern the reference's packers write: the
@@ -1392,6 +1431,16 @@ void mcx_maskdet(Config* cfg) {
 
                     size_t p = ((size_t)((int)(iz + 1.f)) * py + (int)(iy + 1.f)) * px + (int)(ix + 1.f);
 
+                    if (cfg->mediabyte == MEDIA_2LABEL_SPLIT) {
+                        /* src/mcx_utils.c:4157-4166: a split voxel whose lower side is background */
+                        if (!(pad[p] >> 24) && ((pad[p] >> 16) & 0xFFu)) {
+                            cfg->vol[((size_t)(int)iz * ny + (int)iy) * nx + (int)ix] |= DET_MASK;
+                            count++;
+                        }
+
+                        continue;
+                    }
+
                     if (!pad[p]) {
                         continue;
                     }
@@ -1511,10 +1560,11 @@ void mcx_preprocess(Config* cfg) {
 
     if (cfg->isrowmajor) {
         if (cfg->mediabyte == MEDIA_2LABEL_SPLIT || cfg->mediabyte == MEDIA_MUA_MUS_FLOAT) {
-            MCX_ERROR(-4, "row-major input is not supported for two-word media formats");
+            mcx_convertrow2col64(&cfg->vol, &cfg->dim);
+        } else {
+            mcx_convertrow2col(&cfg->vol, &cfg->dim);
         }
 
-        mcx_convertrow2col(&cfg->vol, &cfg->dim);
         cfg->isrowmajor = 0;
     }
 
@@ -1597,11 +1647,24 @@ void mcx_preprocess(Config* cfg) {
                     MCX_ERROR(-4, "input media optical properties are less than the labels in the volume");
                 }
             }
+        } else if (cfg->mediabyte == MEDIA_2LABEL_SPLIT) {
+            /* both labels index the media table, and the lower one must leave bit 31
+               (the detector flag) clear: the kernel masks it off (the reference reads
+               past the table for larger labels) */
+            const unsigned char* b = (const unsigned char*)cfg->vol;
+
+            for (size_t i = 0; i < n; i++) {
+                if (b[i << 3] >= cfg->medianum || b[(i << 3) + 1] >= cfg->medianum || b[i << 3] >= 128) {
+                    MCX_ERROR(-4, "svmc labels must be below the number of media (and the lower label below 128)");
+                }
+            }
+
+            mcx_split_svmc(cfg);
         } else if (cfg->mediabyte != MEDIA_MUA_MUS_FLOAT && cfg->mediabyte != MEDIA_MUA_FLOAT &&
                    cfg->mediabyte != MEDIA_AS_F2H && cfg->mediabyte != MEDIA_AS_HALF &&
                    cfg->mediabyte != MEDIA_ASGN_BYTE && cfg->mediabyte != MEDIA_AS_SHORT) {
-            MCX_ERROR(-4, "this build supports the label (byte/short/integer), labelplus, muamus_float, mua_float, "
-                      "muamus_half, asgn_byte, muamus_short and muamus_f32 media formats");
+            MCX_ERROR(-4, "this build supports the label (byte/short/integer), svmc, labelplus, muamus_float, "
+                      "mua_float, muamus_half, asgn_byte, muamus_short and muamus_f32 media formats");
         }
     }
 
@@ -1665,9 +1728,10 @@ void mcx_preprocess(Config* cfg) {
         cfg->srcparam2.w = 0.f;
     }
 
-    /* one word per voxel: the detector mask marks non-zero words (src/mcx_utils.c:1751-1753) */
-    if (cfg->issavedet && (cfg->mediabyte <= 4 || (cfg->mediabyte >= MEDIA_LABEL_HALF &&
-                           cfg->mediabyte <= MEDIA_AS_SHORT))) {
+    /* one word per voxel: the detector mask marks non-zero words; split voxels:
+       those holding background below a labelled upper side (src/mcx_utils.c:1751-1753) */
+    if (cfg->issavedet && (cfg->mediabyte <= 4 || cfg->mediabyte == MEDIA_2LABEL_SPLIT ||
+                           (cfg->mediabyte >= MEDIA_LABEL_HALF && cfg->mediabyte <= MEDIA_AS_SHORT))) {
         mcx_maskdet(cfg);
     }
 

@@ -56,6 +56,7 @@ def _carray(arr, dtype):
 
 # packed media formats (include/mcx_utils.h)
 MEDIA_LABEL_HALF, MEDIA_AS_F2H, MEDIA_MUA_FLOAT, MEDIA_ASGN_BYTE, MEDIA_AS_SHORT = 99, 100, 101, 103, 104
+MEDIA_2LABEL_SPLIT = 97
 MCX_DEBUG_MOVE = 2
 
 
@@ -78,8 +79,10 @@ def _parse_volume(user, c):
     (mua_float), float32 [2,...] {mua, mus} as two halves (muamus_float),
     float32 [3,...] {value, component, label} (labelplus), int8 [4,...] four
     mixing bytes (asgn_byte), int16 [2,...] two mixing shorts (muamus_short).
-    Per-voxel float32 {mua, mus} without half rounding go in 'muamus' beside a
-    label volume."""
+    uint8/int8 [8,...] are split voxels (svmc): {lower label, upper label,
+    reference point x, y, z, normal x, y, z} per voxel, passed as read (two
+    words per voxel) and laid out by the library.  Per-voxel float32 {mua, mus}
+    without half rounding go in 'muamus' beside a label volume."""
     if "vol" not in user:
         raise ValueError("Configuration must specify a 2/3/4D volume.")
     vol = np.asarray(user["vol"])
@@ -91,12 +94,15 @@ def _parse_volume(user, c):
         elif vol.dtype == np.int8 and k == 4:
             c.mediabyte = MEDIA_ASGN_BYTE
             words = np.ascontiguousarray(np.asfortranarray(vol).ravel(order="F")).view(np.uint32)
+        elif vol.dtype in (np.int8, np.uint8) and k == 8:
+            c.mediabyte = MEDIA_2LABEL_SPLIT
+            words = np.ascontiguousarray(np.asfortranarray(vol).ravel(order="F")).view(np.uint32)
         elif vol.dtype == np.int16 and k == 2:
             c.mediabyte = MEDIA_AS_SHORT
             words = np.ascontiguousarray(np.asfortranarray(vol).ravel(order="F")).view(np.uint32)
         else:
-            raise ValueError("4-D volumes must be float32 [1|2|3, Nx, Ny, Nz], int8 [4, Nx, Ny, Nz] or int16 "
-                             "[2, Nx, Ny, Nz] (the svmc and mixlabel formats are not supported)")
+            raise ValueError("4-D volumes must be float32 [1|2|3, Nx, Ny, Nz], int8 [4, Nx, Ny, Nz], uint8/int8 "
+                             "[8, Nx, Ny, Nz] or int16 [2, Nx, Ny, Nz] (the mixlabel format is not supported)")
         c.dim.x, c.dim.y, c.dim.z = (int(d) for d in vol.shape[1:])
         c.vol = _carray(words.astype(np.uint32), np.uint32)
         return

@@ -24,15 +24,20 @@
  *   media_word_prop       <- updateproperty, packed media formats (src/mcx_core.cl:1079-1193)
  *   pattern_weights       <- the pattern/pattern3d pixel weights (src/mcx_core.cl:1656-1766)
  *   save_traj             <- savedebugdata            (src/mcx_core.cl:929-944)
+ *   svmc_update           <- updateproperty_svmc      (src/mcx_core.cl:1231-1278)
+ *   ray_plane             <- ray_plane_intersect      (src/mcx_core.cl:1280-1299)
+ *   reflect_svmc          <- reflectray_svmc          (src/mcx_core.cl:1302-1345)
  *   (in run_thread)       <- RF forward weights and deposits (src/mcx_core.cl:2750-2760, :2833-2839),
- *                            RF/time-weighted replay deposits (:2568-2623, :2845-2858)
+ *                            RF/time-weighted replay deposits (:2568-2623, :2845-2858),
+ *                            split-voxel hooks (:2638-2701, :2931-2966, :3069-3243)
  *
  * Every function is pinned by a test that a plausible restatement mistake
  * fails (closed forms, distributions, identities between runs, libm bounds,
  * the reference's published absorbed fractions): tests/test_oracle_pins.py,
  * test_sampler_pins.py, test_output_pins.py, test_replay.py,
  * test_replay_outputs.py, test_media_formats.py, test_trajectory.py,
- * test_rf_forward.py, test_respin.py, test_multisource.py, test_anchors.py.
+ * test_rf_forward.py, test_svmc.py, test_respin.py, test_multisource.py,
+ * test_anchors.py.
  *
  * Arithmetic: every transcendental goes through the deterministic FP32
  * functions of mcx_fpmath.h (shared with the kernel's parity mode and pinned
@@ -70,9 +75,9 @@
  *    reads the next photon's, src/mcx_core.cl:2257-2263), and the imaginary
  *    part of the RFmus deposit goes to the imaginary half (the reference writes
  *    it past its buffer, :2598);
- *  - packed media: the specular entry loss of a source outside the volume uses
- *    the decoded n of the entry voxel (the reference indexes the label table
- *    with the packed word, :1424-1428).
+ *  - packed and split-voxel media: the specular entry loss of a source outside
+ *    the volume uses the decoded n of the entry voxel (the reference indexes
+ *    the label table with the packed word, :1424-1428).
  */
 #include <float.h>
 #include <math.h>
@@ -162,6 +167,7 @@ typedef struct {
     uint32_t rfforward;        /* complex photon weights: omega > 0 without replay */
     float omega;               /* RF replay: angular modulation frequency (rad/s) */
     uint32_t savetraj, maxjumpdebug;  /* trajectory records (src/mcx_core.cl:929-944) */
+    uint32_t svmc;                    /* split-voxel media: a second block of nvox plane words */
     float* debugdata;                 /* maxjumpdebug records of 6 floats */
     uint32_t* jumpdebug;              /* records written, counted past maxjumpdebug */
 } OParam;
@@ -385,8 +391,160 @@ static inline F4 getvoxprop(const OParam* g, uint32_t mediaid, uint32_t idx1d) {
     return p;
 }
 
+/* ---------------- split-voxel media (svmc) ---------------- */
+/* MCXsp (src/mcx_core.cl:594-598): the plane n.x = pd of the photon's voxel,
+   n turned away from the photon's side, and the packed labels and flags
+   (src/mcx_core.cl:577-589) */
+typedef struct {
+    float nvx, nvy, nvz, pd;
+    uint32_t sv;
+} OSplit;
+
+#define SV_LOWER(sv)     ((sv) & 0xFFu)
+#define SV_UPPER(sv)     (((sv) >> 8) & 0xFFu)
+#define SV_ISSPLIT(sv)   (((sv) >> 16) & 1u)
+#define SV_ISUPPER(sv)   (((sv) >> 17) & 1u)
+#define SV_FLIP_ISUPPER(sv) ((sv) ^= 0x20000u)
+#define SV_CURLABEL(sv)  (SV_ISUPPER(sv) ? SV_UPPER(sv) : SV_LOWER(sv))
+
+static inline void sv_flip(OSplit* sp) {
+    sp->nvx = -sp->nvx;
+    sp->nvy = -sp->nvy;
+    sp->nvz = -sp->nvz;
+    sp->pd = -sp->pd;
+}
+
+/* updateproperty_svmc (src/mcx_core.cl:1231-1278): the properties at position p
+   of the voxel idx1d with integer corner fd.  The label word holds bytes
+   {py, px, upper, lower} from the lowest up, the plane word {nz, ny, nx, pz};
+   a voxel with upper label 0 is not split.  Outside the grid: the background,
+   and the split state is left as it was. */
+static F4 svmc_update(const OParam* g, uint32_t mediaid, uint32_t idx1d, const F4* p, const S4* fd, OSplit* sp,
+                      F4 prop) {
+    if (idx1d == OUTSIDE_VOLUME_MIN || idx1d == OUTSIDE_VOLUME_MAX) {
+        return g->prop[0];
+    }
+
+    unsigned char c[8];
+    const uint32_t lo = g->media[idx1d + g->nvox], hi = mediaid & MED_MASK;
+
+    for (int i = 0; i < 4; i++) {
+        c[i] = (unsigned char)(lo >> (8 * i));
+        c[i + 4] = (unsigned char)(hi >> (8 * i));
+    }
+
+    uint32_t svpacked = (uint32_t)c[7] | ((uint32_t)c[6] << 8);
+
+    if (c[6]) {
+        const float rpx = c[5] * (1.f / 255.f) + (float)fd->x;
+        const float rpy = c[4] * (1.f / 255.f) + (float)fd->y;
+        const float rpz = c[3] * (1.f / 255.f) + (float)fd->z;
+        sp->nvx = c[2] * (2.f / 255.f) - 1.f;
+        sp->nvy = c[1] * (2.f / 255.f) - 1.f;
+        sp->nvz = c[0] * (2.f / 255.f) - 1.f;
+        const float r = pm_rsqrt(sp->nvx * sp->nvx + sp->nvy * sp->nvy + sp->nvz * sp->nvz);
+        sp->nvx = sp->nvx * r;
+        sp->nvy = sp->nvy * r;
+        sp->nvz = sp->nvz * r;
+        sp->pd = rpx * sp->nvx + rpy * sp->nvy + rpz * sp->nvz;
+
+        if (p->x * sp->nvx + p->y * sp->nvy + p->z * sp->nvz > sp->pd) {
+            prop = g->prop[SV_UPPER(svpacked)];
+            svpacked |= 0x20000u;
+            sv_flip(sp);
+        } else {
+            prop = g->prop[SV_LOWER(svpacked)];
+        }
+
+        svpacked |= 0x10000u;
+    } else {
+        prop = g->prop[c[7]];
+        svpacked &= 0xFFFFu;
+    }
+
+    sp->sv = svpacked;
+    return prop;
+}
+
+/* ray_plane_intersect (src/mcx_core.cl:1280-1299): a step of length *len from p0
+   that crosses the plane towards the other side ends on it; *slen becomes the
+   scattering length that step uses */
+static int ray_plane(const OParam* g, const F4* p0, const F4* v, const F4* prop, float* len, float* slen,
+                     const OSplit* sp) {
+    const float vdotn = v->x * sp->nvx + v->y * sp->nvy + v->z * sp->nvz;
+
+    if (vdotn <= 0.f) {
+        return 0;
+    }
+
+    const float d0 = p0->x * sp->nvx + p0->y * sp->nvy + p0->z * sp->nvz - sp->pd;
+    const float d1 = d0 + (*len) * vdotn;
+
+    if (d0 * d1 > 0.f) {
+        return 0;
+    }
+
+    const float len0 = ((*len) * d0) / (d0 - d1);
+    *len = (len0 > 0.f) ? len0 : *len;
+    *slen = (*len) * prop->y * (v->w + 1.f > (float)g->gscatter ? (1.f - prop->z) : 1.f);
+    return 1;
+}
+
+/* reflectray_svmc (src/mcx_core.cl:1302-1345): Fresnel reflection or
+   refraction of direction c at the plane (sp->nv pointing into the side being
+   entered), n1 the index of the side being left.  Returns 1 when the photon
+   refracts into label 0. */
+static int reflect_svmc(const OParam* g, OWork* w, uint64_t t[2], float n1, float c[3], OSplit* sp, F4* prop) {
+    const float icos = pm_fabs(c[0] * sp->nvx + c[1] * sp->nvy + c[2] * sp->nvz);
+    const float n2 = SV_ISUPPER(sp->sv) ? g->prop[SV_UPPER(sp->sv)].w : g->prop[SV_LOWER(sp->sv)].w;
+    const float tmp0 = n1 * n1, tmp1 = n2 * n2;
+    float tmp2 = 1.f - tmp0 / tmp1 * (1.f - icos * icos);
+
+    if (tmp2 > 0.f) {
+        float re = tmp0 * icos * icos + tmp1 * tmp2;
+        tmp2 = pm_sqrt(tmp2);
+        const float im = 2.f * n1 * n2 * icos * tmp2;
+        float rtotal = (re - im) / (re + im);
+        re = tmp1 * icos * icos + tmp0 * tmp2 * tmp2;
+        rtotal = (rtotal + (re - im) / (re + im)) * 0.5f;
+
+        if (rng_next(t, w) <= rtotal) {
+            c[0] += (-2.f * icos) * sp->nvx;
+            c[1] += (-2.f * icos) * sp->nvy;
+            c[2] += (-2.f * icos) * sp->nvz;
+            SV_FLIP_ISUPPER(sp->sv);
+        } else {
+            const float ratio = n1 / n2;
+            c[0] += (-icos) * sp->nvx;
+            c[1] += (-icos) * sp->nvy;
+            c[2] += (-icos) * sp->nvz;
+            c[0] = tmp2 * sp->nvx + ratio * c[0];
+            c[1] = tmp2 * sp->nvy + ratio * c[1];
+            c[2] = tmp2 * sp->nvz + ratio * c[2];
+            sv_flip(sp);
+
+            if (SV_CURLABEL(sp->sv) == 0) {
+                return 1;
+            }
+
+            *prop = g->prop[SV_CURLABEL(sp->sv)];
+        }
+    } else {
+        c[0] += (-2.f * icos) * sp->nvx;
+        c[1] += (-2.f * icos) * sp->nvy;
+        c[2] += (-2.f * icos) * sp->nvz;
+        SV_FLIP_ISUPPER(sp->sv);
+    }
+
+    const float r = pm_rsqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    c[0] = c[0] * r;
+    c[1] = c[1] * r;
+    c[2] = c[2] * r;
+    return 0;
+}
+
 /* src/mcx_core.cl:1350-1455 */
-static int skipvoid(const OParam* g, F4* p, F4* v, F4* f, S4* fd) {
+static int skipvoid(const OParam* g, F4* p, F4* v, F4* f, S4* fd, OSplit* sp) {
     int count = 1, idx1d;
     fd->x = (short)floorf(p->x);
     fd->y = (short)floorf(p->y);
@@ -440,7 +598,8 @@ static int skipvoid(const OParam* g, F4* p, F4* v, F4* f, S4* fd) {
                     return -1;   /* entry search failed: the reference would read outside the volume */
                 }
 
-                F4 htime = getprop(g, g->media[idx1d]);
+                F4 htime = g->svmc ? svmc_update(g, g->media[idx1d], (uint32_t)idx1d, p, fd, sp, g->prop[0]) :
+                           getprop(g, g->media[idx1d]);
 
                 /* the entry voxel's n: a packed word's decoded n (the reference
                    indexes the label table with the word, src/mcx_core.cl:1424-1428) */
@@ -485,6 +644,7 @@ typedef struct {
     S4 fd;
     uint32_t idx1d, mediaid;
     float w0, Lmove;
+    OSplit sp;            /* the split voxel the photon is in */
     /* per-thread detected-photon scratch (src/mcx_host.cpp:486-492 layout):
        [0, partialdata)   nscat (uint bits) / partial path / momentum per medium
        [partialdata]      escaped energy tally
@@ -653,7 +813,9 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
             }
         }
 
-        if (g->savedet && (isdet & DET_MASK) == DET_MASK && s->mediaid == 0 && g->issaveref < 2) {
+        /* a split voxel's label-0 side counts as outside (src/mcx_core.cl:1556-1559) */
+        if (g->savedet && (isdet & DET_MASK) == DET_MASK &&
+                (s->mediaid == 0 || (g->svmc && SV_CURLABEL(s->sp.sv) == 0)) && g->issaveref < 2) {
             save_detected(g, w, s, isdet);
         }
     }
@@ -718,6 +880,10 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
         f->z = g->minaccumtime;
         s->idx1d = pm_asuint(lp2.z);
         s->mediaid = pm_asuint(lp2.w);
+
+        if (g->svmc) {
+            s->sp.sv = 0;    /* src/mcx_core.cl:1640-1642 */
+        }
         F4 tmp = {lsrcpos.x, lsrcpos.y, lsrcpos.z, 0.f};
         s->prop = tmp;
         int st = (int)g->srctype;
@@ -1055,7 +1221,7 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
         }
 
         if ((s->mediaid & MED_MASK) == 0) {
-            int id = skipvoid(g, p, v, f, fd);
+            int id = skipvoid(g, p, v, f, fd, &s->sp);
 
             if (id >= 0) {
                 s->idx1d = (uint32_t)id;
@@ -1074,7 +1240,8 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
     } while ((s->mediaid & MED_MASK) == 0 || pm_fabs(p->w) <= g->minenergy);
 
     f->w += 1.f;
-    s->prop = getvoxprop(g, s->mediaid, s->idx1d);
+    s->prop = g->svmc ? svmc_update(g, s->mediaid, s->idx1d, p, fd, &s->sp, s->prop) :
+              getvoxprop(g, s->mediaid, s->idx1d);
 
     if (g->savetraj) {
         /* ppath[3] of the shifted pointer: the source-id slot (src/mcx_core.cl:2247) */
@@ -1159,6 +1326,11 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
     uint32_t isdet = 0, mediaidold = 0, idx1dold;
     float pathlen = 0.f, n1;
     const uint32_t nlimit = g->threadphoton + (idx < g->oddphoton);
+    /* split voxels (src/mcx_core.cl:2339-2343): test the plane on this step, and
+       whether the step ended on it */
+    int testint = 0, hitintf = 0;
+    s->sp.sv = 0;
+    s->sp.nvx = s->sp.nvy = s->sp.nvz = s->sp.pd = 0.f;
 
     s->pp[pd].f = genergy[idx * 2];
     s->pp[pd + 1].f = genergy[idx * 2 + 1];
@@ -1198,14 +1370,17 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                 float cphi, sphi, stheta, ctheta, tmp0;
                 scatter_angles(g, w, s, &stheta, &ctheta, &sphi, &cphi);
 
-                if (g->savedet && (s->mediaid & MED_MASK) > 0) {
+                /* split voxels tally the label of the photon's side (src/mcx_core.cl:2503-2540) */
+                const uint32_t tlab = g->svmc ? SV_CURLABEL(s->sp.sv) : (s->mediaid & MED_MASK);
+
+                if (g->savedet && tlab > 0) {
                     if (SAVE_NSCAT(g->savedetflag)) {
-                        s->pp[(s->mediaid & MED_MASK) - 1].u++;
+                        s->pp[tlab - 1].u++;
                     }
 
                     if (SAVE_MOM(g->savedetflag)) {
                         s->pp[g->maxmedia * (SAVE_NSCAT(g->savedetflag) + SAVE_PPATH(g->savedetflag)) +
-                                          (s->mediaid & MED_MASK) - 1].f += 1.f - ctheta;
+                                          tlab - 1].f += 1.f - ctheta;
                     }
                 }
 
@@ -1257,30 +1432,47 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
             }
 
             v->w = (float)(int)v->w;
+
+            if (g->svmc) {
+                testint = 1;    /* src/mcx_core.cl:2638-2647 */
+            }
         }
 
         n1 = s->prop.w;
-        s->prop = getvoxprop(g, s->mediaid, s->idx1d);
+        /* a split voxel's side is taken from the position at every step (src/mcx_core.cl:2656-2672) */
+        s->prop = g->svmc ? svmc_update(g, s->mediaid, s->idx1d, p, fd, &s->sp, s->prop) :
+                  getvoxprop(g, s->mediaid, s->idx1d);
         f->z = hitgrid(p, v, fd);
         float gfac = (v->w + 1.f > (float)g->gscatter ? (1.f - s->prop.z) : 1.f);
         float slen = f->z * s->prop.y * gfac;
         slen = fminf(slen, f->x);
         f->z = slen / (s->prop.y * gfac);
+
+        if (g->svmc) {    /* src/mcx_core.cl:2684-2701 */
+            if (SV_ISSPLIT(s->sp.sv) && testint) {
+                float tmplen = f->z;
+                hitintf = ray_plane(g, p, v, &s->prop, &tmplen, &slen, &s->sp);
+                f->z = tmplen;
+            } else {
+                hitintf = 0;
+            }
+        }
+
         pathlen += f->z;
         p->x = p->x + f->z * v->x;
         p->y = p->y + f->z * v->y;
         p->z = p->z + f->z * v->z;
 
         if (fd->w == 0) {
-            fd->x += (slen == f->x) ? 0 : (v->x > 0.f ? 1 : -1);
+            fd->x += (slen == f->x || hitintf) ? 0 : (v->x > 0.f ? 1 : -1);
         }
 
         if (fd->w == 1) {
-            fd->y += (slen == f->x) ? 0 : (v->y > 0.f ? 1 : -1);
+            fd->y += (slen == f->x || hitintf) ? 0 : (v->y > 0.f ? 1 : -1);
         }
 
         if (fd->w == 2) {
-            fd->z += (slen == f->x) ? 0 : (v->z > 0.f ? 1 : -1);
+            fd->z += (slen == f->x || hitintf) ? 0 : (v->z > 0.f ? 1 : -1);
         }
 
         if (g->rfforward) {
@@ -1299,8 +1491,12 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
         f->x -= slen;
         f->y += f->z * s->prop.w * g->oneoverc0;
 
-        if (g->savedet && SAVE_PPATH(g->savedetflag) && (s->mediaid & MED_MASK) > 0) {
-            s->pp[g->maxmedia * SAVE_NSCAT(g->savedetflag) + (s->mediaid & MED_MASK) - 1].f += f->z;
+        {
+            const uint32_t plab = g->svmc ? SV_CURLABEL(s->sp.sv) : (s->mediaid & MED_MASK);
+
+            if (g->savedet && SAVE_PPATH(g->savedetflag) && plab > 0) {    /* src/mcx_core.cl:2772-2792 */
+                s->pp[g->maxmedia * SAVE_NSCAT(g->savedetflag) + plab - 1].f += f->z;
+            }
         }
 
         mediaidold = s->mediaid | isdet;
@@ -1318,7 +1514,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
             s->mediaid &= MED_MASK;
         }
 
-        if (s->idx1d != idx1dold && idx1dold < g->nvox && mediaidold) {
+        if ((s->idx1d != idx1dold || hitintf) && idx1dold < g->nvox && mediaidold) {
             if (g->save2pt && f->y >= g->twin0 && f->y < g->twin1) {
                 float weight = 0.f, weight_im = 0.f;
                 int tshift = gate_of(g, f->y);
@@ -1394,8 +1590,22 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
             s->mediaid = mediaidold;
         }
 
+        if (g->svmc) {    /* src/mcx_core.cl:2931-2952 */
+            if (s->idx1d != idx1dold) {
+                s->prop = svmc_update(g, s->mediaid, s->idx1d, p, fd, &s->sp, s->prop);
+                testint = 1;
+            } else if (hitintf) {
+                sv_flip(&s->sp);
+                SV_FLIP_ISUPPER(s->sp.sv);
+                testint = 0;
+            }
+        }
+
         if ((s->mediaid == 0 && ((isdet & 0xF) == bcAbsorb || (isdet & 0xF) == bcCyclic ||
-                                 ((isdet & 0xF) == bcReflect && n1 == g->prop[0].w))) || !(f->y <= g->twin1)) {
+                                 ((isdet & 0xF) == bcReflect && n1 == g->prop[0].w))) ||
+                (g->svmc && (s->idx1d != idx1dold || hitintf) && !SV_ISUPPER(s->sp.sv) && !SV_LOWER(s->sp.sv) &&
+                 (!g->doreflect || n1 == g->prop[0].w)) ||
+                !(f->y <= g->twin1)) {
             if (isdet == bcCyclic) {
                 if (fd->w == 0) {
                     p->x = mcx_nextafterf(rintf((s->idx1d == OUTSIDE_VOLUME_MIN) ? g->maxidx.x : 0.f),
@@ -1435,6 +1645,11 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
             isdet = s->mediaid & DET_MASK;
             s->mediaid &= MED_MASK;
             RF_RESET();
+
+            if (g->svmc) {
+                testint = 1;    /* src/mcx_core.cl:3016-3024 */
+            }
+
             continue;
         }
 
@@ -1461,16 +1676,54 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
         }
 
         if (g->reflectblock) {
-            s->prop = getvoxprop(g, s->mediaid, s->idx1d);
+            int atface = 1;
 
-            if (((s->mediaid && g->doreflect) ||
+            if (g->svmc) {
+                /* the split-voxel build updates no property here (its updateproperty is
+                   empty); at the plane, reflection or refraction between the labels
+                   replaces the voxel-face test (src/mcx_core.cl:3069-3140) */
+                if (hitintf) {
+                    atface = 0;
+
+                    if (g->prop[SV_LOWER(s->sp.sv)].w != g->prop[SV_UPPER(s->sp.sv)].w) {
+                        float c0[3] = {v->x, v->y, v->z};
+                        sv_flip(&s->sp);
+
+                        if (reflect_svmc(g, w, s->t, n1, c0, &s->sp, &s->prop)) {
+                            if (launch_photon(g, w, s, idx, mediaidold & DET_MASK)) {
+                                break;
+                            }
+
+                            isdet = s->mediaid & DET_MASK;
+                            s->mediaid &= MED_MASK;
+                            RF_RESET();
+                            testint = 1;
+                            continue;
+                        }
+
+                        v->x = c0[0];
+                        v->y = c0[1];
+                        v->z = c0[2];
+                    } else {
+                        s->prop = g->prop[SV_CURLABEL(s->sp.sv)];
+                    }
+                }
+            } else {
+                s->prop = getvoxprop(g, s->mediaid, s->idx1d);
+            }
+
+            if (atface && ((s->mediaid && g->doreflect) ||
                     (s->mediaid == 0 && (((isdet & 0xF) == bcUnknown && g->doreflect) || ((isdet & 0xF) == bcReflect ||
                                          (isdet & 0xF) == bcMirror)))) &&
                     (((isdet & 0xF) == bcMirror) || n1 != s->prop.w)) {
                 float Rtotal = 1.f;
                 float cphi, sphi, stheta, ctheta;
                 OCOUNT(w, nbnd);
-                s->prop = getvoxprop(g, s->mediaid, s->idx1d);
+
+                if (!g->svmc) {
+                    s->prop = getvoxprop(g, s->mediaid, s->idx1d);
+                }
+
                 float tmp0 = n1 * n1;
                 float tmp1 = s->prop.w * s->prop.w;
                 cphi = pm_fabs((fd->w == 0) ? v->x : (fd->w == 1 ? v->y : v->z));
@@ -1522,7 +1775,25 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
 
                     s->idx1d = idx1dold;
                     s->mediaid = (g->media[s->idx1d] & MED_MASK);
-                    s->prop = getvoxprop(g, s->mediaid, s->idx1d);
+
+                    if (g->svmc) {
+                        /* the side the reflected photon is on; label 0 ends it (src/mcx_core.cl:3221-3243) */
+                        s->prop = svmc_update(g, s->mediaid, s->idx1d, p, fd, &s->sp, s->prop);
+
+                        if (SV_CURLABEL(s->sp.sv) == 0) {
+                            if (launch_photon(g, w, s, idx, mediaidold & DET_MASK)) {
+                                break;
+                            }
+
+                            isdet = s->mediaid & DET_MASK;
+                            s->mediaid &= MED_MASK;
+                            RF_RESET();
+                            continue;
+                        }
+                    } else {
+                        s->prop = getvoxprop(g, s->mediaid, s->idx1d);
+                    }
+
                     n1 = s->prop.w;
                     isdet &= DET_MASK;
                 }
@@ -1621,6 +1892,7 @@ static void oracle_setup(Config* cfg, OParam* gp, F4** props_out, float** muamus
     gp->reclen = mcx_detreclen(cfg);
     gp->w0offset = gp->partialdata + 4;
     gp->mediaformat = cfg->mediabyte;
+    gp->svmc = (cfg->mediabyte == MEDIA_2LABEL_SPLIT);
     gp->gscatter = cfg->gscatter;
     gp->srcnum = cfg->srcnum;
     gp->doreflect = (uint32_t)cfg->isreflect;

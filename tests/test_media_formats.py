@@ -278,6 +278,11 @@ def test_unsupported_combinations_are_rejected_before_device_work():
     mua = prop[vol, 0][None].astype(np.float32)
     with pytest.raises(RuntimeError, match="label volume"):
         mcxcl_amd.run(dict(cfg, vol=mua, muamus=np.stack([prop[vol, 0], prop[vol, 1]]).astype(np.float32)))
-    # split-voxel and mixed-label volumes are not built
-    with pytest.raises(ValueError, match="svmc"):
-        mcxcl_amd.run(dict(cfg, vol=np.zeros((8,) + vol.shape, np.int8)))
+    # nor with split-voxel volumes
+    sv = np.zeros((8,) + vol.shape, np.uint8)
+    sv[0] = vol
+    with pytest.raises(RuntimeError, match="label volume"):
+        mcxcl_amd.run(dict(cfg, vol=sv, muamus=np.stack([prop[vol, 0], prop[vol, 1]]).astype(np.float32)))
+    # mixed-label volumes are not built
+    with pytest.raises(ValueError, match="mixlabel"):
+        mcxcl_amd.run(dict(cfg, vol=np.zeros((3,) + vol.shape, np.int16)))
