@@ -868,7 +868,17 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
                 tshift += ((int)pp[g.w0offset - 1] - 1) * (int)g.maxgate;
             }
 
-            if (t2 >= 0) {
+            if (t2 >= 0 && FULL && g.npattern > 1) {
+                /* patterns sharing the photon: one interleaved slot per pattern,
+                   scaled by its weight (src/mcx_core.cl:1525-1540) */
+                const uint32_t base = (s.idx1d + (uint32_t)tshift * g.nvox) * g.npattern;
+
+                for (uint32_t i = 0; i < g.npattern; i++) {
+                    if (fabsf(pp[g.w0offset + i]) > 0.f) {
+                        deposit(g, sh.repoff, base + i, -(p.w * pp[g.w0offset + i]));
+                    }
+                }
+            } else if (t2 >= 0) {
                 deposit(g, sh.repoff, s.idx1d + (uint32_t)tshift * g.nvox, -p.w);
             }
         }

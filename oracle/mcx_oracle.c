@@ -808,7 +808,16 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
                 tshift += ((int)s->pp[g->w0offset - 1].f - 1) * (int)g->maxgate;
             }
 
-            if (t2 >= 0) {
+            if (t2 >= 0 && g->npattern > 1) {
+                /* patterns sharing the photon: the interleaved slot of each pattern,
+                   scaled by its weight (src/mcx_core.cl:1525-1540) */
+                for (uint32_t i = 0; i < g->npattern; i++) {
+                    if (pm_fabs(s->pp[g->w0offset + i].f) > 0.f) {
+                        deposit(g, w, (s->idx1d + (uint32_t)tshift * g->nvox) * g->npattern + i,
+                                -(s->p.w * s->pp[g->w0offset + i].f));
+                    }
+                }
+            } else if (t2 >= 0) {
                 deposit(g, w, s->idx1d + (uint32_t)tshift * g->nvox, -s->p.w);
             }
         }

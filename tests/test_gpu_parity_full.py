@@ -75,6 +75,19 @@ def test_full_path_launches(name, extra):
     _same(gpu, ref)
 
 
+def test_shared_patterns_with_reflectance():
+    """issaveref = 1 with two patterns sharing photons: escaping weights go to the
+    interleaved slot of each pattern (src/mcx_core.cl:1525-1540)"""
+    vol = workloads.sphere_volume().copy()
+    vol[:, :, 0] = 0                                       # a void layer over the tissue
+    extra = dict(LAUNCHES)["pattern-shared"]
+    cfg = workloads.sphere_cube_workload(nphoton=30000, savedetflag="DSPXVW", issaveseed=1, tstep=1e-9, vol=vol,
+                                         issaveref=1, **extra)
+    gpu, ref = _pair(cfg)
+    assert (ref["flux64"] < 0).sum() > 100
+    _same(gpu, ref)
+
+
 @pytest.fixture(scope="module")
 def forward():
     base = workloads.sphere_cube_workload(nphoton=100000, issaveseed=1, savedetflag="DSP", tstep=1e-9)

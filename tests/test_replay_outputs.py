@@ -153,3 +153,30 @@ def test_pattern_photon_sharing_slots_equal_single_pattern_runs():
         # fluence up to the sample mean of the pixel weights
         fk = both["flux"].reshape(2, 60, 60, 60, order="F")[k]
         assert abs(fk.sum() / one["flux"].sum() - 1) < 0.02
+
+
+def test_pattern_photon_sharing_reflectance_slots():
+    """issaveref = 1 with patterns sharing photons: the escaping weight goes to
+    each pattern's interleaved slot scaled by that pattern's weight
+    (src/mcx_core.cl:1525-1540), so slot k is again the single-pattern run"""
+    rng = np.random.default_rng(12)
+    pa = rng.uniform(0.2, 1.0, (4, 3)).astype(np.float32)
+    pb = rng.uniform(0.2, 1.0, (4, 3)).astype(np.float32)
+
+    def case(pats):
+        cfg = _pattern_case(pats)
+        vol = np.ones((60, 60, 60), np.uint8)
+        vol[:, :, 0] = 0                                   # a void layer over the tissue
+        cfg.update(vol=vol, issaveref=1)
+        return cfg
+
+    both = oracle.run(case([pa, pb]), nthread=1024)
+    raw = both["flux64"].ravel(order="F")
+    for k, pat in enumerate((pa, pb)):
+        one = oracle.run(case([pat]), nthread=1024)
+        ref = one["flux64"].ravel(order="F")
+        dref = ref.reshape(60, 60, 60, order="F")[:, :, 0]
+        assert (dref < 0).sum() > 100                      # the reflectance tally is there
+        np.testing.assert_allclose(raw[k::2], ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+        got = raw[k::2].reshape(60, 60, 60, order="F")[:, :, 0]
+        np.testing.assert_allclose(got.sum(), dref.sum(), rtol=1e-6)
