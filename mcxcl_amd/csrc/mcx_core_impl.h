@@ -858,6 +858,22 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
             save_traj(g, p, photon_index(g, idx, f.w), pp[g.w0offset - 1]);
         }
 
+        if (!SIMPLE && SAVEDET && g.issaveref > 1 && s.mediaid == 0 && s.idx1d != K_OUT_MIN &&
+                s.idx1d != K_OUT_MAX && p.w > 0.f && s.idx1d < g.maxdetphoton) {
+            /* exit tally per voxel (src/mcx_core.cl:852-865): the detected-photon
+               buffer holds one record per voxel; its first column adds the exit
+               weight, and columns 0..maxmedia-1 add each medium's partial path
+               times that weight (both writes start at column 0, as in the
+               reference).  Atomic here; the reference's plain adds can lose
+               updates of concurrent photons. */
+            float* rec = g.detrec + (size_t)s.idx1d * g.reclen;
+            atomicAdd(rec, p.w);
+
+            for (uint32_t i = 0; i < g.maxmedia; i++) {
+                atomicAdd(rec + i, pp[i] * p.w);
+            }
+        }
+
         if (!SIMPLE && g.issaveref == 1 && s.mediaid == 0 && s.idx1d != K_OUT_MIN && s.idx1d != K_OUT_MAX &&
                 p.w > 0.f) {
             int tshift = (int)g.maxgate - 1;

@@ -280,10 +280,15 @@ void mcx_check_supported(const Config* cfg) {
         MCX_ERROR(-4, "photon sharing (srcnum > 1) needs a pattern or pattern3d source");
     }
 
-    if (cfg->polmedianum || cfg->issaveref > 1 || cfg->issavedet == FILL_MAXDETPHOTON ||
-            (cfg->debuglevel & MCX_DEBUG_MOVE_ONLY)) {
-        MCX_ERROR(-4, "polarization, issaveref>1, fill-max mode and move-only trajectories (debug flag T) are not "
+    if (cfg->polmedianum || cfg->issavedet == FILL_MAXDETPHOTON || (cfg->debuglevel & MCX_DEBUG_MOVE_ONLY)) {
+        MCX_ERROR(-4, "polarization, fill-max mode and move-only trajectories (debug flag T) are not "
                   "supported by the MI355X kernel");
+    }
+
+    /* the exit tally indexes the detected-photon buffer by voxel (src/mcx_utils.c:1797-1806
+       grows maxdetphoton to the voxel count) */
+    if (cfg->issaveref > 1 && (!cfg->issavedet || cfg->maxdetphoton < cfg->dim.x * cfg->dim.y * cfg->dim.z)) {
+        MCX_ERROR(-4, "issaveref > 1 needs detected-photon output with maxdetphoton of at least the voxel count");
     }
 }
 
@@ -935,6 +940,11 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     if (cfg->issavedet) {
         HIP_TRY(dalloc(bufs, &ddetrec, (size_t)cfg->maxdetphoton * (reclen ? reclen : 1)));
 
+        if (cfg->issaveref > 1) {
+            /* the exit tally accumulates into the records, one per voxel */
+            HIP_TRY(hipMemsetAsync(ddetrec, 0, (size_t)cfg->maxdetphoton * reclen * sizeof(float), stream));
+        }
+
         if (cfg->issaveseed) {
             HIP_TRY(dalloc(bufs, &ddetseed, (size_t)cfg->maxdetphoton * 2));
         }
@@ -1101,6 +1111,11 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     /* detected photons (src/mcx_host.cpp:1195-1237) */
     if (cfg->issavedet) {
         HIP_TRY(hipMemcpy(&hdetcount, ddetcount, sizeof(uint32_t), hipMemcpyDeviceToHost));
+
+        if (cfg->issaveref > 1) {
+            hdetcount = cfg->maxdetphoton;   /* every voxel's exit record (src/mcx_core.cl:3304-3306) */
+        }
+
         cfg->his.detected = hdetcount;
 
         if (hdetcount > cfg->maxdetphoton) {

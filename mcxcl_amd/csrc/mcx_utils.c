@@ -1758,6 +1758,8 @@ void mcx_preprocess(Config* cfg) {
         cfg->savedetflag &= ~(0x2u | 0x4u | 0x8u);
     }
 
+    /* exit tally per voxel (src/mcx_utils.c:1797-1806): the detected-photon
+       buffer holds one {weight, partial paths} record per voxel */
     if (cfg->issaveref > 1) {
         if (cfg->issavedet == 0) {
             MCX_ERROR(-4, "you must have at least two outputs if issaveref is greater than 1");

@@ -24,6 +24,7 @@
  *   media_word_prop       <- updateproperty, packed media formats (src/mcx_core.cl:1079-1193)
  *   pattern_weights       <- the pattern/pattern3d pixel weights (src/mcx_core.cl:1656-1766)
  *   save_traj             <- savedebugdata            (src/mcx_core.cl:929-944)
+ *   (in launch_photon)    <- saveexitppath            (src/mcx_core.cl:852-865)
  *   svmc_update           <- updateproperty_svmc      (src/mcx_core.cl:1231-1278)
  *   ray_plane             <- ray_plane_intersect      (src/mcx_core.cl:1280-1299)
  *   reflect_svmc          <- reflectray_svmc          (src/mcx_core.cl:1302-1345)
@@ -36,8 +37,8 @@
  * the reference's published absorbed fractions): tests/test_oracle_pins.py,
  * test_sampler_pins.py, test_output_pins.py, test_replay.py,
  * test_replay_outputs.py, test_media_formats.py, test_trajectory.py,
- * test_rf_forward.py, test_svmc.py, test_respin.py, test_multisource.py,
- * test_anchors.py.
+ * test_rf_forward.py, test_svmc.py, test_exit_tally.py, test_respin.py,
+ * test_multisource.py, test_anchors.py.
  *
  * Arithmetic: every transcendental goes through the deterministic FP32
  * functions of mcx_fpmath.h (shared with the kernel's parity mode and pinned
@@ -176,6 +177,7 @@ typedef struct {
     OTally* field;        /* private field, fieldlen entries (double in the checker) */
     float* det;           /* private detected-photon records */
     uint64_t* seeds;      /* private detected-photon seeds (2 words each) */
+    OTally* exitrec;      /* private per-voxel exit tally, issaveref > 1 (maxdetphoton records) */
     float* detw;          /* private exit weights of the detected photons */
     size_t ndet, capdet;
     OracleCounters cnt;
@@ -796,6 +798,19 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
 
         if (g->savetraj) {
             save_traj(g, &s->p, photon_index(g, idx, s->f.w), s->pp[g->w0offset - 1].f);
+        }
+
+        if (s->mediaid == 0 && s->idx1d != OUTSIDE_VOLUME_MIN && s->idx1d != OUTSIDE_VOLUME_MAX && g->issaveref > 1 &&
+                s->p.w > 0.f && s->idx1d < g->maxdetphoton) {
+            /* saveexitppath (src/mcx_core.cl:852-865): the first column adds the exit
+               weight and columns 0..maxmedia-1 each medium's partial path times it,
+               both from column 0 as in the reference */
+            OTally* rec = w->exitrec + (size_t)s->idx1d * g->reclen;
+            rec[0] += s->p.w;
+
+            for (uint32_t i = 0; i < g->maxmedia; i++) {
+                rec[i] += s->pp[i].f * s->p.w;
+            }
         }
 
         if (s->mediaid == 0 && s->idx1d != OUTSIDE_VOLUME_MIN && s->idx1d != OUTSIDE_VOLUME_MAX && g->issaveref &&
@@ -2019,6 +2034,10 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
 
     for (int i = 0; i < nworkers; i++) {
         works[i].field = (OTally*)calloc(fieldlen, sizeof(OTally));
+
+        if (cfg->issaveref > 1) {
+            works[i].exitrec = (OTally*)calloc((size_t)cfg->maxdetphoton * g.reclen, sizeof(OTally));
+        }
     }
 
     /* trajectory records (src/mcx_core.cl:929-944), one buffer over all repetitions */
@@ -2109,7 +2128,26 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
     cfg->his.detected = (unsigned int)ndet;
     size_t nsave = ndet < cfg->maxdetphoton ? ndet : cfg->maxdetphoton;
 
-    if (cfg->issavedet) {
+    if (cfg->issavedet && cfg->issaveref > 1) {
+        /* one exit record per voxel, summed over the workers in order; the
+           detected count is maxdetphoton (src/mcx_core.cl:3304-3306) */
+        const size_t nrec = (size_t)cfg->maxdetphoton * g.reclen;
+        free(cfg->exportdetected);
+        cfg->exportdetected = (float*)malloc((nrec ? nrec : 1) * sizeof(float));
+
+        for (size_t j = 0; j < nrec; j++) {
+            double acc = 0.0;
+
+            for (int i = 0; i < nworkers; i++) {
+                acc += works[i].exitrec[j];
+            }
+
+            cfg->exportdetected[j] = (float)acc;
+        }
+
+        cfg->his.detected = cfg->maxdetphoton;
+        cfg->detectedcount = cfg->maxdetphoton;
+    } else if (cfg->issavedet) {
         free(cfg->exportdetected);
         cfg->exportdetected = (float*)malloc((nsave ? nsave : 1) * g.reclen * sizeof(float));
 
@@ -2140,6 +2178,7 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
 
     for (int i = 0; i < nworkers; i++) {
         free(works[i].field);
+        free(works[i].exitrec);
         free(works[i].det);
         free(works[i].seeds);
         free(works[i].detw);
