@@ -75,6 +75,8 @@ struct MCXKParam {
     uint32_t maxjumpdebug;        /* trajectory records the buffer holds */
     uint32_t svmc;                /* 1: split-voxel media, two words per voxel (the second block of
                                      nvox words holds each voxel's plane) */
+    uint32_t moveonly;            /* 1: move-only trajectories: a lane's run ends when its photon moves
+                                     into the background without being relaunched */
     unsigned char bc[12];
 
     const uint32_t* media;        /* label volume, x fastest; bit 31 marks detector-adjacent voxels */

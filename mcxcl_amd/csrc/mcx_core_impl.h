@@ -45,6 +45,8 @@
 #define K_SEED_FROM_FILE (-999)
 
 enum {kReflect = 1, kAbsorb = 2, kMirror = 3, kCyclic = 4};
+/* issavedet value of fill-max mode: stop once maxdetphoton photons are detected */
+#define K_FILL_MAXDET 3u
 enum {kFlux = 0, kFluence = 1, kEnergy = 2, kJacobian = 3, kWP = 4, kDCS = 5, kRF = 6, kL = 7, kRFmus = 8,
       kWLTOF = 9, kWPTOF = 10
      };
@@ -775,6 +777,10 @@ __device__ void save_detected(const MCXKParam& g, const Shared& sh, const Photon
     uint32_t slot = atomicAdd(g.detcount, 1u);
 
     if (slot >= g.maxdetphoton) {
+        if (g.savedet == K_FILL_MAXDET) {
+            atomicSub(g.detcount, 1u);   /* fill-max mode keeps the count at the buffer size (:922-923) */
+        }
+
         return;
     }
 
@@ -850,6 +856,13 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
     float4& f = s.f;
     s.w0 = 1.f;
     s.Lmove = -1.f;
+
+    if (SAVEDET && g.savedet == K_FILL_MAXDET && __hip_atomic_load(g.detcount, __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT) >= g.maxdetphoton) {
+        /* fill-max mode (src/mcx_core.cl:1484-1489): the lane stops once the
+           detected-photon buffer is full */
+        return 1;
+    }
 
     if (fabsf(p.w) >= 0.f) {
         pp[pd] += p.w;
@@ -1376,7 +1389,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
    fresh; the RNG, the LDS tables and the lane's scratch come in, and the
    lane's pending grid deposits are flushed before returning. */
 template <bool SAVEDET, bool REFLECT, bool SIMPLE, bool FULL>
-__device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared& sh, Rng& rng, const int idx) {
+__device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared& sh, Rng& rng, const int idx) {
     Photon s;
     s.p = make_float4(0.f, 0.f, 0.f, __uint_as_float(0x7FC00000u));
     s.v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1411,6 +1424,9 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
     bool testint = false, hitintf = false;
     s.sv = 0;
     s.snx = s.sny = s.snz = s.spd = 0.f;
+    /* move-only trajectories: the lane's run ends when a photon moves into the
+       background without being relaunched */
+    bool moveend = false;
 
     for (;;) {
         if (relaunch) {
@@ -1884,10 +1900,18 @@ __device__ __forceinline__ void transport_lane(const MCXKParam& g, const Shared&
                     isdet &= K_DET_MASK;
                 }
             }
+
+            if (FULL && g.moveonly && (s.mediaid == 0 || s.idx1d == K_OUT_MIN || s.idx1d == K_OUT_MAX)) {
+                /* move-only trajectories (debug flag T, src/mcx_core.cl:3287-3291): the
+                   lane's whole run ends here, without its energy write-back */
+                moveend = true;
+                break;
+            }
         }
     }
 
     depcache_flush(g, sh, dcache);
+    return !moveend;
 }
 
 #ifdef MCX_WAVES_PER_EU
@@ -1959,8 +1983,7 @@ __global__ void __launch_bounds__(MCX_KBOUND(SAVEDET, REFLECT, SIMPLE, FULL)) MC
     rng.s0 = (uint64_t)g.nseed[sidx * 4] << 32 | g.nseed[sidx * 4 + 1];
     rng.s1 = (uint64_t)g.nseed[sidx * 4 + 2] << 32 | g.nseed[sidx * 4 + 3];
 
-    if (active) {
-        transport_lane<SAVEDET, REFLECT, SIMPLE, FULL>(g, sh, rng, idx);
+    if (active && transport_lane<SAVEDET, REFLECT, SIMPLE, FULL>(g, sh, rng, idx)) {
         g.genergy[idx * 2] = sh.pp[pd];
         g.genergy[idx * 2 + 1] = sh.pp[pd + 1];
     }

@@ -280,9 +280,8 @@ void mcx_check_supported(const Config* cfg) {
         MCX_ERROR(-4, "photon sharing (srcnum > 1) needs a pattern or pattern3d source");
     }
 
-    if (cfg->polmedianum || cfg->issavedet == FILL_MAXDETPHOTON || (cfg->debuglevel & MCX_DEBUG_MOVE_ONLY)) {
-        MCX_ERROR(-4, "polarization, fill-max mode and move-only trajectories (debug flag T) are not "
-                  "supported by the MI355X kernel");
+    if (cfg->polmedianum) {
+        MCX_ERROR(-4, "polarization is not supported by the MI355X kernel");
     }
 
     /* the exit tally indexes the detected-photon buffer by voxel (src/mcx_utils.c:1797-1806
@@ -407,7 +406,8 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
         } lab;
         lab.f = cfg->srcparam2.w;
         g.simple = cfg->srctype == MCX_SRC_PENCIL && cfg->srcdir.w == 0.f && (lab.u & MED_MASK) != 0 &&
-                   cfg->mediabyte <= 4 && !(cfg->debuglevel & MCX_DEBUG_MOVE) && !(cfg->omega > 0.f) &&
+                   cfg->mediabyte <= 4 && !(cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) &&
+                   !(cfg->omega > 0.f) && cfg->issavedet != FILL_MAXDETPHOTON &&
                    cfg->extrasrclen == 0 && cfg->seed != SEED_FROM_FILE && cfg->issaveref == 0 && g.is2d == 0 &&
                    cfg->nphase <= 2 && cfg->nangle == 0 && cfg->srcpos.w > 0.f && cfg->minenergy < cfg->srcpos.w &&
                    (cfg->outputtype == otFlux || cfg->outputtype == otFluence || cfg->outputtype == otEnergy ||
@@ -419,11 +419,12 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
              (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D || cfg->srctype == MCX_SRC_FOURIERX ||
               cfg->srctype == MCX_SRC_FOURIERX2D || cfg->srctype == MCX_SRC_HYPERBOLOID_GAUSSIAN || cfg->nangle ||
               cfg->outputtype == otRF || cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
-              cfg->outputtype == otWPTOF || cfg->mediabyte > 4 || (cfg->debuglevel & MCX_DEBUG_MOVE) ||
+              cfg->outputtype == otWPTOF || cfg->mediabyte > 4 || (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) ||
               (cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE));
 
     /* trajectory records (src/mcx_core.cl:929-944) */
-    g.savetraj = (cfg->debuglevel & MCX_DEBUG_MOVE) ? 1u : 0u;
+    g.savetraj = (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) ? 1u : 0u;
+    g.moveonly = (cfg->debuglevel & MCX_DEBUG_MOVE_ONLY) ? 1u : 0u;
     g.maxjumpdebug = cfg->maxjumpdebug;
     g.svmc = (cfg->mediabyte == MEDIA_2LABEL_SPLIT) ? 1u : 0u;
 
@@ -627,7 +628,7 @@ static void mcx_plan_run(Config* cfg, int dev, int ncu, size_t* nthread_io, MCXK
     /* fast mode hands photons out at run time (below) unless replaying, where
        each lane must rerun the photons of its recorded seeds, or recording
        trajectories, whose photon ids follow the fixed map */
-    const int dynamic = fast && cfg->seed != SEED_FROM_FILE && !(cfg->debuglevel & MCX_DEBUG_MOVE) &&
+    const int dynamic = fast && cfg->seed != SEED_FROM_FILE && !(cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) &&
                         !(envdyn && envdyn[0] == '0');
     size_t lblock = MCX_LAUNCH_BLOCK;
     MCXKParam& g = *gp;
@@ -1594,8 +1595,8 @@ static void noalloc_run(Config* cfg, const uint32_t* media, size_t nmedia, doubl
     mcx_check_supported(cfg);
 
     if (cfg->issavedet || cfg->seed == SEED_FROM_FILE || MCX_IS_ADJOINT_TYPE(cfg->outputtype) ||
-            (cfg->debuglevel & (MCX_DEBUG_RNG | MCX_DEBUG_MOVE)) || cfg->muamus || cfg->mediabyte > 4 ||
-            cfg->omega > 0.f || !cfg->issave2pt) {
+            (cfg->debuglevel & (MCX_DEBUG_RNG | MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) || cfg->muamus ||
+            cfg->mediabyte > 4 || cfg->omega > 0.f || !cfg->issave2pt) {
         MCX_ERROR(-4, "mcx_run_simulation_noalloc runs forward fluence simulations only: no detected photons, "
                   "replay, adjoint outputs, RNG dump, trajectories, RF or per-voxel media, and the field must be saved");
     }

@@ -58,6 +58,7 @@ def _carray(arr, dtype):
 MEDIA_LABEL_HALF, MEDIA_AS_F2H, MEDIA_MUA_FLOAT, MEDIA_ASGN_BYTE, MEDIA_AS_SHORT = 99, 100, 101, 103, 104
 MEDIA_2LABEL_SPLIT = 97
 MCX_DEBUG_MOVE = 2
+MCX_DEBUG_MOVE_ONLY = 8
 
 
 def _pack_float_media(vol, fmt, unitinmm):
@@ -506,7 +507,7 @@ def _collect(c, user, ngate, ndet, nsrc):
     if c.issavedet and n > 0 and reclen > 0:
         det = np.ctypeslib.as_array((ctypes.c_float * (reclen * n)).from_address(c.exportdetected)).copy()
         out["detp"] = np.asfortranarray(det.reshape(n, reclen).T)
-    if (c.debuglevel & MCX_DEBUG_MOVE) and c.exportdebugdata and c.debugdatalen:
+    if (c.debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) and c.exportdebugdata and c.debugdatalen:
         # trajectory positions [6, N]: photon id (uint32 bits), x, y, z, weight, source id
         # (src/pmcxcl.cpp:1266-1283)
         traj = np.ctypeslib.as_array((ctypes.c_float * (6 * c.debugdatalen)).from_address(c.exportdebugdata)).copy()

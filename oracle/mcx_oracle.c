@@ -169,6 +169,8 @@ typedef struct {
     float omega;               /* RF replay: angular modulation frequency (rad/s) */
     uint32_t savetraj, maxjumpdebug;  /* trajectory records (src/mcx_core.cl:929-944) */
     uint32_t svmc;                    /* split-voxel media: a second block of nvox plane words */
+    uint32_t moveonly;                /* move-only trajectories (debug flag T) */
+    uint32_t* detcount;               /* detected photons over all threads (fill-max mode) */
     float* debugdata;                 /* maxjumpdebug records of 6 floats */
     uint32_t* jumpdebug;              /* records written, counted past maxjumpdebug */
 } OParam;
@@ -680,6 +682,14 @@ static void save_detected(const OParam* g, OWork* w, OPhoton* s, uint32_t isdet)
         return;
     }
 
+    if (g->savedet == FILL_MAXDETPHOTON) {
+        /* fill-max mode: a slot past the buffer is given back (src/mcx_core.cl:921-923) */
+        if (__atomic_fetch_add(g->detcount, 1u, __ATOMIC_RELAXED) >= g->maxdetphoton) {
+            __atomic_fetch_sub(g->detcount, 1u, __ATOMIC_RELAXED);
+            return;
+        }
+    }
+
     if (w->ndet >= w->capdet) {
         w->capdet = w->capdet ? w->capdet * 2 : 1024;
         w->det = (float*)realloc(w->det, w->capdet * g->reclen * sizeof(float));
@@ -791,6 +801,10 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
     const uint32_t pd = g->partialdata;
     s->w0 = 1.f;
     s->Lmove = -1.f;
+
+    if (g->savedet == FILL_MAXDETPHOTON && __atomic_load_n(g->detcount, __ATOMIC_RELAXED) >= g->maxdetphoton) {
+        return 1;   /* fill-max mode: the detected-photon buffer is full (src/mcx_core.cl:1484-1489) */
+    }
     F4 lsrcpos = g->srcpos, lsrcdir = g->srcdir, lp1 = g->srcparam1, lp2 = g->srcparam2;
 
     if (pm_fabs(s->p.w) >= 0.f) {    /* false only for the NaN of the first call */
@@ -1822,6 +1836,10 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                     isdet &= DET_MASK;
                 }
             }
+
+            if (g->moveonly && (s->mediaid == 0 || s->idx1d == OUTSIDE_VOLUME_MIN || s->idx1d == OUTSIDE_VOLUME_MAX)) {
+                return;   /* move-only trajectories end the thread, energies unwritten (src/mcx_core.cl:3287-3291) */
+            }
         }
     }
 
@@ -2042,7 +2060,10 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
 
     /* trajectory records (src/mcx_core.cl:929-944), one buffer over all repetitions */
     uint32_t njump = 0;
-    g.savetraj = (cfg->debuglevel & MCX_DEBUG_MOVE) ? 1u : 0u;
+    g.savetraj = (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) ? 1u : 0u;
+    g.moveonly = (cfg->debuglevel & MCX_DEBUG_MOVE_ONLY) ? 1u : 0u;
+    uint32_t ndetall = 0;
+    g.detcount = &ndetall;
     g.maxjumpdebug = cfg->maxjumpdebug;
     g.jumpdebug = &njump;
     g.debugdata = g.savetraj ? (float*)calloc((size_t)cfg->maxjumpdebug * MCX_DEBUG_REC_LEN + 1, sizeof(float)) : NULL;
