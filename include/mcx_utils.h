@@ -77,6 +77,7 @@ typedef MCXUInt3 uint3;
 
 #define MEDIA_ASGN_F2H     96
 #define MEDIA_2LABEL_SPLIT 97
+#define MCX_NANGLES        181  /* scattering angles of a polarised medium's Mie table, 0..180 degrees */
 #define MEDIA_2LABEL_MIX   98
 #define MEDIA_LABEL_HALF   99
 #define MEDIA_AS_F2H       100
@@ -339,6 +340,11 @@ int  mcx_loadjson_string(const char* json, Config* cfg);
 void mcx_loadbenchmark(char* key, Config* cfg);                             /* src/mcx_utils.h:335 */
 void mcx_prepdomain(char* filename, Config* cfg);                           /* src/mcx_utils.h:305 */
 void mcx_preprocess(Config* cfg);                                           /* src/mcx_utils.c:1521 (not in the reference header) */
+void mcx_prep_polarized(Config* cfg);                                       /* src/mcx_utils.h:340 */
+/* replaces void Mie(double x, double mx, const double* mu, float4* smatrix, double* qsca,
+   double* g) (src/mcx_mie.h:42): nangle angles, smatrix as 4 floats per angle, an error
+   code instead of an exit for x outside (0, 20000] */
+int  mcx_mie(double x, double m, int nangle, const double* mu, float* smatrix, double* qsca, double* g);
 void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte); /* src/mcx_utils.h:337 */
 void mcx_append_detector_sources(Config* cfg);                              /* the detector-as-source append of
                                                                                src/mcx_utils.c:1902-1925 and

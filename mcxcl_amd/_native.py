@@ -191,6 +191,9 @@ lib.mcx_try_savedetphoton.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c
 lib.mcx_try_savedetphoton.restype = ctypes.c_int
 lib.mcx_pack_media.argtypes = [ctypes.c_uint, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_void_p]
 lib.mcx_pack_media.restype = ctypes.c_int
+lib.mcx_mie.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+lib.mcx_mie.restype = ctypes.c_int
 lib.mcx_try_run_simulation.argtypes = [_P(Config)]
 lib.mcx_try_run_simulation.restype = ctypes.c_int
 lib.mcx_last_error.restype = ctypes.c_char_p

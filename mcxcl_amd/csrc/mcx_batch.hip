@@ -376,7 +376,7 @@ static void batch_create_impl(MCXBatch* b, Config** cfgs, unsigned int n, MCXBat
         }
 
         if (cfg->issavedet || cfg->seed == SEED_FROM_FILE ||
-                (cfg->debuglevel & (MCX_DEBUG_RNG | MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) ||
+                (cfg->debuglevel & (MCX_DEBUG_RNG | MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) || cfg->polmedianum ||
                 cfg->muamus || cfg->mediabyte > 4 || cfg->omega > 0.f || MCX_IS_ADJOINT_TYPE(cfg->outputtype)) {
             MCX_ERROR(-4, "the batched launch takes no detected photons, replay, RNG dump, trajectories, RF, "
                       "per-voxel or packed media or adjoint outputs");

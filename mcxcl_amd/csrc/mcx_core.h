@@ -77,6 +77,11 @@ struct MCXKParam {
                                      nvox words holds each voxel's plane) */
     uint32_t moveonly;            /* 1: move-only trajectories: a lane's run ends when its photon moves
                                      into the background without being relaunched */
+    uint32_t maxpolmedia;         /* polarised media (1..maxpolmedia carry Mie tables); 0: unpolarised */
+    uint32_t istrajstokes;        /* 1: trajectory records carry the Stokes vector (10 floats, else 6) */
+    uint32_t trajlen;             /* floats per trajectory record */
+    float4 s0;                    /* launch Stokes vector {I, Q, U, V} */
+    const float4* smatrix;        /* {S11, S12, S33, S34} at 181 angles (0..180 degrees) per polarised medium */
     unsigned char bc[12];
 
     const uint32_t* media;        /* label volume, x fastest; bit 31 marks detector-adjacent voxels */

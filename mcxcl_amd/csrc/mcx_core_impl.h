@@ -35,6 +35,8 @@
 
 #define K_EPS          FLT_EPSILON
 #define K_TWO_PI       6.28318530717959f
+#define K_R_PI         0.318309886183791f
+#define K_NANGLES      181u
 #define K_ONE_PI       3.1415926535897932f
 #define K_JUST_BELOW_ONE 0.9998f
 #define K_ROULETTE     10.f
@@ -303,6 +305,7 @@ struct Photon {
        lower label | upper label << 8 | split << 16 | on the upper side << 17 */
     float snx, sny, snz, spd;
     uint32_t sv;
+    float4 stokes;               /* polarised runs: {I, Q, U, V}, I kept at 1 after each scattering */
 };
 
 struct Shared {
@@ -720,18 +723,70 @@ __device__ __forceinline__ uint32_t photon_index(const MCXKParam& g, int idx, fl
 
 /* trajectory record (src/mcx_core.cl:929-944): {photon id as bits, x, y, z, w,
    source id}; positions past maxjumpdebug are counted, not stored */
-__device__ __forceinline__ void save_traj(const MCXKParam& g, const float4& p, uint32_t id, float srcid) {
+__device__ __forceinline__ void save_traj(const MCXKParam& g, const float4& p, uint32_t id, float srcid,
+        const float4* stokes = nullptr) {
     const uint32_t pos = atomicAdd(g.jumpdebug, 1u);
 
     if (pos < g.maxjumpdebug) {
-        float* r = g.debugdata + (size_t)pos * 6;
+        /* records are 6 floats, 10 with istrajstokes; the Stokes vector follows in
+           polarised runs (src/mcx_core.cl:929-959) */
+        float* r = g.debugdata + (size_t)pos * g.trajlen;
         r[0] = __uint_as_float(id);
         r[1] = p.x;
         r[2] = p.y;
         r[3] = p.z;
         r[4] = p.w;
         r[5] = (float)(int)srcid;
+
+        if (stokes && g.maxpolmedia && g.istrajstokes) {
+            r[6] = stokes->x;
+            r[7] = stokes->y;
+            r[8] = stokes->z;
+            r[9] = stokes->w;
+        }
     }
+}
+
+/* Stokes vector after a scattering by theta at azimuth phi from direction u to
+   u2 in a polarised medium (src/mcx_core.cl:781-836): rotate into the
+   scattering plane, apply the Mueller matrix of the angle's table entry, rotate
+   into the new meridian plane and normalise to I = 1 */
+__device__ __forceinline__ void update_stokes(const MCXKParam& g, float4& st, float theta, float phi, const float4& u,
+        const float4& u2, uint32_t mediaid) {
+    const float costheta = kcos(theta);
+    float sin2phi, cos2phi;
+    ksincos(2.f * phi, &sin2phi, &cos2phi);
+    float4 s2;
+    s2.x = st.x;
+    s2.y = st.y * cos2phi + st.z * sin2phi;
+    s2.z = -st.y * sin2phi + st.z * cos2phi;
+    s2.w = st.w;
+
+    const uint32_t lab = mediaid & K_MED_MASK;
+    const float4 m = g.smatrix[K_NANGLES * (lab ? lab - 1 : 0u) + (uint32_t)(theta * (float)K_NANGLES * (K_R_PI - K_EPS))];
+    st.x = m.x * s2.x + m.y * s2.y;
+    st.y = m.y * s2.x + m.x * s2.y;
+    st.z = m.z * s2.z + m.w * s2.w;
+    st.w = -m.w * s2.z + m.z * s2.w;
+
+    float temp = (u2.z > -1.f && u2.z < 1.f) ? krsqrt((1.f - costheta * costheta) * (1.f - u2.z * u2.z)) : 0.f;
+    float cosi = (temp == 0.f) ? 0.f : (((phi > K_ONE_PI && phi < K_TWO_PI) ? 1.f : -1.f) *
+                                        (u2.z * costheta - u.z) * temp);
+    cosi = fmaxf(-1.f, fminf(cosi, 1.f));
+    const float sini = ksqrt(1.f - cosi * cosi);
+    const float cos22 = 2.f * cosi * cosi - 1.f;
+    const float sin22 = 2.f * sini * cosi;
+
+    s2.x = st.x;
+    s2.y = st.y * cos22 - st.z * sin22;
+    s2.z = st.y * sin22 + st.z * cos22;
+    s2.w = st.w;
+
+    temp = kdiv(1.f, s2.x);
+    st.y = s2.y * temp;
+    st.z = s2.z * temp;
+    st.w = s2.w * temp;
+    st.x = 1.f;
 }
 
 /* the photon id the reference writes into launch and scattering records
@@ -752,7 +807,7 @@ __device__ __forceinline__ void deposit(const MCXKParam& g, size_t repoff, uint3
 }
 
 /* src/mcx_core.cl:838-926 */
-template <bool SAVEDET>
+template <bool SAVEDET, bool FULL>
 __device__ void save_detected(const MCXKParam& g, const Shared& sh, const Photon& s, uint32_t isdet) {
     int detid = 0;
 
@@ -823,6 +878,14 @@ __device__ void save_detected(const MCXKParam& g, const Shared& sh, const Photon
     if ((g.savedetflag >> 6) & 1) {
         rec[k++] = sh.pp[g.w0offset - 2];
     }
+
+    if (FULL && ((g.savedetflag >> 7) & 1)) {
+        /* the Stokes vector at exit (src/mcx_core.cl:915-920) */
+        rec[k++] = s.stokes.x;
+        rec[k++] = s.stokes.y;
+        rec[k++] = s.stokes.z;
+        rec[k++] = s.stokes.w;
+    }
 }
 
 /* launch weight of a pattern pixel (src/mcx_core.cl:1678-1727): one pattern
@@ -868,7 +931,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
         pp[pd] += p.w;
 
         if (FULL && g.savetraj) {
-            save_traj(g, p, photon_index(g, idx, f.w), pp[g.w0offset - 1]);
+            save_traj(g, p, photon_index(g, idx, f.w), pp[g.w0offset - 1], &s.stokes);
         }
 
         if (!SIMPLE && SAVEDET && g.issaveref > 1 && s.mediaid == 0 && s.idx1d != K_OUT_MIN &&
@@ -914,7 +977,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
 
         if (SAVEDET && (isdet & K_DET_MASK) == K_DET_MASK &&
                 (s.mediaid == 0 || (FULL && g.svmc && K_SV_CURLABEL(s.sv) == 0)) && g.issaveref < 2) {
-            save_detected<SAVEDET>(g, sh, s, isdet);
+            save_detected<SAVEDET, FULL>(g, sh, s, isdet);
         }
     }
 
@@ -1008,6 +1071,10 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
 
         if (FULL && g.svmc) {
             s.sv = 0;
+        }
+
+        if (FULL && g.maxpolmedia) {
+            s.stokes = g.s0;    /* src/mcx_core.cl:1633-1638 */
         }
 
         if (!SIMPLE) {
@@ -1363,7 +1430,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
              getprop<FULL>(g, sh, s.mediaid, s.idx1d);
 
     if (FULL && g.savetraj) {
-        save_traj(g, p, traj_index(g, idx, f.w), ppl[3]);   /* the source-id slot */
+        save_traj(g, p, traj_index(g, idx, f.w), ppl[3], &s.stokes);   /* the source-id slot */
     }
 
     ppl[1] += p.w;
@@ -1453,8 +1520,29 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
             f.x = rng.scatlen();
 
             if (v.w != K_EPS) {
-                float cphi = 1.f, sphi = 0.f, theta, stheta, ctheta, tmp0 = 0.f;
+                float cphi = 1.f, sphi = 0.f, theta = 0.f, stheta, ctheta, tmp0 = 0.f;
 
+                if (FULL && g.maxpolmedia) {
+                    /* polarised media (src/mcx_core.cl:2454-2468): (theta, phi) by rejection
+                       against I(theta, phi) = S11 I + S12 (Q cos 2phi + U sin 2phi) of the
+                       medium's table, the forward entry as the bound */
+                    const uint32_t lab = s.mediaid & K_MED_MASK;
+                    const uint32_t ipol = K_NANGLES * (lab ? lab - 1 : 0u);   /* scattering needs mus > 0: label >= 1 */
+                    float i0, ii, sin2phi, cos2phi;
+
+                    do {
+                        theta = kacos(2.f * rng.next() - 1.f);
+                        tmp0 = K_TWO_PI * rng.next();
+                        ksincos(2.f * tmp0, &sin2phi, &cos2phi);
+                        const float4 m0 = g.smatrix[ipol];
+                        i0 = m0.x * s.stokes.x + m0.y * (s.stokes.y * cos2phi + s.stokes.z * sin2phi);
+                        const float4 mt = g.smatrix[ipol + (uint32_t)(theta * (float)K_NANGLES * (K_R_PI - K_EPS))];
+                        ii = mt.x * s.stokes.x + mt.y * (s.stokes.y * cos2phi + s.stokes.z * sin2phi);
+                    } while (rng.next() * i0 >= ii);
+
+                    ksincos(tmp0, &sphi, &cphi);
+                    ksincos(theta, &stheta, &ctheta);
+                } else {
                 if (SIMPLE || !g.is2d) {
                     tmp0 = K_TWO_PI * rng.next();
                     ksincos(tmp0, &sphi, &cphi);
@@ -1488,6 +1576,7 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
 #endif
                     }
                 }
+                }
 
                 /* per-medium tallies: a split voxel's current side (src/mcx_core.cl:2502-2540) */
                 const uint32_t tlab = (FULL && g.svmc) ? K_SV_CURLABEL(s.sv) : (s.mediaid & K_MED_MASK);
@@ -1504,6 +1593,8 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                     }
                 }
 
+                const float4 olddir = v;
+
                 if (!SIMPLE && g.is2d) {
                     rotatevector2d(v, (rng.next() > 0.5f ? stheta : -stheta), ctheta, (int)g.is2d);
                 } else {
@@ -1511,6 +1602,10 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                 }
 
                 v.w += 1.f;
+
+                if (FULL && g.maxpolmedia) {
+                    update_stokes(g, s.stokes, theta, tmp0, olddir, v, s.mediaid);   /* :2550-2565 */
+                }
 
                 if (!SIMPLE && (g.outputtype == kWP || g.outputtype == kDCS ||
                                 (FULL && (g.outputtype == kWPTOF ||
@@ -1548,7 +1643,7 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                 }
 
                 if (FULL && g.savetraj) {
-                    save_traj(g, p, traj_index(g, idx, f.w), sh.pp[g.w0offset - 1]);
+                    save_traj(g, p, traj_index(g, idx, f.w), sh.pp[g.w0offset - 1], &s.stokes);
                 }
             }
 

@@ -280,8 +280,14 @@ void mcx_check_supported(const Config* cfg) {
         MCX_ERROR(-4, "photon sharing (srcnum > 1) needs a pattern or pattern3d source");
     }
 
-    if (cfg->polmedianum) {
-        MCX_ERROR(-4, "polarization is not supported by the MI355X kernel");
+    /* polarised runs scatter by the Mie tables in 3D only: the reference's 2D branch
+       updates the Stokes vector with the cosine it sampled in place of the azimuth */
+    if (cfg->polmedianum && (cfg->dim.x == 1 || cfg->dim.y == 1 || cfg->dim.z == 1)) {
+        MCX_ERROR(-4, "polarized simulations need a 3D domain");
+    }
+
+    if (cfg->polmedianum && !cfg->smatrix) {
+        MCX_ERROR(-4, "polarized media need their Mie tables (mcx_prep_polarized)");
     }
 
     /* the exit tally indexes the detected-photon buffer by voxel (src/mcx_utils.c:1797-1806
@@ -407,7 +413,7 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
         lab.f = cfg->srcparam2.w;
         g.simple = cfg->srctype == MCX_SRC_PENCIL && cfg->srcdir.w == 0.f && (lab.u & MED_MASK) != 0 &&
                    cfg->mediabyte <= 4 && !(cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) &&
-                   !(cfg->omega > 0.f) && cfg->issavedet != FILL_MAXDETPHOTON &&
+                   !(cfg->omega > 0.f) && cfg->issavedet != FILL_MAXDETPHOTON && cfg->polmedianum == 0 &&
                    cfg->extrasrclen == 0 && cfg->seed != SEED_FROM_FILE && cfg->issaveref == 0 && g.is2d == 0 &&
                    cfg->nphase <= 2 && cfg->nangle == 0 && cfg->srcpos.w > 0.f && cfg->minenergy < cfg->srcpos.w &&
                    (cfg->outputtype == otFlux || cfg->outputtype == otFluence || cfg->outputtype == otEnergy ||
@@ -420,12 +426,18 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
               cfg->srctype == MCX_SRC_FOURIERX2D || cfg->srctype == MCX_SRC_HYPERBOLOID_GAUSSIAN || cfg->nangle ||
               cfg->outputtype == otRF || cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
               cfg->outputtype == otWPTOF || cfg->mediabyte > 4 || (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) ||
+              cfg->polmedianum > 0 ||
               (cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE));
 
     /* trajectory records (src/mcx_core.cl:929-944) */
     g.savetraj = (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) ? 1u : 0u;
     g.moveonly = (cfg->debuglevel & MCX_DEBUG_MOVE_ONLY) ? 1u : 0u;
     g.maxjumpdebug = cfg->maxjumpdebug;
+    /* polarised media (src/mcx_core.cl:2454-2565) and Stokes trajectory records */
+    g.maxpolmedia = cfg->polmedianum;
+    g.istrajstokes = cfg->istrajstokes ? 1u : 0u;
+    g.trajlen = MCX_DEBUG_REC_LEN + (g.istrajstokes << 2);
+    g.s0 = make_float4(cfg->srciquv.x, cfg->srciquv.y, cfg->srciquv.z, cfg->srciquv.w);
     g.svmc = (cfg->mediabyte == MEDIA_2LABEL_SPLIT) ? 1u : 0u;
 
     /* reflection block rule (src/mcx_host.cpp:945-956) */
@@ -834,6 +846,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     float kernelms = 0.f;
     uint32_t* dmedia = NULL;
     float4* dprop = NULL;
+    float4* dsmatrix = NULL;
     float4* ddetpos = NULL;
     float* dinvcdf = NULL;
     float* dangle = NULL;
@@ -873,6 +886,12 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     HIP_TRY(hipMemcpyAsync(dmedia, cfg->vol, nvox * (cfg->mediabyte == MEDIA_2LABEL_SPLIT ? 2 : 1) * sizeof(uint32_t),
                            hipMemcpyHostToDevice, stream));
     HIP_TRY(dalloc(bufs, &dprop, cfg->medianum));
+
+    if (cfg->polmedianum) {
+        HIP_TRY(dalloc(bufs, &dsmatrix, (size_t)cfg->polmedianum * MCX_NANGLES));
+        HIP_TRY(hipMemcpyAsync(dsmatrix, cfg->smatrix, (size_t)cfg->polmedianum * MCX_NANGLES * sizeof(float4),
+                               hipMemcpyHostToDevice, stream));
+    }
     HIP_TRY(hipMemcpyAsync(dprop, cfg->prop, cfg->medianum * sizeof(float4), hipMemcpyHostToDevice, stream));
 
     if (cfg->detnum) {
@@ -933,7 +952,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     HIP_TRY(hipMemsetAsync(ddetcount, 0, sizeof(uint32_t), stream));
 
     if (g.savetraj) {
-        HIP_TRY(dalloc(bufs, &ddebug, (size_t)cfg->maxjumpdebug * MCX_DEBUG_REC_LEN));
+        HIP_TRY(dalloc(bufs, &ddebug, (size_t)cfg->maxjumpdebug * g.trajlen));
         HIP_TRY(dalloc(bufs, &djump, 1));
         HIP_TRY(hipMemsetAsync(djump, 0, sizeof(uint32_t), stream));
     }
@@ -980,6 +999,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
 
     g.media = dmedia;
     g.prop = dprop;
+    g.smatrix = dsmatrix;
     g.detpos = ddetpos;
     g.invcdf = dinvcdf;
     g.angleinvcdf = dangle;
@@ -1096,15 +1116,15 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
 
         if (njump) {
             float* buf = (float*)realloc(cfg->exportdebugdata, ((size_t)cfg->debugdatalen + njump) *
-                                         MCX_DEBUG_REC_LEN * sizeof(float));
+                                         g.trajlen * sizeof(float));
 
             if (!buf) {
                 HIP_TRY(hipErrorOutOfMemory);
             }
 
             cfg->exportdebugdata = buf;
-            HIP_TRY(hipMemcpy(buf + (size_t)cfg->debugdatalen * MCX_DEBUG_REC_LEN, ddebug,
-                              (size_t)njump * MCX_DEBUG_REC_LEN * sizeof(float), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(buf + (size_t)cfg->debugdatalen * g.trajlen, ddebug,
+                              (size_t)njump * g.trajlen * sizeof(float), hipMemcpyDeviceToHost));
             cfg->debugdatalen += njump;
         }
     }
@@ -1335,10 +1355,11 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
             mcx_savedetphoton(cfg->exportdetected, cfg->seeddata, (int)cfg->detectedcount, 0, cfg);
         }
 
-        /* trajectories: a .mct file, the .mch layout with 6 columns (src/mcx_host.cpp:1666-1672) */
+        /* trajectories: a .mct file in the .mch layout (src/mcx_host.cpp:1666-1672), 6 columns or
+           10 with Stokes vectors (the reference's header says 6 for those too) */
         if (g.savetraj && cfg->exportdebugdata) {
             const unsigned int colcount = cfg->his.colcount, seedbyte = cfg->his.seedbyte;
-            cfg->his.colcount = MCX_DEBUG_REC_LEN;
+            cfg->his.colcount = g.trajlen;
             cfg->his.savedphoton = cfg->debugdatalen;
             cfg->his.totalphoton = (unsigned int)cfg->nphoton;
             cfg->his.detected = 0;
@@ -1596,7 +1617,7 @@ static void noalloc_run(Config* cfg, const uint32_t* media, size_t nmedia, doubl
 
     if (cfg->issavedet || cfg->seed == SEED_FROM_FILE || MCX_IS_ADJOINT_TYPE(cfg->outputtype) ||
             (cfg->debuglevel & (MCX_DEBUG_RNG | MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) || cfg->muamus ||
-            cfg->mediabyte > 4 || cfg->omega > 0.f || !cfg->issave2pt) {
+            cfg->mediabyte > 4 || cfg->omega > 0.f || !cfg->issave2pt || cfg->polmedianum) {
         MCX_ERROR(-4, "mcx_run_simulation_noalloc runs forward fluence simulations only: no detected photons, "
                   "replay, adjoint outputs, RNG dump, trajectories, RF or per-voxel media, and the field must be saved");
     }

@@ -652,6 +652,31 @@ static int mcx_loadjson(MJNode* root, Config* cfg) {
             }
         }
 
+        /* polarised media (src/mcx_utils.c:2510-2566): {mua, radius, rho, nsph, nmed}
+           objects or 5-element arrays, one per medium from 1 */
+        MJNode* mies = jfind(root, Domain, "MieScatter", "Domain.MieScatter");
+
+        if (mies && mies->count > 0) {
+            const char* keys[5] = {"mua", "radius", "rho", "nsph", "nmed"};
+            unsigned int i = 0;
+            cfg->polmedianum = mies->count;
+            free(cfg->polprop);
+            cfg->polprop = (POLMedium*)calloc(cfg->polmedianum, sizeof(POLMedium));
+
+            for (MJNode* m = mies->child; m && i < cfg->polmedianum; m = m->next, i++) {
+                float* dst = &cfg->polprop[i].mua;
+
+                for (int k = 0; k < 5; k++) {
+                    MJNode* x = (m->type == MJ_OBJECT) ? mj_get(m, keys[k]) :
+                                ((m->type == MJ_ARRAY && m->count >= 5) ? mj_at(m, k) : NULL);
+
+                    if (x) {
+                        dst[k] = (float)mj_num(x, 0.0);
+                    }
+                }
+            }
+        }
+
         v = jfind(root, Domain, "Dim", "Domain.Dim");
 
         if (v && v->count >= 3) {
@@ -1473,6 +1498,55 @@ void mcx_maskdet(Config* cfg) {
 }
 
 /* ---------------- preprocessing (reference src/mcx_utils.c:1521-1809) ---------------- */
+/* scattering of the polarised media (src/mcx_utils.c:1483-1519): medium i+1 is
+   a suspension of spheres (radius r in um, n_sph) in a background (n_med, mua)
+   at wavelength lambda (nm).  Each gets its Mie table over NANGLES angles
+   (theta_k = k degrees) in cfg->smatrix, g from the series, and mus from the
+   particle density rho (1/um^3): mus = Qsca pi r^2 rho (1/mm).  A positive mus
+   given for the medium instead sets rho so that the medium keeps that mus, or
+   its reduced scattering mus (1 - g) when its g is below 1. */
+void mcx_prep_polarized(Config* cfg) {
+    double* mu = (double*)malloc(MCX_NANGLES * sizeof(double));
+
+    for (int i = 0; i < MCX_NANGLES; i++) {
+        mu[i] = cos(M_PI * i / (MCX_NANGLES - 1));
+    }
+
+    free(cfg->smatrix);
+    cfg->smatrix = (MCXFloat4*)malloc((size_t)cfg->polmedianum * MCX_NANGLES * sizeof(MCXFloat4));
+
+    for (unsigned int i = 0; i < cfg->polmedianum; i++) {
+        POLMedium* pm = cfg->polprop + i;
+        Medium* md = cfg->prop + i + 1;
+        double qsca = 0.0, g = 0.0;
+        md->mua = pm->mua;
+        md->n = pm->nmed;
+        const double x = 2.0 * M_PI * pm->r * pm->nmed / (cfg->lambda * 1e-3);
+        const double area = M_PI * pm->r * pm->r;
+
+        if (mcx_mie(x, pm->nsph / pm->nmed, MCX_NANGLES, mu, (float*)(cfg->smatrix + (size_t)i * MCX_NANGLES), &qsca,
+                    &g)) {
+            free(mu);
+            MCX_ERROR(-6, "sphere size parameter must lie in (0, 20000]");
+        }
+
+        if (md->mus > EPS) {
+            float target = md->mus;
+
+            if (md->g < 1.f - EPS) {
+                target = (float)(md->mus * (1.0f - md->g) / (1.0 - g));
+            }
+
+            pm->rho = (float)(target / qsca / area * 1e-3);
+        }
+
+        md->mus = (float)(qsca * area * pm->rho * 1e3);
+        md->g = (float)g;
+    }
+
+    free(mu);
+}
+
 void mcx_preprocess(Config* cfg) {
     int isbcdet = 0;
     double len = sqrt((double)cfg->srcdir.x * cfg->srcdir.x + (double)cfg->srcdir.y * cfg->srcdir.y +
@@ -1488,7 +1562,19 @@ void mcx_preprocess(Config* cfg) {
     cfg->srcdir.z *= inv;
 
     if (cfg->vol && cfg->polprop) {
-        MCX_ERROR(-1, "polarized photon simulation is not supported by this build");
+        if (!(cfg->mediabyte <= 4)) {
+            MCX_ERROR(-1, "Unsupported media format for polarized simulation");
+        }
+
+        if (cfg->medianum != cfg->polmedianum + 1) {
+            MCX_ERROR(-6, "number of particle types does not match number of media");
+        }
+
+        if (cfg->lambda == 0.f) {
+            MCX_ERROR(-1, "you must specify light wavelength lambda to run polarized photon simulation");
+        }
+
+        mcx_prep_polarized(cfg);
     }
 
     if (cfg->debuglevel & MCX_DEBUG_MOVE_ONLY) {
@@ -1748,7 +1834,13 @@ void mcx_preprocess(Config* cfg) {
         cfg->savedetflag |= 0x10 | 0x20;
     }
 
-    cfg->savedetflag &= ~0x80u;     /* no polarization in this build */
+    /* polarised runs record partial paths, exit directions, initial weights and
+       the Stokes vector of each detected photon (src/mcx_utils.c:1777-1784) */
+    if (cfg->polmedianum) {
+        cfg->savedetflag |= 0x4u | 0x20u | 0x40u | 0x80u;
+    } else {
+        cfg->savedetflag &= ~0x80u;
+    }
 
     if (cfg->issavedet && cfg->savedetflag == 0) {
         cfg->savedetflag = 0x5;

@@ -189,6 +189,16 @@ def _parse_config(user, c):
             raise ValueError("the 'prop' field must have 4 columns (mua,mus,g,n)")
         c.medianum = p.shape[0]
         c.prop = _carray(p, np.float32)
+    if "polprop" in user:
+        # polarised media (src/pmcxcl.cpp:754-785): one row per medium 1.., columns
+        # {mua, sphere radius (um), density (1/um^3), n_sphere, n_background}
+        pp = np.atleast_2d(np.asarray(user["polprop"], dtype=np.float32))
+        if pp.ndim != 2:
+            raise ValueError("the 'polprop' field must a 2D array")
+        if pp.shape[1] != 5:
+            raise ValueError("the 'polprop' field must have 5 columns (mua, radius, rho, n_sph,n_bkg)")
+        c.polmedianum = pp.shape[0]
+        c.polprop = _carray(pp, np.float32)
     if "session" in user:
         s = str(user["session"]).encode()
         if len(s) >= 64:
@@ -510,8 +520,10 @@ def _collect(c, user, ngate, ndet, nsrc):
     if (c.debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY)) and c.exportdebugdata and c.debugdatalen:
         # trajectory positions [6, N]: photon id (uint32 bits), x, y, z, weight, source id
         # (src/pmcxcl.cpp:1266-1283)
-        traj = np.ctypeslib.as_array((ctypes.c_float * (6 * c.debugdatalen)).from_address(c.exportdebugdata)).copy()
-        out["traj"] = np.asfortranarray(traj.reshape(c.debugdatalen, 6).T)
+        ts = c.istrajstokes
+        nrow = 6 + 4 * int(ts[0] if isinstance(ts, bytes) else ts)     # Stokes vectors follow with istrajstokes
+        traj = np.ctypeslib.as_array((ctypes.c_float * (nrow * c.debugdatalen)).from_address(c.exportdebugdata)).copy()
+        out["traj"] = np.asfortranarray(traj.reshape(c.debugdatalen, nrow).T)
     if c.issave2pt and c.exportfield:
         dims = [c.srcnum * c.dim.x, c.dim.y, c.dim.z, ngate]
         nrf = 2 if _is_complex(c) else 1
@@ -570,6 +582,12 @@ def _collect(c, user, ngate, ndet, nsrc):
         "runtimems": float(c.runtimems),
         "detected": int(c.his.detected),
     }
+    if c.polprop and c.polmedianum:
+        # the optical properties the polarised run used, per mm (src/pmcxcl.cpp:1511-1521)
+        prop = np.ctypeslib.as_array((ctypes.c_float * (4 * c.medianum)).from_address(c.prop)).copy()
+        prop = prop.reshape(c.medianum, 4)
+        prop[1:c.polmedianum + 1, :2] /= np.float32(c.unitinmm)
+        out["prop"] = np.asfortranarray(prop.T)
     return out
 
 

@@ -25,6 +25,8 @@
  *   pattern_weights       <- the pattern/pattern3d pixel weights (src/mcx_core.cl:1656-1766)
  *   save_traj             <- savedebugdata            (src/mcx_core.cl:929-944)
  *   (in launch_photon)    <- saveexitppath            (src/mcx_core.cl:852-865)
+ *   update_stokes         <- rotsphi + updatestokes   (src/mcx_core.cl:781-836)
+ *   scatter_angles        <- the polarised (theta, phi) rejection (src/mcx_core.cl:2454-2468)
  *   svmc_update           <- updateproperty_svmc      (src/mcx_core.cl:1231-1278)
  *   ray_plane             <- ray_plane_intersect      (src/mcx_core.cl:1280-1299)
  *   reflect_svmc          <- reflectray_svmc          (src/mcx_core.cl:1302-1345)
@@ -37,8 +39,8 @@
  * the reference's published absorbed fractions): tests/test_oracle_pins.py,
  * test_sampler_pins.py, test_output_pins.py, test_replay.py,
  * test_replay_outputs.py, test_media_formats.py, test_trajectory.py,
- * test_rf_forward.py, test_svmc.py, test_exit_tally.py, test_respin.py,
- * test_multisource.py, test_anchors.py.
+ * test_rf_forward.py, test_svmc.py, test_exit_tally.py, test_debug_modes.py,
+ * test_polarization.py, test_respin.py, test_multisource.py, test_anchors.py.
  *
  * Arithmetic: every transcendental goes through the deterministic FP32
  * functions of mcx_fpmath.h (shared with the kernel's parity mode and pinned
@@ -116,6 +118,8 @@ typedef double OTally;
 #endif
 
 #define K_EPS          FLT_EPSILON
+#define R_PI           0.318309886183791f
+#define NANGLES        MCX_NANGLES
 #define JUST_BELOW_ONE 0.9998f
 #define ROULETTE_SIZE  10.f
 #define MAX_ACCUM      1000.f
@@ -170,6 +174,10 @@ typedef struct {
     uint32_t savetraj, maxjumpdebug;  /* trajectory records (src/mcx_core.cl:929-944) */
     uint32_t svmc;                    /* split-voxel media: a second block of nvox plane words */
     uint32_t moveonly;                /* move-only trajectories (debug flag T) */
+    uint32_t maxpolmedia;             /* polarised media 1..maxpolmedia (Mie tables in smatrix) */
+    uint32_t istrajstokes, trajlen;   /* Stokes vectors in trajectory records; floats per record */
+    F4 s0;                            /* launch Stokes vector */
+    const F4* smatrix;                /* {S11, S12, S33, S34} x NANGLES per polarised medium */
     uint32_t* detcount;               /* detected photons over all threads (fill-max mode) */
     float* debugdata;                 /* maxjumpdebug records of 6 floats */
     uint32_t* jumpdebug;              /* records written, counted past maxjumpdebug */
@@ -649,6 +657,7 @@ typedef struct {
     uint32_t idx1d, mediaid;
     float w0, Lmove;
     OSplit sp;            /* the split voxel the photon is in */
+    F4 stokes;            /* polarised runs: the Stokes vector {I, Q, U, V} */
     /* per-thread detected-photon scratch (src/mcx_host.cpp:486-492 layout):
        [0, partialdata)   nscat (uint bits) / partial path / momentum per medium
        [partialdata]      escaped energy tally
@@ -731,6 +740,13 @@ static void save_detected(const OParam* g, OWork* w, OPhoton* s, uint32_t isdet)
         rec[k++] = s->pp[g->w0offset - 2].f;
     }
 
+    if (SAVE_IQUV(g->savedetflag)) {    /* src/mcx_core.cl:915-920 */
+        rec[k++] = s->stokes.x;
+        rec[k++] = s->stokes.y;
+        rec[k++] = s->stokes.z;
+        rec[k++] = s->stokes.w;
+    }
+
     w->ndet++;
 }
 
@@ -759,20 +775,69 @@ static inline uint32_t traj_index(const OParam* g, int idx, float fw) {
 }
 
 /* src/mcx_core.cl:929-944: {id as bits, x, y, z, w, (int) source id} */
-static void save_traj(const OParam* g, const F4* p, uint32_t id, float srcid) {
+/* savedebugdata / savedebugstokes (src/mcx_core.cl:929-959): records of 6 floats,
+   10 with istrajstokes, the Stokes vector appended in polarised runs */
+static void save_traj(const OParam* g, const F4* p, uint32_t id, float srcid, const F4* stokes) {
     uint32_t pos;
 #pragma omp atomic capture
     pos = (*g->jumpdebug)++;
 
     if (pos < g->maxjumpdebug) {
-        float* r = g->debugdata + (size_t)pos * 6;
+        float* r = g->debugdata + (size_t)pos * g->trajlen;
         memcpy(&r[0], &id, sizeof(id));
         r[1] = p->x;
         r[2] = p->y;
         r[3] = p->z;
         r[4] = p->w;
         r[5] = (float)(int)srcid;
+
+        if (g->maxpolmedia > 0 && g->istrajstokes) {
+            r[6] = stokes->x;
+            r[7] = stokes->y;
+            r[8] = stokes->z;
+            r[9] = stokes->w;
+        }
     }
+}
+
+/* the table entry of scattering angle theta: (uint)(theta NANGLES (1/pi - EPS)) */
+static inline const F4* mie_entry(const OParam* g, uint32_t mediaid, float theta) {
+    return g->smatrix + (size_t)NANGLES * ((mediaid & MED_MASK) - 1) +
+           (uint32_t)(theta * (float)NANGLES * (R_PI - K_EPS));
+}
+
+/* rotsphi + updatestokes (src/mcx_core.cl:781-836): into the scattering plane,
+   through the Mueller matrix of the angle, into the new meridian plane (u the
+   old direction, u2 the new one), normalised to I = 1 */
+static void update_stokes(const OParam* g, F4* st, float theta, float phi, const F4* u, const F4* u2,
+                          uint32_t mediaid) {
+    const float costheta = pm_cos(theta);
+    float sin2phi, cos2phi;
+    pm_sincos(2.f * phi, &sin2phi, &cos2phi);
+    F4 s2 = {st->x, st->y * cos2phi + st->z * sin2phi, -st->y * sin2phi + st->z * cos2phi, st->w};
+    const F4* m = mie_entry(g, mediaid, theta);
+    st->x = m->x * s2.x + m->y * s2.y;
+    st->y = m->y * s2.x + m->x * s2.y;
+    st->z = m->z * s2.z + m->w * s2.w;
+    st->w = -m->w * s2.z + m->z * s2.w;
+
+    float temp = (u2->z > -1.f && u2->z < 1.f) ? pm_rsqrt((1.f - costheta * costheta) * (1.f - u2->z * u2->z)) : 0.f;
+    float cosi = (temp == 0.f) ? 0.f : (((phi > ONE_PI && phi < TWO_PI) ? 1.f : -1.f) * (u2->z * costheta - u->z) * temp);
+    cosi = fmaxf(-1.f, fminf(cosi, 1.f));
+    const float sini = pm_sqrt(1.f - cosi * cosi);
+    const float cos22 = 2.f * cosi * cosi - 1.f;
+    const float sin22 = 2.f * sini * cosi;
+
+    s2.x = st->x;
+    s2.y = st->y * cos22 - st->z * sin22;
+    s2.z = st->y * sin22 + st->z * cos22;
+    s2.w = st->w;
+
+    temp = 1.f / s2.x;
+    st->y = s2.y * temp;
+    st->z = s2.z * temp;
+    st->w = s2.w * temp;
+    st->x = 1.f;
 }
 
 /* launch weights of a pattern pixel (src/mcx_core.cl:1678-1727): one pattern
@@ -811,7 +876,7 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
         s->pp[pd].f += s->p.w;
 
         if (g->savetraj) {
-            save_traj(g, &s->p, photon_index(g, idx, s->f.w), s->pp[g->w0offset - 1].f);
+            save_traj(g, &s->p, photon_index(g, idx, s->f.w), s->pp[g->w0offset - 1].f, &s->stokes);
         }
 
         if (s->mediaid == 0 && s->idx1d != OUTSIDE_VOLUME_MIN && s->idx1d != OUTSIDE_VOLUME_MAX && g->issaveref > 1 &&
@@ -921,6 +986,10 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
 
         if (g->svmc) {
             s->sp.sv = 0;    /* src/mcx_core.cl:1640-1642 */
+        }
+
+        if (g->maxpolmedia > 0) {
+            s->stokes = g->s0;    /* src/mcx_core.cl:1633-1638 */
         }
         F4 tmp = {lsrcpos.x, lsrcpos.y, lsrcpos.z, 0.f};
         s->prop = tmp;
@@ -1283,7 +1352,7 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
 
     if (g->savetraj) {
         /* ppath[3] of the shifted pointer: the source-id slot (src/mcx_core.cl:2247) */
-        save_traj(g, p, traj_index(g, idx, f->w), pp[3]);
+        save_traj(g, p, traj_index(g, idx, f->w), pp[3], &s->stokes);
     }
 
     pp[1] += p->w;
@@ -1312,10 +1381,32 @@ static int launch_photon(const OParam* g, OWork* w, OPhoton* s, int idx, uint32_
    inversion, isotropic for |g| <= EPS and after gscatter events (the
    similarity rule, :2484-2497) */
 static void scatter_angles(const OParam* g, OWork* w, OPhoton* s, float* stheta, float* ctheta, float* sphi,
-                           float* cphi) {
+                           float* cphi, float* theta_out, float* phi_out) {
     float theta, tmp0;
     *cphi = 1.f;
     *sphi = 0.f;
+
+    if (g->maxpolmedia > 0 && !g->is2d) {
+        /* polarised media (src/mcx_core.cl:2454-2468): (theta, phi) by rejection
+           against S11 I + S12 (Q cos 2phi + U sin 2phi), bounded by the forward entry */
+        float i0, ii, sin2phi, cos2phi;
+
+        do {
+            theta = pm_acos(2.f * rng_next(s->t, w) - 1.f);
+            tmp0 = TWO_PI * rng_next(s->t, w);
+            pm_sincos(2.f * tmp0, &sin2phi, &cos2phi);
+            const F4* m0 = mie_entry(g, s->mediaid, 0.f);
+            i0 = m0->x * s->stokes.x + m0->y * (s->stokes.y * cos2phi + s->stokes.z * sin2phi);
+            const F4* mt = mie_entry(g, s->mediaid, theta);
+            ii = mt->x * s->stokes.x + mt->y * (s->stokes.y * cos2phi + s->stokes.z * sin2phi);
+        } while (rng_next(s->t, w) * i0 >= ii);
+
+        pm_sincos(tmp0, sphi, cphi);
+        pm_sincos(theta, stheta, ctheta);
+        *theta_out = theta;
+        *phi_out = tmp0;
+        return;
+    }
 
     if (!g->is2d) {
         tmp0 = TWO_PI * rng_next(s->t, w);
@@ -1405,8 +1496,8 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
             f->x = rng_scatlen(s->t, w);
 
             if (v->w != K_EPS) {
-                float cphi, sphi, stheta, ctheta, tmp0;
-                scatter_angles(g, w, s, &stheta, &ctheta, &sphi, &cphi);
+                float cphi, sphi, stheta, ctheta, tmp0, theta = 0.f, phi = 0.f;
+                scatter_angles(g, w, s, &stheta, &ctheta, &sphi, &cphi, &theta, &phi);
 
                 /* split voxels tally the label of the photon's side (src/mcx_core.cl:2503-2540) */
                 const uint32_t tlab = g->svmc ? SV_CURLABEL(s->sp.sv) : (s->mediaid & MED_MASK);
@@ -1422,6 +1513,8 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                     }
                 }
 
+                const F4 olddir = *v;
+
                 if (g->is2d) {
                     rotatevector2d(v, (rng_next(s->t, w) > 0.5f ? stheta : -stheta), ctheta, (int)g->is2d);
                 } else {
@@ -1430,6 +1523,10 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
 
                 v->w += 1.f;
                 OCOUNT(w, nscat);
+
+                if (g->maxpolmedia > 0) {
+                    update_stokes(g, &s->stokes, theta, phi, &olddir, v, s->mediaid);   /* :2550-2565 */
+                }
 
                 if (g->outputtype == otWP || g->outputtype == otDCS || g->outputtype == otWPTOF ||
                         (g->seed == SEED_FROM_FILE && g->outputtype == otRFmus)) {
@@ -1465,7 +1562,7 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                 }
 
                 if (g->savetraj) {    /* src/mcx_core.cl:2625-2633 */
-                    save_traj(g, p, traj_index(g, idx, f->w), s->pp[g->w0offset - 1].f);
+                    save_traj(g, p, traj_index(g, idx, f->w), s->pp[g->w0offset - 1].f, &s->stokes);
                 }
             }
 
@@ -1935,6 +2032,11 @@ static void oracle_setup(Config* cfg, OParam* gp, F4** props_out, float** muamus
     gp->w0offset = gp->partialdata + 4;
     gp->mediaformat = cfg->mediabyte;
     gp->svmc = (cfg->mediabyte == MEDIA_2LABEL_SPLIT);
+    gp->maxpolmedia = cfg->polmedianum;
+    gp->istrajstokes = cfg->istrajstokes ? 1u : 0u;
+    gp->trajlen = MCX_DEBUG_REC_LEN + (gp->istrajstokes << 2);
+    memcpy(&gp->s0, &cfg->srciquv, sizeof(F4));
+    gp->smatrix = (const F4*)cfg->smatrix;
     gp->gscatter = cfg->gscatter;
     gp->srcnum = cfg->srcnum;
     gp->doreflect = (uint32_t)cfg->isreflect;
@@ -2066,7 +2168,7 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
     g.detcount = &ndetall;
     g.maxjumpdebug = cfg->maxjumpdebug;
     g.jumpdebug = &njump;
-    g.debugdata = g.savetraj ? (float*)calloc((size_t)cfg->maxjumpdebug * MCX_DEBUG_REC_LEN + 1, sizeof(float)) : NULL;
+    g.debugdata = g.savetraj ? (float*)calloc((size_t)cfg->maxjumpdebug * g.trajlen + 1, sizeof(float)) : NULL;
 
     double tic = now_sec();
 
@@ -2135,9 +2237,9 @@ int oracle_run(Config* cfg, unsigned int nthread, unsigned int threadoffset, uns
 
         if (nj) {
             cfg->exportdebugdata = (float*)realloc(cfg->exportdebugdata, ((size_t)cfg->debugdatalen + nj) *
-                                                   MCX_DEBUG_REC_LEN * sizeof(float));
-            memcpy(cfg->exportdebugdata + (size_t)cfg->debugdatalen * MCX_DEBUG_REC_LEN, g.debugdata,
-                   (size_t)nj * MCX_DEBUG_REC_LEN * sizeof(float));
+                                                   g.trajlen * sizeof(float));
+            memcpy(cfg->exportdebugdata + (size_t)cfg->debugdatalen * g.trajlen, g.debugdata,
+                   (size_t)nj * g.trajlen * sizeof(float));
             cfg->debugdatalen += nj;
         }
 
@@ -2502,10 +2604,10 @@ void oracle_scatter_samples(Config* cfg, unsigned int label, unsigned int nscat,
     s->prop = g.prop[label];
 
     for (unsigned int k = 0; k < n; k++) {
-        float stheta, ctheta, sphi, cphi;
+        float stheta, ctheta, sphi, cphi, theta, phi;
         F4 v0 = {0.6f, 0.f, 0.8f, (float)nscat};
         s->v = v0;
-        scatter_angles(&g, &w, s, &stheta, &ctheta, &sphi, &cphi);
+        scatter_angles(&g, &w, s, &stheta, &ctheta, &sphi, &cphi, &theta, &phi);
 
         if (g.is2d) {
             rotatevector2d(&s->v, (rng_next(s->t, &w) > 0.5f ? stheta : -stheta), ctheta, (int)g.is2d);
@@ -2516,6 +2618,49 @@ void oracle_scatter_samples(Config* cfg, unsigned int label, unsigned int nscat,
         out[(size_t)k * 3] = ctheta;
         out[(size_t)k * 3 + 1] = stheta;
         out[(size_t)k * 3 + 2] = s->v.x * v0.x + s->v.y * v0.y + s->v.z * v0.z;
+    }
+
+    free(s);
+    free(props);
+    free(muamus);
+}
+
+/* Polarised scattering check: n events in polarised medium `label` of a photon
+   travelling along (0.6, 0, 0.8) with Stokes vector stokes4 before each; 9
+   floats per event {theta, phi, new direction x, y, z, new I, Q, U, V}. */
+void oracle_pol_scatter_samples(Config* cfg, unsigned int label, const float* stokes4, unsigned int n, float* out) {
+    OParam g;
+    F4* props;
+    float* muamus;
+    oracle_setup(cfg, &g, &props, &muamus);
+    set_ieee_fpu();
+    OWork w;
+    memset(&w, 0, sizeof(w));
+    OPhoton* s = (OPhoton*)calloc(1, sizeof(OPhoton));
+    uint32_t seed4[4] = {0x2545F491u, 0x4F6CDD1Du, (uint32_t)cfg->seed, 0x9E3779B9u};
+    s->t[0] = (uint64_t)seed4[0] << 32 | seed4[1];
+    s->t[1] = (uint64_t)seed4[2] << 32 | seed4[3];
+    s->prop = g.prop[label];
+    s->mediaid = label;
+
+    for (unsigned int k = 0; k < n; k++) {
+        float stheta, ctheta, sphi, cphi, theta = 0.f, phi = 0.f;
+        F4 v0 = {0.6f, 0.f, 0.8f, 1.f};
+        s->v = v0;
+        memcpy(&s->stokes, stokes4, sizeof(F4));
+        scatter_angles(&g, &w, s, &stheta, &ctheta, &sphi, &cphi, &theta, &phi);
+        rotatevector(&s->v, stheta, ctheta, sphi, cphi);
+        update_stokes(&g, &s->stokes, theta, phi, &v0, &s->v, s->mediaid);
+        float* o = out + (size_t)k * 9;
+        o[0] = theta;
+        o[1] = phi;
+        o[2] = s->v.x;
+        o[3] = s->v.y;
+        o[4] = s->v.z;
+        o[5] = s->stokes.x;
+        o[6] = s->stokes.y;
+        o[7] = s->stokes.z;
+        o[8] = s->stokes.w;
     }
 
     free(s);

@@ -54,6 +54,8 @@ def load(path=ORACLE_SO):
     lib.oracle_result_sizeof.restype = ctypes.c_size_t
     lib.oracle_launch_samples.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p]
     lib.oracle_launch_samples.restype = ctypes.c_int
+    lib.oracle_pol_scatter_samples.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p, ctypes.c_uint,
+                                               ctypes.c_void_p]
     lib.oracle_scatter_samples.argtypes = [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
                                            ctypes.c_void_p]
     assert lib.oracle_result_sizeof() == ctypes.sizeof(OracleResult)
@@ -272,3 +274,17 @@ def scatter_samples(user, label, nscat, n):
     with pmcxcl.prepare(user) as pc:
         lib().oracle_scatter_samples(ctypes.addressof(pc.c), label, nscat, n, out.ctypes.data)
     return {"ctheta": out[:, 0], "stheta": out[:, 1], "cosdev": out[:, 2]}
+
+
+def pol_scatter_samples(user, label, stokes, n):
+    """n scattering events in polarised medium `label` from direction (0.6, 0, 0.8)
+    with the given Stokes vector: {"theta", "phi", "v" [n, 3], "stokes" [n, 4]}"""
+    out = np.zeros((n, 9), np.float32)
+    s4 = np.asarray(stokes, np.float32)
+    with pmcxcl.prepare(user) as pc:
+        lib().oracle_pol_scatter_samples(ctypes.addressof(pc.c), label, s4.ctypes.data, n, out.ctypes.data)
+        table = np.ctypeslib.as_array((ctypes.c_float * (pc.c.polmedianum * 181 * 4)).from_address(pc.c.smatrix))
+        table = table.copy().reshape(pc.c.polmedianum, 181, 4)
+        prop = np.ctypeslib.as_array((ctypes.c_float * (4 * pc.c.medianum)).from_address(pc.c.prop)).copy()
+    return {"theta": out[:, 0], "phi": out[:, 1], "v": out[:, 2:5], "stokes": out[:, 5:9], "smatrix": table,
+            "prop": prop.reshape(-1, 4)}
