@@ -1800,7 +1800,10 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                 } else if (fabsf(weight) > 0.f) {
                     deposit_voxel(g, sh, dcache, ox, oy, oz, idx1dold, tshift, weight);
 
-                    if (FULL && g.rfforward) {
+                    /* a pattern source's deposit carries no imaginary term in the
+                       reference (src/mcx_core.cl:2902-2914): its RF forward field
+                       stays real */
+                    if (FULL && g.rfforward && g.srctype != kPattern && g.srctype != kPattern3D) {
                         deposit(g, sh.repoff, idx1dold + (uint32_t)tshift * g.nvox + g.imoff, weight_im);
                     }
                 }

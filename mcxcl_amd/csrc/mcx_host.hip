@@ -249,12 +249,6 @@ void mcx_check_supported(const Config* cfg) {
         MCX_ERROR(-4, "replay, per-voxel mua/mus arrays and adjoint outputs need a label volume");
     }
 
-    /* RF forward (complex photon weights) with patterns sharing photons: the
-       complex field has no per-pattern layout here */
-    if (cfg->omega > 0.f && cfg->seed != SEED_FROM_FILE && cfg->srcnum > 1) {
-        MCX_ERROR(-4, "RF forward simulations (omega > 0) take no patterns sharing photons (srcnum > 1)");
-    }
-
     if (cfg->mediabyte == MEDIA_LABEL_HALF && (cfg->savedetflag & 0xEu)) {
         MCX_ERROR(-4, "scattering counts, partial path lengths and momentum transfer of detected photons need a "
                   "label volume (labelplus words carry a replaced property)");

@@ -1703,7 +1703,9 @@ static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed
                 } else if (pm_fabs(weight) > 0.f) {
                     deposit(g, w, idx1dold + (uint32_t)tshift * g->nvox, weight);
 
-                    if (g->rfforward) {    /* the imaginary part, gated by the real one (:2884-2893) */
+                    /* the imaginary part, gated by the real one (:2884-2893); a pattern
+                       source's deposit has none (:2902-2914) */
+                    if (g->rfforward && g->srctype != MCX_SRC_PATTERN && g->srctype != MCX_SRC_PATTERN3D) {
                         deposit(g, w, idx1dold + (uint32_t)tshift * g->nvox + g->fieldlen, weight_im);
                     }
                 }

@@ -79,3 +79,19 @@ def test_complex_adjoint_device_parity(outputtype):
         scale = np.abs(ref[k]).max()
         assert scale > 0 and np.abs(ref[k].imag).max() > 0
         np.testing.assert_allclose(gpu[k], ref[k], rtol=1e-5, atol=1e-6 * scale)
+
+
+def test_rf_forward_shared_patterns_parity():
+    """patterns sharing photons with complex weights: real deposits per pattern
+    slot, no imaginary term (src/mcx_core.cl:2902-2914)"""
+    pats = np.random.default_rng(6).uniform(0.2, 1.0, (2, 4, 3)).astype(np.float32)
+    cfg = workloads.sphere_cube_workload(nphoton=30000, tstep=1e-9, srctype="pattern", srcpos=[10, 15, 0],
+                                         srcparam1=[40, 0, 0, 4], srcparam2=[0, 30, 0, 3], srcnum=2,
+                                         srcpattern=pats)
+    cfg = dict(cfg, autopilot=0, nthread=NT, nblocksize=64, mathmode=1, rawfield=1, omega=OMEGA)
+    gpu, ref = mcxcl_amd.run(cfg), oracle.run(cfg, nthread=NT)
+    assert gpu["stat"]["energyabs"] == ref["stat"]["energyabs"]
+    raw = ref["flux64"].ravel(order="F")
+    assert np.abs(raw).max() > 0 and not raw[raw.size // 2:].any()
+    np.testing.assert_allclose(gpu["flux64"], ref["flux64"], rtol=1e-9, atol=1e-12 * np.abs(raw).max())
+    np.testing.assert_allclose(gpu["flux"], ref["flux"], rtol=1e-6, atol=1e-9 * np.abs(ref["flux"]).max())

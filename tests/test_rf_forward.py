@@ -69,13 +69,33 @@ def test_complex_output_is_the_normalised_grid():
     assert (flux.imag <= 1e-3).all() and flux.imag.min() < -1e-4     # the phase lags
 
 
-def test_rf_forward_restrictions():
-    cfg = workloads.homogeneous_cube_workload(nphoton=100, srctype="pattern", srcpos=[10, 15, 0],
-                                              srcparam1=[40, 0, 0, 4], srcparam2=[0, 30, 0, 3], srcnum=2,
-                                              srcpattern=np.ones((2, 4, 3), np.float32))
-    with pytest.raises(RuntimeError, match="RF forward"):
-        # patterns sharing photons keep one slot per pattern: not built for complex fields
-        mcxcl_amd.run(dict(cfg, omega=1e8))
+def test_rf_forward_pattern_sources_stay_real():
+    """a pattern source's deposit has no imaginary term (src/mcx_core.cl:2902-2914):
+    the imaginary half stays zero, and with patterns sharing photons slot k is the
+    single-pattern run of pattern k"""
+    rng = np.random.default_rng(4)
+    pa = rng.uniform(0.2, 1.0, (4, 3)).astype(np.float32)
+    pb = rng.uniform(0.2, 1.0, (4, 3)).astype(np.float32)
+
+    def case(pats):
+        cfg = workloads.homogeneous_cube_workload(nphoton=20000, srctype="pattern", srcpos=[10, 15, 0],
+                                                  srcparam1=[40, 0, 0, 4], srcparam2=[0, 30, 0, 3], tstep=5e-9)
+        if len(pats) > 1:
+            cfg.update(srcnum=len(pats), srcpattern=np.stack(pats))
+        else:
+            cfg.update(srcpattern=pats[0])
+        return _run(cfg, omega=2 * np.pi * 100e6)
+
+    both = case([pa, pb])
+    raw = both["flux64"].ravel(order="F")
+    half = raw.size // 2
+    assert not raw[half:].any() and np.abs(raw[:half]).max() > 0
+    for k, pat in enumerate((pa, pb)):
+        one = case([pat])
+        ref = one["flux64"].ravel(order="F")
+        assert not ref[ref.size // 2:].any()
+        np.testing.assert_allclose(raw[:half][k::2], ref[:ref.size // 2], rtol=0,
+                                   atol=1e-5 * np.abs(ref).max())
 
 
 def _adjoint_case(outputtype, omega, nphoton=40000):
