@@ -1522,7 +1522,7 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
             if (v.w != K_EPS) {
                 float cphi = 1.f, sphi = 0.f, theta = 0.f, stheta, ctheta, tmp0 = 0.f;
 
-                if (FULL && g.maxpolmedia) {
+                if (FULL && g.maxpolmedia && !g.is2d) {
                     /* polarised media (src/mcx_core.cl:2454-2468): (theta, phi) by rejection
                        against I(theta, phi) = S11 I + S12 (Q cos 2phi + U sin 2phi) of the
                        medium's table, the forward entry as the bound */
@@ -1556,6 +1556,10 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                     tmp0 = (1.f - theta) * sh.invcdf[i0] + theta * sh.invcdf[i1];
                     stheta = ksin_acos(tmp0);
                     ctheta = tmp0;
+
+                    if (FULL && g.maxpolmedia) {
+                        theta = kacos(tmp0);    /* the angle the Stokes update takes (2D polarised runs) */
+                    }
                 } else {
                     tmp0 = (v.w > (float)g.gscatter) ? 0.f : s.prop.z;
 
@@ -1566,6 +1570,10 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                         tmp0 = fmaxf(-1.f, fminf(1.f, tmp0));
                         stheta = ksin_acos(tmp0);
                         ctheta = tmp0;
+
+                        if (FULL && g.maxpolmedia) {
+                            theta = kacos(tmp0);
+                        }
                     } else {
 #if MCX_PARITY
                         theta = kacos(2.f * rng.next() - 1.f);
@@ -1573,6 +1581,10 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
 #else
                         ctheta = 2.f * rng.next() - 1.f;
                         stheta = ksin_acos(ctheta);
+
+                        if (FULL && g.maxpolmedia) {
+                            theta = kacos(ctheta);
+                        }
 #endif
                     }
                 }
@@ -1604,7 +1616,9 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                 v.w += 1.f;
 
                 if (FULL && g.maxpolmedia) {
-                    update_stokes(g, s.stokes, theta, tmp0, olddir, v, s.mediaid);   /* :2550-2565 */
+                    /* :2550-2565; in 2D the reference samples by HG with the medium's g and
+                       passes the sampled cosine (tmp0) where the azimuth goes */
+                    update_stokes(g, s.stokes, theta, tmp0, olddir, v, s.mediaid);
                 }
 
                 if (!SIMPLE && (g.outputtype == kWP || g.outputtype == kDCS ||

@@ -274,12 +274,6 @@ void mcx_check_supported(const Config* cfg) {
         MCX_ERROR(-4, "photon sharing (srcnum > 1) needs a pattern or pattern3d source");
     }
 
-    /* polarised runs scatter by the Mie tables in 3D only: the reference's 2D branch
-       updates the Stokes vector with the cosine it sampled in place of the azimuth */
-    if (cfg->polmedianum && (cfg->dim.x == 1 || cfg->dim.y == 1 || cfg->dim.z == 1)) {
-        MCX_ERROR(-4, "polarized simulations need a 3D domain");
-    }
-
     if (cfg->polmedianum && !cfg->smatrix) {
         MCX_ERROR(-4, "polarized media need their Mie tables (mcx_prep_polarized)");
     }

@@ -1438,6 +1438,11 @@ static void scatter_angles(const OParam* g, OWork* w, OPhoton* s, float* stheta,
             pm_sincos(theta, stheta, ctheta);
         }
     }
+
+    /* 2D polarised runs update the Stokes vector with this angle and, where the
+       azimuth goes, the sampled cosine (src/mcx_core.cl:2550-2565) */
+    *theta_out = theta;
+    *phi_out = tmp0;
 }
 
 static void run_thread(const OParam* g, OWork* w, int idx, const uint32_t* nseed, float* genergy) {

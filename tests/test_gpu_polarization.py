@@ -53,3 +53,16 @@ def test_polarised_fast_build():
     assert abs(af - ap) <= 6 * np.sqrt(2 * ap * (1 - ap) / n), (af, ap)
     qf, qp = fast["detp"][-3], slow["detp"][-3]
     assert abs(qf.mean() - qp.mean()) < 6 * np.sqrt(qf.var() / qf.size + qp.var() / qp.size)
+
+
+def test_polarised_2d_parity():
+    """a 2D domain: HG with the Mie g, the Stokes vector updated with the sampled
+    cosine in the azimuth's place, as the reference's 2D branch does"""
+    cfg = dict(_cfg(20000, vol=np.ones((1, 40, 40), np.uint8), srcpos=[0.5, 20, 0], detpos=[[0.5, 20, 0, 8]],
+                    issrcfrom0=1),
+               autopilot=0, nthread=NT, nblocksize=64, mathmode=1, rawfield=1)
+    gpu, ref = mcxcl_amd.run(cfg), oracle.run(cfg, nthread=NT)
+    assert ref["detp"].shape[1] > 100
+    assert gpu["stat"]["energyabs"] == ref["stat"]["energyabs"]
+    np.testing.assert_allclose(gpu["flux64"], ref["flux64"], rtol=1e-9, atol=1e-12)
+    assert np.array_equal(sorted_records(gpu["detp"]), sorted_records(ref["detp"]))

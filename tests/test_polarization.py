@@ -206,3 +206,21 @@ def test_detected_stokes_columns():
     # fully polarised at launch, and single scattering by a sphere keeps it so:
     # degree 1 up to the rounding accumulated over the photon's scatterings
     np.testing.assert_allclose(np.sum(iquv[1:] ** 2, axis=0), 1.0, atol=1e-3)
+
+
+def test_2d_polarised_runs_scatter_by_hg_with_the_mie_g():
+    """a 2D domain takes the reference's 2D branch: HG with the medium's g (from
+    the Mie series), the Stokes vector carried along without touching the paths,
+    so the fluence is the unpolarised run's with the prepared properties, bit
+    for bit; detected photons still carry their Stokes columns"""
+    vol = np.ones((1, 40, 40), np.uint8)
+    base = dict(nphoton=4000, vol=vol, srcpos=[0.5, 20, 0], srcdir=[0, 0, 1], tstart=0, tend=5e-9, tstep=5e-9,
+                seed=5, autopilot=0, nthread=128, nblocksize=64, isreflect=0, issrcfrom0=1)
+    pol = oracle.run(dict(base, prop=[[0, 0, 1, 1], [0.005, 0, 0, 1.33]], polprop=[[0.005, 0.3, 0.002, 1.59, 1.33]],
+                          **{"lambda": 632.8}, srciquv=[1, 1, 0, 0]), nthread=128)
+    prop = pol["prop"].T.copy()
+    prop[0] = [0, 0, 1, 1]
+    plain = oracle.run(dict(base, prop=prop), nthread=128)
+    assert pol["stat"]["energyabs"] > 100
+    assert np.array_equal(pol["flux64"], plain["flux64"])
+    assert pol["stat"]["energyabs"] == plain["stat"]["energyabs"]
