@@ -31,3 +31,18 @@ def test_exit_tally_parity():
     assert (r[0] > 0).sum() > 500                          # voxels of the background layer
     np.testing.assert_allclose(d, r, rtol=2e-5, atol=1e-6 * np.abs(r).max())
     assert not r[2].any() and not d[2].any()               # the last column is never written
+
+
+def test_exit_tally_fast_build():
+    """fast arithmetic: the per-voxel exit weights sum to the bit-reproducible
+    build's within Monte Carlo error"""
+    vol = workloads.sphere_volume().copy()
+    vol[:, :, 0] = 0
+    cfg = workloads.sphere_cube_workload(nphoton=400000, tstep=1e-9, vol=vol, issaveref=2)
+    fast = mcxcl_amd.run(dict(cfg, mathmode=0))
+    slow = mcxcl_amd.run(dict(cfg, mathmode=1))
+    sf, sp = (float(o["detp"][0].astype(np.float64).sum()) for o in (fast, slow))
+    n = cfg["nphoton"]
+    # column 0 holds w (1 + L1): its total is a sum over escaping photons, spread about its mean
+    assert abs(sf / sp - 1) < 0.02, (sf, sp)
+    assert fast["stat"]["energytot"] == slow["stat"]["energytot"] == n
