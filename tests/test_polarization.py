@@ -224,3 +224,29 @@ def test_2d_polarised_runs_scatter_by_hg_with_the_mie_g():
     assert pol["stat"]["energyabs"] > 100
     assert np.array_equal(pol["flux64"], plain["flux64"])
     assert pol["stat"]["energyabs"] == plain["stat"]["energyabs"]
+
+
+def test_json_input_mie_scatter(tmp_path):
+    """Domain.MieScatter (objects or 5-element arrays) and Optode.Source.WaveLength
+    (src/mcx_utils.c:2510-2566, :2948-2949) reach the same preparation as the binding"""
+    import json
+    from test_config import _readconfig
+    (tmp_path / "vol.bin").write_bytes(np.ones((12, 12, 12), np.uint8).tobytes())
+    cfg = {"Session": {"ID": "pol", "Photons": 1000, "Seed": 3},
+           "Domain": {"VolumeFile": "vol.bin", "Dim": [12, 12, 12], "OriginType": 1,
+                      "Media": [{"mua": 0, "mus": 0, "g": 1, "n": 1}, {"mua": 0.005, "mus": 0, "g": 0, "n": 1.0},
+                                {"mua": 0.005, "mus": 0, "g": 0, "n": 1.0}],
+                      "MieScatter": [{"mua": 0.005, "radius": 0.3, "rho": 0.002, "nsph": 1.59, "nmed": 1.33},
+                                     [0.01, 0.5, 0.001, 1.5, 1.33]]},
+           "Forward": {"T0": 0, "T1": 5e-9, "Dt": 5e-9},
+           "Optode": {"Source": {"Pos": [6, 6, 0], "Dir": [0, 0, 1], "WaveLength": 632.8}}}
+    (tmp_path / "pol.json").write_text(json.dumps(cfg))
+    c = _readconfig(str(tmp_path / "pol.json"))
+    assert c.polmedianum == 2 and getattr(c, "lambda") == np.float32(632.8)
+    prop = np.ctypeslib.as_array((ctypes.c_float * 12).from_address(c.prop)).reshape(3, 4)
+    assert prop[2][0] == np.float32(0.01) and prop[2][3] == np.float32(1.33)
+    r = oracle.pol_scatter_samples(_cfg(), 1, [1, 0, 0, 0], 1)
+    np.testing.assert_array_equal(prop[1], r["prop"][1])     # the same medium as the binding's run
+    table = np.ctypeslib.as_array((ctypes.c_float * (2 * 181 * 4)).from_address(c.smatrix)).reshape(2, 181, 4)
+    np.testing.assert_array_equal(table[0], r["smatrix"][0])
+    lib.mcx_clearcfg(ctypes.byref(c))
