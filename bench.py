@@ -315,6 +315,58 @@ def phiphi_leg(photons, nrep=3):
     }
 
 
+def features_leg(photons):
+    """Fast-build throughput of the full general path's features on the 60^3
+    sphere-in-cube geometry of the headline workload (same properties, 10
+    gates, 4 detectors, reflection): the plain label run for reference, packed
+    labelplus words, the ball as split voxels (svmc), RF forward (150 MHz), two
+    patterns sharing photons, and a polarised cube (0.3 um spheres in water,
+    632.8 nm) with the same pencil beam.  kernelms of one run each, after one
+    warm-up run."""
+    import mcxcl_amd
+    from mcxcl_amd import workloads
+
+    n = int(photons)
+    base = workloads.sphere_cube_workload(nphoton=n, tstep=5e-10)
+    vol = np.asarray(base["vol"])
+    prop = np.asarray(base["prop"], np.float32)
+    lplus = np.stack([prop[vol, 0], np.zeros(vol.shape, np.float32), vol.astype(np.float32)])
+    q = np.moveaxis(np.indices(vol.shape), 0, -1) + 0.5
+    d = np.linalg.norm(q - 30.0, axis=-1)
+    sv = np.zeros((8,) + vol.shape, np.uint8)
+    sv[0] = np.where(d < 15, 2, 1)
+    near = np.abs(d - 15) < 0.9
+    nhat = (q - 30.0) / np.maximum(d, 1e-9)[..., None]
+    off = 30.0 + 15 * nhat - np.moveaxis(np.indices(vol.shape), 0, -1)
+    ok = near & np.all((off >= 0) & (off <= 1), axis=-1)
+    sv[0][ok], sv[1][ok] = 2, 1                               # lower: inside the ball, upper: outside
+    pb = np.clip(np.rint(off * 255), 0, 255).astype(np.uint8)
+    nb = np.clip(np.rint((nhat + 1) * 127.5), 0, 255).astype(np.uint8)
+    for k in range(3):
+        sv[2 + k][ok], sv[5 + k][ok] = pb[..., k][ok], nb[..., k][ok]
+    pats = np.random.default_rng(1).uniform(0.2, 1.0, (2, 60, 60)).astype(np.float32)
+    cases = {
+        "label": {},
+        "labelplus": {"vol": lplus, "savedetflag": "DX"},
+        "svmc": {"vol": sv},
+        "rf_forward": {"omega": 2 * np.pi * 150e6},
+        "shared_patterns": {"srctype": "pattern", "srcpos": [0, 0, 0], "srcparam1": [60, 0, 0, 60],
+                            "srcparam2": [0, 60, 0, 60], "srcnum": 2, "srcpattern": pats},
+        "polarised": {"vol": np.ones(vol.shape, np.uint8), "prop": [[0, 0, 1, 1], [0.005, 0, 0, 1.33]],
+                      "polprop": [[0.005, 0.3, 0.002, 1.59, 1.33]], "lambda": 632.8, "srciquv": [1, 1, 0, 0]},
+    }
+    out = {}
+    for name, extra in cases.items():
+        cfg = dict(base, **extra)
+        mcxcl_amd.run(dict(cfg, nphoton=max(n // 10, 1)))
+        r = mcxcl_amd.run(cfg)
+        ms = float(r["stat"]["kernelms"])
+        out[name] = {"kernel_ms": round(ms, 3), "photons_per_ms": round(n / ms, 1) if ms > 0 else None}
+    out["workload"] = ("60^3 sphere-in-cube (r 15, 10 gates, 4 detectors, reflection), %d photons per case, "
+                       "fast build; polarised: a 60^3 Mie suspension, same beam" % n)
+    return out
+
+
 def batch_leg(ninst, steps, count_photons=200):
     """Many independent simulations in one persistent launch: ninst 64^3 volumes
     with a random ellipsoid each, per-instance media, log-uniform photon counts
@@ -407,6 +459,8 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=256,
                     help="simulations of the batched-launch leg (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--feature-photons", type=float, default=1e7,
+                    help="photons per case of the full-path feature throughput leg (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -535,6 +589,8 @@ def main():
         res["phiphi"] = phiphi_leg(args.phiphi_photons)
     if args.batch_instances > 0:
         res["batch"] = batch_leg(args.batch_instances, args.batch_steps, 0 if args.no_cpu else 200)
+    if args.feature_photons > 0:
+        res["features"] = features_leg(args.feature_photons)
     print(json.dumps(res), flush=True)
 
 

@@ -82,6 +82,8 @@ struct MCXKParam {
     uint32_t trajlen;             /* floats per trajectory record */
     float4 s0;                    /* launch Stokes vector {I, Q, U, V} */
     const float4* smatrix;        /* {S11, S12, S33, S34} at 181 angles (0..180 degrees) per polarised medium */
+    const float* polcdf;          /* fast build: the 182 cos(theta) bin edges, then per polarised medium the
+                                     182-entry CDF of the angle bins weighted by S11 (theta's marginal) */
     unsigned char bc[12];
 
     const uint32_t* media;        /* label volume, x fastest; bit 31 marks detector-adjacent voxels */

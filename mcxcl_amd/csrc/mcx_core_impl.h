@@ -1528,7 +1528,9 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                        medium's table, the forward entry as the bound */
                     const uint32_t lab = s.mediaid & K_MED_MASK;
                     const uint32_t ipol = K_NANGLES * (lab ? lab - 1 : 0u);   /* scattering needs mus > 0: label >= 1 */
-                    float i0, ii, sin2phi, cos2phi;
+                    float ii, sin2phi, cos2phi;
+#if MCX_PARITY
+                    float i0;
 
                     do {
                         theta = kacos(2.f * rng.next() - 1.f);
@@ -1540,8 +1542,42 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                         ii = mt.x * s.stokes.x + mt.y * (s.stokes.y * cos2phi + s.stokes.z * sin2phi);
                     } while (rng.next() * i0 >= ii);
 
-                    ksincos(tmp0, &sphi, &cphi);
                     ksincos(theta, &stheta, &ctheta);
+#else
+                    /* the same distribution in fewer draws: theta from its marginal, S11 of
+                       its table bin times sin(theta) (a CDF over the bins, uniform in
+                       cos(theta) inside one), then phi given theta by rejection against
+                       S11 I + |S12| |(Q, U)|, which accepts at least half the draws (the
+                       joint rejection above takes ~17 rounds for 0.3 um spheres) */
+                    const float* cdf = g.polcdf + (K_NANGLES + 1) * (1u + (lab ? lab - 1 : 0u));
+                    const float u = rng.next();
+                    uint32_t lo = 0, hi = K_NANGLES;
+
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+
+                        if (cdf[mid] <= u) {
+                            lo = mid;
+                        } else {
+                            hi = mid;
+                        }
+                    }
+
+                    const float ce0 = g.polcdf[lo], ce1 = g.polcdf[lo + 1];
+                    ctheta = ce1 + (ce0 - ce1) * rng.next();
+                    theta = kacos(ctheta);
+                    stheta = ksin_acos(ctheta);
+                    const float4 mt = g.smatrix[ipol + lo];
+                    const float bound = mt.x * s.stokes.x +
+                                        fabsf(mt.y) * ksqrt(s.stokes.y * s.stokes.y + s.stokes.z * s.stokes.z);
+
+                    do {
+                        tmp0 = K_TWO_PI * rng.next();
+                        ksincos(2.f * tmp0, &sin2phi, &cos2phi);
+                        ii = mt.x * s.stokes.x + mt.y * (s.stokes.y * cos2phi + s.stokes.z * sin2phi);
+                    } while (rng.next() * bound >= ii);
+#endif
+                    ksincos(tmp0, &sphi, &cphi);
                 } else {
                 if (SIMPLE || !g.is2d) {
                     tmp0 = K_TWO_PI * rng.next();

@@ -14,6 +14,7 @@
  * All time gates are simulated in a single launch (288 GB of HBM removes the
  * reference's maxgate windowing, src/mcx_host.cpp:628-632, :1080).
  */
+#include <cfloat>
 #include <math.h>
 #include <setjmp.h>
 #include <stdio.h>
@@ -276,6 +277,17 @@ void mcx_check_supported(const Config* cfg) {
 
     if (cfg->polmedianum && !cfg->smatrix) {
         MCX_ERROR(-4, "polarized media need their Mie tables (mcx_prep_polarized)");
+    }
+
+    /* the scattering angles are drawn by rejection against I and the linear part
+       of the Stokes vector: an intensity of 0 or a degree above 1 can keep the
+       draws from ever being accepted */
+    if (cfg->polmedianum) {
+        const MCXFloat4 s0 = cfg->srciquv;
+
+        if (!(s0.x > 0.f) || s0.y * s0.y + s0.z * s0.z + s0.w * s0.w > s0.x * s0.x * (1.f + 1e-6f)) {
+            MCX_ERROR(-4, "the source Stokes vector (srciquv) needs I > 0 and a degree of polarization of at most 1");
+        }
     }
 
     /* the exit tally indexes the detected-photon buffer by voxel (src/mcx_utils.c:1797-1806
@@ -835,6 +847,8 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     uint32_t* dmedia = NULL;
     float4* dprop = NULL;
     float4* dsmatrix = NULL;
+    float* dpolcdf = NULL;
+    float* hpolcdf = NULL;
     float4* ddetpos = NULL;
     float* dinvcdf = NULL;
     float* dangle = NULL;
@@ -879,6 +893,38 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
         HIP_TRY(dalloc(bufs, &dsmatrix, (size_t)cfg->polmedianum * MCX_NANGLES));
         HIP_TRY(hipMemcpyAsync(dsmatrix, cfg->smatrix, (size_t)cfg->polmedianum * MCX_NANGLES * sizeof(float4),
                                hipMemcpyHostToDevice, stream));
+
+        /* the fast build's angle sampler: theta's marginal S11(bin) sin(theta) as a CDF over
+           the table's bins (bin b = floor(theta 181 (1/pi - EPS)), uniform in cos(theta)
+           inside a bin) */
+        const size_t ne = MCX_NANGLES + 1;
+        hpolcdf = (float*)malloc(ne * (1 + cfg->polmedianum) * sizeof(float));
+        const double width = 1.0 / (MCX_NANGLES * (1.0 / M_PI - (double)FLT_EPSILON));
+
+        for (size_t b = 0; b < ne; b++) {
+            hpolcdf[b] = (float)cos(fmin(b * width, M_PI));
+        }
+
+        for (unsigned int m = 0; m < cfg->polmedianum; m++) {
+            float* cdf = hpolcdf + ne * (1 + m);
+            double acc = 0.0;
+            cdf[0] = 0.f;
+
+            for (size_t b = 0; b < MCX_NANGLES; b++) {
+                acc += (double)cfg->smatrix[m * MCX_NANGLES + b].x * ((double)hpolcdf[b] - (double)hpolcdf[b + 1]);
+                cdf[b + 1] = (float)acc;
+            }
+
+            for (size_t b = 1; b < ne; b++) {
+                cdf[b] = (float)(cdf[b] / acc);
+            }
+
+            cdf[MCX_NANGLES] = 1.f;
+        }
+
+        HIP_TRY(dalloc(bufs, &dpolcdf, ne * (1 + cfg->polmedianum)));
+        HIP_TRY(hipMemcpyAsync(dpolcdf, hpolcdf, ne * (1 + cfg->polmedianum) * sizeof(float), hipMemcpyHostToDevice,
+                               stream));
     }
     HIP_TRY(hipMemcpyAsync(dprop, cfg->prop, cfg->medianum * sizeof(float4), hipMemcpyHostToDevice, stream));
 
@@ -988,6 +1034,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     g.media = dmedia;
     g.prop = dprop;
     g.smatrix = dsmatrix;
+    g.polcdf = dpolcdf;
     g.detpos = ddetpos;
     g.invcdf = dinvcdf;
     g.angleinvcdf = dangle;
@@ -1361,6 +1408,7 @@ extern "C" void mcx_run_simulation(Config* cfg, float* fluence, float* totalener
     free(hseed);
     free(hgenergy);
     free(hfield);
+    free(hpolcdf);
     bufs.release();
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
@@ -1372,6 +1420,7 @@ fail:
     free(hseed);
     free(hgenergy);
     free(hfield);
+    free(hpolcdf);
     bufs.release();
 
     if (ev0) {

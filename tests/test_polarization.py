@@ -250,3 +250,10 @@ def test_json_input_mie_scatter(tmp_path):
     table = np.ctypeslib.as_array((ctypes.c_float * (2 * 181 * 4)).from_address(c.smatrix)).reshape(2, 181, 4)
     np.testing.assert_array_equal(table[0], r["smatrix"][0])
     lib.mcx_clearcfg(ctypes.byref(c))
+
+
+def test_unphysical_source_stokes_vectors_are_rejected():
+    import mcxcl_amd
+    for s0 in ([0, 0, 0, 0], [1, 1, 1, 0], [-1, 0, 0, 0]):
+        with pytest.raises(RuntimeError, match="srciquv"):
+            mcxcl_amd.run(_cfg(srciquv=s0))
