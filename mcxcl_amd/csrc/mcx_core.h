@@ -52,8 +52,11 @@ struct MCXKParam {
     /* deposit privatisation: the box [x0,x0+nx)x[y0,y0+ny)x[z0,z0+nz) of the first ng
        gates is accumulated in LDS per workgroup (float64 cells in the
        bit-reproducible build, float32 in the fast build) and flushed once; ng=0
-       disables */
+       disables.  RF forward runs keep a second set of ng box gates after the first
+       for the imaginary half: box gate ng + t flushes to field gate box_imgate + t
+       (box_imgate = imoff / nvox; 0 when the box holds no imaginary half) */
     uint32_t box_x0, box_y0, box_z0, box_nx, box_ny, box_nz, box_ng, box_cells;
+    uint32_t box_imgate;
     uint32_t nrep;                /* global accumulator replicas (power of two) */
     uint32_t npattern;            /* patterns sharing each photon (a pattern source with srcnum > 1), else 1;
                                      their slots interleave after the voxel: (vox + gate*nvox)*npattern + k */

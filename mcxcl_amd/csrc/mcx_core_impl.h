@@ -806,6 +806,33 @@ __device__ __forceinline__ void deposit(const MCXKParam& g, size_t repoff, uint3
     atomicAdd(g.field + repoff + addr, (double)val);
 }
 
+/* the imaginary half of an RF forward deposit: into the box's second set of
+   gates when the voxel and gate fall inside it (same cell rules as
+   deposit_voxel), else straight to the grid */
+__device__ __forceinline__ void deposit_voxel_im(const MCXKParam& g, const Shared& sh, int ox, int oy, int oz,
+        uint32_t idx1d, int tshift, float val) {
+    const uint32_t dx = (uint32_t)(ox - (int)g.box_x0), dy = (uint32_t)(oy - (int)g.box_y0);
+    const uint32_t dz = (uint32_t)(oz - (int)g.box_z0);
+    const uint32_t addr = idx1d + (uint32_t)tshift * g.nvox + g.imoff;
+
+    if (g.box_imgate && dx < g.box_nx && dy < g.box_ny && dz < g.box_nz && (uint32_t)tshift < g.box_ng) {
+        BoxCell* cell = sh.box + ((((uint32_t)tshift + g.box_ng) * g.box_nz + dz) * g.box_ny + dy) * g.box_nx + dx;
+#if MCX_PARITY
+        atomicAdd(cell, (BoxCell)val);
+#else
+        const float old = atomicAdd(cell, val);
+
+        if (fabsf(old) > K_BOX_MOVE) {
+            const float mv = old > 0.f ? K_BOX_MOVE : -K_BOX_MOVE;
+            atomicAdd(cell, -mv);
+            atomicAdd(g.field + sh.repoff + addr, (double)mv);
+        }
+#endif
+    } else {
+        deposit(g, sh.repoff, addr, val);
+    }
+}
+
 /* src/mcx_core.cl:838-926 */
 template <bool SAVEDET, bool FULL>
 __device__ void save_detected(const MCXKParam& g, const Shared& sh, const Photon& s, uint32_t isdet) {
@@ -1854,7 +1881,7 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                        reference (src/mcx_core.cl:2902-2914): its RF forward field
                        stays real */
                     if (FULL && g.rfforward && g.srctype != kPattern && g.srctype != kPattern3D) {
-                        deposit(g, sh.repoff, idx1dold + (uint32_t)tshift * g.nvox + g.imoff, weight_im);
+                        deposit_voxel_im(g, sh, ox, oy, oz, idx1dold, tshift, weight_im);
                     }
                 }
 
@@ -2150,6 +2177,7 @@ __global__ void __launch_bounds__(MCX_KBOUND(SAVEDET, REFLECT, SIMPLE, FULL)) MC
                 r /= g.box_ny;
                 uint32_t z = r % g.box_nz, gate = r / g.box_nz;
                 uint32_t vox = (g.box_z0 + z) * g.dimxy + (g.box_y0 + y) * g.dimx + (g.box_x0 + x);
+                gate = (gate < g.box_ng) ? gate : gate - g.box_ng + g.box_imgate;
                 atomicAdd(g.field + boff + vox + gate * g.nvox, val);
             }
         }

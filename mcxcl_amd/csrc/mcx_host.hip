@@ -481,7 +481,10 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
             budget = 0;   /* shared patterns deposit straight to their interleaved slots */
         }
 
-        const size_t cells = budget / elem;
+        /* RF forward: the box holds the imaginary half of its gates too (a pattern
+           source's RF field has none) */
+        const int boxim = isrfforward && cfg->srctype != MCX_SRC_PATTERN && cfg->srctype != MCX_SRC_PATTERN3D;
+        const size_t cells = budget / elem / (boxim ? 2 : 1);
         /* leading gates held: three when the budget buys three gates of 20^3, two for
            two of 16^3, else one */
         size_t ng = (cells >= 3u * 20u * 20u * 20u && maxgate >= 3) ? 3 :
@@ -534,7 +537,7 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
         ng = ng < ngmax ? ng : ngmax;
         ng = ng < maxgate ? ng : maxgate;
 
-        if (ng > 0 && shmem + 8 + ng * pergate * elem <= MCX_LDS_LIMIT) {
+        if (ng > 0 && shmem + 8 + ng * pergate * elem * (boxim ? 2 : 1) <= MCX_LDS_LIMIT) {
             g.box_x0 = org[0];
             g.box_y0 = org[1];
             g.box_z0 = org[2];
@@ -542,7 +545,8 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
             g.box_ny = edge[1];
             g.box_nz = edge[2];
             g.box_ng = (uint32_t)ng;
-            g.box_cells = (uint32_t)(ng * pergate);
+            g.box_cells = (uint32_t)(ng * pergate * (boxim ? 2 : 1));
+            g.box_imgate = boxim ? (uint32_t)(halflen / nvox) : 0u;
             /* the box follows the 16-byte aligned float tables */
             shmem += (size_t)g.box_cells * elem + 8;
         }
