@@ -57,6 +57,8 @@ struct MCXKParam {
        (box_imgate = imoff / nvox; 0 when the box holds no imaginary half) */
     uint32_t box_x0, box_y0, box_z0, box_nx, box_ny, box_nz, box_ng, box_cells;
     uint32_t box_imgate;
+    uint32_t stokesoff;           /* polarised runs: the Stokes vector's offset in the lane scratch */
+    uint32_t rfoff;               /* RF forward: offset of the weight at the last voxel exit (re, im) */
     uint32_t nrep;                /* global accumulator replicas (power of two) */
     uint32_t npattern;            /* patterns sharing each photon (a pattern source with srcnum > 1), else 1;
                                      their slots interleave after the voxel: (vox + gate*nvox)*npattern + k */

@@ -305,7 +305,6 @@ struct Photon {
        lower label | upper label << 8 | split << 16 | on the upper side << 17 */
     float snx, sny, snz, spd;
     uint32_t sv;
-    float4 stokes;               /* polarised runs: {I, Q, U, V}, I kept at 1 after each scattering */
 };
 
 struct Shared {
@@ -721,10 +720,24 @@ __device__ __forceinline__ uint32_t photon_index(const MCXKParam& g, int idx, fl
     return (uint32_t)(idx * (int)g.threadphoton + (idx < odd ? idx : odd) + (int)fw);
 }
 
+/* polarised runs keep the Stokes vector {I, Q, U, V} (I kept at 1 after each
+   scattering) in the lane's scratch at stokesoff: it is used only at launches,
+   scattering events and exits, so it holds no registers across the step loop */
+__device__ __forceinline__ float4 stokes_get(const MCXKParam& g, const float* pp) {
+    return make_float4(pp[g.stokesoff], pp[g.stokesoff + 1], pp[g.stokesoff + 2], pp[g.stokesoff + 3]);
+}
+
+__device__ __forceinline__ void stokes_put(const MCXKParam& g, float* pp, const float4& st) {
+    pp[g.stokesoff] = st.x;
+    pp[g.stokesoff + 1] = st.y;
+    pp[g.stokesoff + 2] = st.z;
+    pp[g.stokesoff + 3] = st.w;
+}
+
 /* trajectory record (src/mcx_core.cl:929-944): {photon id as bits, x, y, z, w,
    source id}; positions past maxjumpdebug are counted, not stored */
 __device__ __forceinline__ void save_traj(const MCXKParam& g, const float4& p, uint32_t id, float srcid,
-        const float4* stokes = nullptr) {
+        const float* stokes = nullptr) {
     const uint32_t pos = atomicAdd(g.jumpdebug, 1u);
 
     if (pos < g.maxjumpdebug) {
@@ -739,10 +752,10 @@ __device__ __forceinline__ void save_traj(const MCXKParam& g, const float4& p, u
         r[5] = (float)(int)srcid;
 
         if (stokes && g.maxpolmedia && g.istrajstokes) {
-            r[6] = stokes->x;
-            r[7] = stokes->y;
-            r[8] = stokes->z;
-            r[9] = stokes->w;
+            r[6] = stokes[0];
+            r[7] = stokes[1];
+            r[8] = stokes[2];
+            r[9] = stokes[3];
         }
     }
 }
@@ -908,10 +921,9 @@ __device__ void save_detected(const MCXKParam& g, const Shared& sh, const Photon
 
     if (FULL && ((g.savedetflag >> 7) & 1)) {
         /* the Stokes vector at exit (src/mcx_core.cl:915-920) */
-        rec[k++] = s.stokes.x;
-        rec[k++] = s.stokes.y;
-        rec[k++] = s.stokes.z;
-        rec[k++] = s.stokes.w;
+        for (uint32_t j = 0; j < 4; j++) {
+            rec[k++] = sh.pp[g.stokesoff + j];
+        }
     }
 }
 
@@ -958,7 +970,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
         pp[pd] += p.w;
 
         if (FULL && g.savetraj) {
-            save_traj(g, p, photon_index(g, idx, f.w), pp[g.w0offset - 1], &s.stokes);
+            save_traj(g, p, photon_index(g, idx, f.w), pp[g.w0offset - 1], pp + g.stokesoff);
         }
 
         if (!SIMPLE && SAVEDET && g.issaveref > 1 && s.mediaid == 0 && s.idx1d != K_OUT_MIN &&
@@ -1101,7 +1113,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
         }
 
         if (FULL && g.maxpolmedia) {
-            s.stokes = g.s0;    /* src/mcx_core.cl:1633-1638 */
+            stokes_put(g, pp, g.s0);    /* src/mcx_core.cl:1633-1638 */
         }
 
         if (!SIMPLE) {
@@ -1457,7 +1469,7 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
              getprop<FULL>(g, sh, s.mediaid, s.idx1d);
 
     if (FULL && g.savetraj) {
-        save_traj(g, p, traj_index(g, idx, f.w), ppl[3], &s.stokes);   /* the source-id slot */
+        save_traj(g, p, traj_index(g, idx, f.w), ppl[3], pp + g.stokesoff);   /* the source-id slot */
     }
 
     ppl[1] += p.w;
@@ -1508,10 +1520,11 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
     DepCache dcache;
     dcache.a0 = dcache.a1 = 0xFFFFFFFFu;
     dcache.v0 = dcache.v1 = 0.0;
-    /* RF forward: the complex photon weight and its value at the last voxel exit */
+    /* RF forward: the complex photon weight; its value at the last voxel exit is
+       read only at exits and sits in the lane scratch at rfoff (real, imaginary) */
     struct {
-        float wre, wim, w0re, w0im;
-    } rf = {0.f, 0.f, 0.f, 0.f};
+        float wre, wim;
+    } rf = {0.f, 0.f};
     /* split voxels: test the plane on this step (cleared once the photon crossed
        it, set by a scattering event, a voxel change or some relaunches), and
        whether the step ended on it (src/mcx_core.cl:2342-2343) */
@@ -1534,8 +1547,8 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
             keepold = false;
 
             if (FULL && g.rfforward) {
-                rf.wre = rf.w0re = s.p.w;
-                rf.wim = rf.w0im = 0.f;
+                rf.wre = sh.pp[g.rfoff] = s.p.w;
+                rf.wim = sh.pp[g.rfoff + 1] = 0.f;
             }
         }
 
@@ -1555,6 +1568,7 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                        medium's table, the forward entry as the bound */
                     const uint32_t lab = s.mediaid & K_MED_MASK;
                     const uint32_t ipol = K_NANGLES * (lab ? lab - 1 : 0u);   /* scattering needs mus > 0: label >= 1 */
+                    const float4 st = stokes_get(g, sh.pp);
                     float ii, sin2phi, cos2phi;
 #if MCX_PARITY
                     float i0;
@@ -1564,9 +1578,9 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                         tmp0 = K_TWO_PI * rng.next();
                         ksincos(2.f * tmp0, &sin2phi, &cos2phi);
                         const float4 m0 = g.smatrix[ipol];
-                        i0 = m0.x * s.stokes.x + m0.y * (s.stokes.y * cos2phi + s.stokes.z * sin2phi);
+                        i0 = m0.x * st.x + m0.y * (st.y * cos2phi + st.z * sin2phi);
                         const float4 mt = g.smatrix[ipol + (uint32_t)(theta * (float)K_NANGLES * (K_R_PI - K_EPS))];
-                        ii = mt.x * s.stokes.x + mt.y * (s.stokes.y * cos2phi + s.stokes.z * sin2phi);
+                        ii = mt.x * st.x + mt.y * (st.y * cos2phi + st.z * sin2phi);
                     } while (rng.next() * i0 >= ii);
 
                     ksincos(theta, &stheta, &ctheta);
@@ -1595,13 +1609,13 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                     theta = kacos(ctheta);
                     stheta = ksin_acos(ctheta);
                     const float4 mt = g.smatrix[ipol + lo];
-                    const float bound = mt.x * s.stokes.x +
-                                        fabsf(mt.y) * ksqrt(s.stokes.y * s.stokes.y + s.stokes.z * s.stokes.z);
+                    const float bound = mt.x * st.x +
+                                        fabsf(mt.y) * ksqrt(st.y * st.y + st.z * st.z);
 
                     do {
                         tmp0 = K_TWO_PI * rng.next();
                         ksincos(2.f * tmp0, &sin2phi, &cos2phi);
-                        ii = mt.x * s.stokes.x + mt.y * (s.stokes.y * cos2phi + s.stokes.z * sin2phi);
+                        ii = mt.x * st.x + mt.y * (st.y * cos2phi + st.z * sin2phi);
                     } while (rng.next() * bound >= ii);
 #endif
                     ksincos(tmp0, &sphi, &cphi);
@@ -1681,7 +1695,9 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                 if (FULL && g.maxpolmedia) {
                     /* :2550-2565; in 2D the reference samples by HG with the medium's g and
                        passes the sampled cosine (tmp0) where the azimuth goes */
-                    update_stokes(g, s.stokes, theta, tmp0, olddir, v, s.mediaid);
+                    float4 st = stokes_get(g, sh.pp);
+                    update_stokes(g, st, theta, tmp0, olddir, v, s.mediaid);
+                    stokes_put(g, sh.pp, st);
                 }
 
                 if (!SIMPLE && (g.outputtype == kWP || g.outputtype == kDCS ||
@@ -1720,7 +1736,7 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                 }
 
                 if (FULL && g.savetraj) {
-                    save_traj(g, p, traj_index(g, idx, f.w), sh.pp[g.w0offset - 1], &s.stokes);
+                    save_traj(g, p, traj_index(g, idx, f.w), sh.pp[g.w0offset - 1], sh.pp + g.stokesoff);
                 }
             }
 
@@ -1828,11 +1844,12 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
 
                 if (FULL && g.rfforward) {
                     /* RF forward (src/mcx_core.cl:2833-2839): (w0 - w) / (mua + i omega n / c0) */
-                    const float dre = rf.w0re - rf.wre, dim = rf.w0im - rf.wim;
+                    const float w0re = sh.pp[g.rfoff], w0im = sh.pp[g.rfoff + 1];
+                    const float dre = w0re - rf.wre, dim = w0im - rf.wim;
                     const float aim = g.omega * s.prop.w * g.oneoverc0;
                     const float amag2 = s.prop.x * s.prop.x + aim * aim;
-                    weight = (amag2 < K_EPS) ? (rf.w0re * pathlen) : kdiv(dre * s.prop.x + dim * aim, amag2);
-                    weight_im = (amag2 < K_EPS) ? (rf.w0im * pathlen) : kdiv(dim * s.prop.x - dre * aim, amag2);
+                    weight = (amag2 < K_EPS) ? (w0re * pathlen) : kdiv(dre * s.prop.x + dim * aim, amag2);
+                    weight_im = (amag2 < K_EPS) ? (w0im * pathlen) : kdiv(dim * s.prop.x - dre * aim, amag2);
                 } else if (g.outputtype == kEnergy) {
                     weight = s.w0 - p.w;
                 } else if (g.outputtype == kFluence || g.outputtype == kFlux) {
@@ -1894,8 +1911,8 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
             s.w0 = p.w;
 
             if (FULL && g.rfforward) {
-                rf.w0re = rf.wre;
-                rf.w0im = rf.wim;
+                sh.pp[g.rfoff] = rf.wre;
+                sh.pp[g.rfoff + 1] = rf.wim;
             }
 
             pathlen = 0.f;
@@ -1965,8 +1982,8 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                 if (FULL && g.rfforward) {
                     rf.wre *= K_ROULETTE;
                     rf.wim *= K_ROULETTE;
-                    rf.w0re *= K_ROULETTE;
-                    rf.w0im *= K_ROULETTE;
+                    sh.pp[g.rfoff] *= K_ROULETTE;
+                    sh.pp[g.rfoff + 1] *= K_ROULETTE;
                 }
             } else {
                 launchflag = mediaidold & K_DET_MASK;

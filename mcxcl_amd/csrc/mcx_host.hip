@@ -389,6 +389,12 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     /* lane scratch: record columns, 2 tallies, w0, source id, one weight per
        shared pattern (srcnum), the RF replay phase (cos, sin) */
     g.pplen = g.w0offset + g.srcnum + (isrfreplay ? 2u : 0u);
+    /* polarised runs: the Stokes vector after them */
+    g.stokesoff = g.pplen;
+    g.pplen += cfg->polmedianum ? 4u : 0u;
+    /* RF forward: the complex weight at the last voxel exit */
+    g.rfoff = g.pplen;
+    g.pplen += isrfforward ? 2u : 0u;
     g.rfforward = isrfforward ? 1u : 0u;
     memcpy(g.bc, cfg->bc, 12);
 
