@@ -301,9 +301,12 @@ struct Photon {
     float w0, Lmove;
     uint64_t seed0, seed1;       /* RNG state at launch (for seed saving) */
     /* split voxel (svmc, src/mcx_core.cl:594-598): the interface plane n.x = pd of
-       the current voxel, its normal pointing away from the photon's side, and
-       lower label | upper label << 8 | split << 16 | on the upper side << 17 */
-    float snx, sny, snz, spd;
+       the current voxel, its normal pointing away from the photon's side, as
+       {nx, ny, nz, pd} in the lane's scratch at svoff (sn points there in svmc
+       runs: read on each step, but only by those runs, so it holds no registers
+       in the others), and lower label | upper label << 8 | split << 16 | on the
+       upper side << 17 */
+    float* sn;
     uint32_t sv;
 };
 
@@ -500,10 +503,10 @@ __device__ __forceinline__ float4 svmc_prop(const MCXKParam& g, const float4* t,
         }
 
         sv |= 1u << 16;
-        s.snx = nx;
-        s.sny = ny;
-        s.snz = nz;
-        s.spd = pd;
+        s.sn[0] = nx;
+        s.sn[1] = ny;
+        s.sn[2] = nz;
+        s.sn[3] = pd;
     } else {
         prop = t[c7];
     }
@@ -516,13 +519,13 @@ __device__ __forceinline__ float4 svmc_prop(const MCXKParam& g, const float4* t,
    plane towards the other side, shorten it to the plane and recompute the
    scattering length it uses; returns whether it does */
 __device__ __forceinline__ bool svmc_ray_plane(const Photon& s, const float4& v, float gfac, float* len, float* slen) {
-    const float vdotn = v.x * s.snx + v.y * s.sny + v.z * s.snz;
+    const float vdotn = v.x * s.sn[0] + v.y * s.sn[1] + v.z * s.sn[2];
 
     if (vdotn <= 0.f) {
         return false;
     }
 
-    const float d0 = s.p.x * s.snx + s.p.y * s.sny + s.p.z * s.snz - s.spd;
+    const float d0 = s.p.x * s.sn[0] + s.p.y * s.sn[1] + s.p.z * s.sn[2] - s.sn[3];
     const float d1 = d0 + (*len) * vdotn;
 
     if (d0 * d1 > 0.f) {
@@ -540,7 +543,7 @@ __device__ __forceinline__ bool svmc_ray_plane(const Photon& s, const float4& v,
    refracts into label 0 (the photon leaves the medium) */
 __device__ __forceinline__ int svmc_reflect(float n1, float4& v, Photon& s, const float4* t, Rng& rng) {
     float cx = v.x, cy = v.y, cz = v.z;
-    const float icos = fabsf(cx * s.snx + cy * s.sny + cz * s.snz);
+    const float icos = fabsf(cx * s.sn[0] + cy * s.sn[1] + cz * s.sn[2]);
     const float n2 = K_SV_ISUPPER(s.sv) ? t[K_SV_UPPER(s.sv)].w : t[K_SV_LOWER(s.sv)].w;
     const float tmp0 = n1 * n1, tmp1 = n2 * n2;
     float tmp2 = 1.f - kdiv(tmp0, tmp1) * (1.f - icos * icos);
@@ -555,22 +558,22 @@ __device__ __forceinline__ int svmc_reflect(float n1, float4& v, Photon& s, cons
 
         if (rng.next() <= rtotal) {
             const float k = -2.f * icos;
-            cx += k * s.snx;
-            cy += k * s.sny;
-            cz += k * s.snz;
+            cx += k * s.sn[0];
+            cy += k * s.sn[1];
+            cz += k * s.sn[2];
             s.sv ^= 1u << 17;
         } else {
             const float k = -icos, r = kdiv(n1, n2);
-            cx += k * s.snx;
-            cy += k * s.sny;
-            cz += k * s.snz;
-            cx = tmp2 * s.snx + r * cx;
-            cy = tmp2 * s.sny + r * cy;
-            cz = tmp2 * s.snz + r * cz;
-            s.snx = -s.snx;
-            s.sny = -s.sny;
-            s.snz = -s.snz;
-            s.spd = -s.spd;
+            cx += k * s.sn[0];
+            cy += k * s.sn[1];
+            cz += k * s.sn[2];
+            cx = tmp2 * s.sn[0] + r * cx;
+            cy = tmp2 * s.sn[1] + r * cy;
+            cz = tmp2 * s.sn[2] + r * cz;
+            s.sn[0] = -s.sn[0];
+            s.sn[1] = -s.sn[1];
+            s.sn[2] = -s.sn[2];
+            s.sn[3] = -s.sn[3];
 
             if (K_SV_CURLABEL(s.sv) == 0) {
                 return 1;
@@ -580,9 +583,9 @@ __device__ __forceinline__ int svmc_reflect(float n1, float4& v, Photon& s, cons
         }
     } else {
         const float k = -2.f * icos;
-        cx += k * s.snx;
-        cy += k * s.sny;
-        cz += k * s.snz;
+        cx += k * s.sn[0];
+        cy += k * s.sn[1];
+        cz += k * s.sn[2];
         s.sv ^= 1u << 17;
     }
 
@@ -1530,7 +1533,11 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
        whether the step ended on it (src/mcx_core.cl:2342-2343) */
     bool testint = false, hitintf = false;
     s.sv = 0;
-    s.snx = s.sny = s.snz = s.spd = 0.f;
+
+    if (FULL && g.svmc) {
+        s.sn = sh.pp + g.svoff;
+        s.sn[0] = s.sn[1] = s.sn[2] = s.sn[3] = 0.f;
+    }
     /* move-only trajectories: the lane's run ends when a photon moves into the
        background without being relaunched */
     bool moveend = false;
@@ -1927,10 +1934,10 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                 s.prop = svmc_prop(g, sh.prop, s.mediaid, s.idx1d, p, s.fx, s.fy, s.fz, s);
                 testint = true;
             } else if (hitintf) {
-                s.snx = -s.snx;
-                s.sny = -s.sny;
-                s.snz = -s.snz;
-                s.spd = -s.spd;
+                s.sn[0] = -s.sn[0];
+                s.sn[1] = -s.sn[1];
+                s.sn[2] = -s.sn[2];
+                s.sn[3] = -s.sn[3];
                 s.sv ^= 1u << 17;
                 testint = false;
             }
@@ -2003,10 +2010,10 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
                     atface = false;
 
                     if (sh.prop[K_SV_LOWER(s.sv)].w != sh.prop[K_SV_UPPER(s.sv)].w) {
-                        s.snx = -s.snx;
-                        s.sny = -s.sny;
-                        s.snz = -s.snz;
-                        s.spd = -s.spd;
+                        s.sn[0] = -s.sn[0];
+                        s.sn[1] = -s.sn[1];
+                        s.sn[2] = -s.sn[2];
+                        s.sn[3] = -s.sn[3];
 
                         if (svmc_reflect(n1, v, s, sh.prop, rng)) {
                             launchflag = mediaidold & K_DET_MASK;

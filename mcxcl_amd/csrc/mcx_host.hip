@@ -395,6 +395,9 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     /* RF forward: the complex weight at the last voxel exit */
     g.rfoff = g.pplen;
     g.pplen += isrfforward ? 2u : 0u;
+    /* split voxels: the current voxel's plane {nx, ny, nz, pd} */
+    g.svoff = g.pplen;
+    g.pplen += (cfg->mediabyte == MEDIA_2LABEL_SPLIT) ? 4u : 0u;
     g.rfforward = isrfforward ? 1u : 0u;
     memcpy(g.bc, cfg->bc, 12);
 
