@@ -60,6 +60,7 @@ struct MCXKParam {
     uint32_t stokesoff;           /* polarised runs: the Stokes vector's offset in the lane scratch */
     uint32_t rfoff;               /* RF forward: offset of the weight at the last voxel exit (re, im) */
     uint32_t svoff;               /* split voxels: offset of the current voxel's plane {nx, ny, nz, pd} */
+    uint32_t seedoff;             /* saved seeds: offset of the RNG state at launch (4 words) */
     uint32_t nrep;                /* global accumulator replicas (power of two) */
     uint32_t npattern;            /* patterns sharing each photon (a pattern source with srcnum > 1), else 1;
                                      their slots interleave after the voxel: (vox + gate*nvox)*npattern + k */

@@ -299,7 +299,6 @@ struct Photon {
     int fx, fy, fz, fw;          /* voxel index (short semantics) and hit face */
     uint32_t idx1d, mediaid;
     float w0, Lmove;
-    uint64_t seed0, seed1;       /* RNG state at launch (for seed saving) */
     /* split voxel (svmc, src/mcx_core.cl:594-598): the interface plane n.x = pd of
        the current voxel, its normal pointing away from the photon's side, as
        {nx, ny, nz, pd} in the lane's scratch at svoff (sn points there in svmc
@@ -723,6 +722,17 @@ __device__ __forceinline__ uint32_t photon_index(const MCXKParam& g, int idx, fl
     return (uint32_t)(idx * (int)g.threadphoton + (idx < odd ? idx : odd) + (int)fw);
 }
 
+/* the RNG state at a photon's launch, kept for its detection record in the
+   lane's scratch at seedoff (s0 low, s0 high, s1 low, s1 high) when seeds are
+   saved: read only at a detection, so it holds no registers across the loop */
+__device__ __forceinline__ void seed_put(const MCXKParam& g, float* pp, const Rng& rng) {
+    uint32_t* sw = (uint32_t*)(pp + g.seedoff);
+    sw[0] = (uint32_t)rng.s0;
+    sw[1] = (uint32_t)(rng.s0 >> 32);
+    sw[2] = (uint32_t)rng.s1;
+    sw[3] = (uint32_t)(rng.s1 >> 32);
+}
+
 /* polarised runs keep the Stokes vector {I, Q, U, V} (I kept at 1 after each
    scattering) in the lane's scratch at stokesoff: it is used only at launches,
    scattering events and exits, so it holds no registers across the step loop */
@@ -883,8 +893,9 @@ __device__ void save_detected(const MCXKParam& g, const Shared& sh, const Photon
     }
 
     if (g.issaveseed) {
-        g.detseed[(size_t)slot * 2] = s.seed0;
-        g.detseed[(size_t)slot * 2 + 1] = s.seed1;
+        const uint32_t* sw = (const uint32_t*)(sh.pp + g.seedoff);
+        g.detseed[(size_t)slot * 2] = (uint64_t)sw[1] << 32 | sw[0];
+        g.detseed[(size_t)slot * 2 + 1] = (uint64_t)sw[3] << 32 | sw[2];
     }
 
     if (g.detweight) {
@@ -1041,8 +1052,10 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
     if (SIMPLE) {
         /* pencil beam from a non-void voxel, no focus, no extra sources, no replay:
            the general launch below reduces to this (one pass of its loop) */
-        s.seed0 = rng.s0;
-        s.seed1 = rng.s1;
+        if (SAVEDET && g.issaveseed) {
+            seed_put(g, pp, rng);
+        }
+
         p = g.srcpos;
         v = g.srcdir;
         f.x = 0.f;
@@ -1069,8 +1082,10 @@ __device__ int launch_photon(const MCXKParam& g, const Shared& sh, Photon& s, Rn
         rng.s1 = g.replayseed[(size_t)pid * 2 + 1];
     }
 
-    s.seed0 = rng.s0;
-    s.seed1 = rng.s1;
+    if (SAVEDET && g.issaveseed) {
+        seed_put(g, pp, rng);
+    }
+
     float4 lpos = g.srcpos, ldir = g.srcdir, lp1 = g.srcparam1, lp2 = g.srcparam2;
     int cursid = 1;
 
@@ -1510,7 +1525,6 @@ __device__ __forceinline__ bool transport_lane(const MCXKParam& g, const Shared&
     s.mediaid = __float_as_uint(g.srcparam2.w);
     s.w0 = 0.f;
     s.Lmove = 0.f;
-    s.seed0 = s.seed1 = 0;
 
     uint32_t isdet = 0, mediaidold = 0, idx1dold = 0, launchflag = 0;
     float pathlen = 0.f, n1 = 0.f;

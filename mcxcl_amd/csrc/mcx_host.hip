@@ -398,6 +398,9 @@ size_t mcx_kparam_setup(Config* cfg, MCXKParam* gp, size_t nthread, size_t lbloc
     /* split voxels: the current voxel's plane {nx, ny, nz, pd} */
     g.svoff = g.pplen;
     g.pplen += (cfg->mediabyte == MEDIA_2LABEL_SPLIT) ? 4u : 0u;
+    /* saved seeds: the RNG state at the photon's launch */
+    g.seedoff = g.pplen;
+    g.pplen += (cfg->issaveseed > 0) ? 4u : 0u;
     g.rfforward = isrfforward ? 1u : 0u;
     memcpy(g.bc, cfg->bc, 12);
 
